@@ -1,0 +1,209 @@
+"""Benchmark of the GPU gSpan miner on one MI355X.
+
+A step is one complete mining run (every level step, every pattern) of the
+configuration-1 workload -- 10,000 planted-pattern transactions (seed 2),
+absolute min support 500 (5 %) -- with the database already resident in HBM.
+The line reports extended embeddings/s (E_child: embedding rows of every
+reported pattern with >= 2 edges, per step), frequent patterns/s and the
+per-step mining seconds, plus the roofline of the dominant kernel (the
+extension + support kernel) and the CPU baseline (the oracle restatement of
+the reference's sequential gSpan, one core, same input).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|4] [--no-cpu] [--jobs N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def _workload(cfg: int):
+    from gpu_graph_mining_amd import workloads as wl
+    if cfg == 1:
+        db, ms = wl.config1()
+        return db, ms, {"workload": "config1: planted-pattern transactions", "n_graphs": 10000, "seed": 2,
+                        "min_support": ms, "min_support_frac": 0.05}
+    if cfg == 2:
+        db, ms = wl.config2()
+        return db, ms, {"workload": "config2: planted-pattern transactions", "n_graphs": 100000, "seed": 3,
+                        "min_support": ms, "min_support_frac": 0.01}
+    if cfg == 4:
+        db = wl.config4()
+        ms = wl.abs_support(0.05, 50000)
+        return db, ms, {"workload": "config4: molecule-like graphs (mining request)", "n_graphs": 50000, "seed": 4,
+                        "min_support": ms, "min_support_frac": 0.05}
+    raise SystemExit(f"unknown config {cfg}")
+
+
+def _traffic(kernel: str, workload: str):
+    """HBM bytes per launch of the kernel measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), from
+    the committed summary, or None."""
+    p = ROOT / "profiles" / "pmc_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return d.get(workload, {}).get(kernel)
+    except Exception:
+        return None
+
+
+def _prefix(arrays: dict, n: int) -> dict:
+    vb, eb = arrays["graph_vbegin"], arrays["graph_ebegin"]
+    return {"graph_vbegin": vb[:n + 1], "vlabels": arrays["vlabels"][:vb[n]], "graph_ebegin": eb[:n + 1],
+            "efrom": arrays["efrom"][:eb[n]], "eto": arrays["eto"][:eb[n]], "elabel": arrays["elabel"][:eb[n]]}
+
+
+def _cpu_model() -> str:
+    try:
+        for l in open("/proc/cpuinfo"):
+            if l.startswith("model name"):
+                return l.split(":", 1)[1].strip() + f" ({os.cpu_count()} logical cpus visible)"
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def _batch_section(n_jobs: int, out: dict) -> None:
+    """Config 3: n_jobs independent jobs, batched in one schedule vs one job
+    at a time on the same GPU (same library, same results)."""
+    import gpu_graph_mining_amd as gm
+    from gpu_graph_mining_amd import workloads as wl
+    jobs = wl.config3_jobs()[:n_jobs]
+    dbs = [j.database() for j in jobs]
+    mins = [j.minsup for j in jobs]
+    for d in dbs:
+        d.upload()
+    t0 = time.perf_counter()
+    seq = []
+    for d, m in zip(dbs, mins):
+        _, st = gm.mine(d, m, count_only=True)
+        seq.append(st["n_patterns"])
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    res, st = gm.mine_batch(dbs, mins)
+    t_bat = time.perf_counter() - t0
+    same = [len(r) for r in res] == seq
+    out["batch"] = {"jobs": n_jobs, "jobs_per_s_batched": n_jobs / t_bat, "jobs_per_s_sequential": n_jobs / t_seq,
+                    "speedup": t_seq / t_bat, "patterns": int(sum(seq)), "counts_identical": same}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--jobs", type=int, default=0, help="also run the first N config-3 jobs batched vs sequential")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single GPU; "
+              f"multi-GPU sharding by graph id is a design note only)", file=sys.stderr)
+        return 2
+
+    import gpu_graph_mining_amd as gm
+    if gm.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 3
+
+    db, minsup, config = _workload(args.config)
+    t0 = time.perf_counter()
+    db.upload()
+    upload_s = time.perf_counter() - t0
+    counts = db.counts()
+    config.update({"n_vertices": counts["n_vertices"], "n_edges": counts["n_edges"]})
+
+    # reference-checked first run (outside the timed region)
+    pats, st0 = gm.mine(db, minsup)
+    print(f"# first run: {st0['n_patterns']} patterns, E_child={st0['e_child']}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        gm.mine(db, minsup, count_only=True)
+    ext_s = 0.0
+    launches = 0
+    canon_s = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, st = gm.mine(db, minsup, count_only=True)
+        ext_s += st["seconds_ext_kernel"]
+        launches += st["ext_kernel_launches"]
+        canon_s += st["seconds_canon"]
+    elapsed = time.perf_counter() - t0
+    step_s = elapsed / args.steps
+    e_child = st["e_child"]
+    value = e_child / step_s
+
+    line = {
+        "metric": "extended embeddings/s",
+        "value": value,
+        "unit": "embeddings/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": config,
+        "patterns": st["n_patterns"],
+        "patterns_per_s": st["n_patterns"] / step_s,
+        "e_child": e_child,
+        "e_cand": st["e_cand"],
+        "mine_seconds": step_s,
+        "mine_seconds_with_upload": step_s + upload_s,
+        "levels": st["n_levels"],
+        "canon_seconds_per_step": canon_s / args.steps,
+        "ext_kernel_seconds_per_step": ext_s / args.steps,
+    }
+
+    roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+            "traffic": _traffic("k_support", config["workload"]), "kernel": "k_support (extension + support)",
+            "launches_per_step": launches / args.steps,
+            "avg_launch_ms": (ext_s / launches * 1e3) if launches else None}
+    if not args.no_cpu:
+        from oracle import py_oracle as po
+        arrays = db.arrays()
+        sample = "the whole workload"
+        if args.config == 2:  # bound the CPU leg to ~10-30 s: the first 10,000 transactions
+            n = 10000
+            arrays = _prefix(arrays, n)
+            sample = f"first {n} of 100000 transactions at the same 1 % support"
+        ms_cpu = minsup if args.config != 2 else max(1, minsup * 10000 // 100000)
+        opats, ost = po.mine_arrays(arrays, ms_cpu)
+        if args.config != 2:
+            got = sorted((po.normalize(c), s) for c, s, _ in pats)
+            want = sorted((c, s) for c, s, _ in opats)
+            line["parity"] = "identical" if (got == want and ost["e_child"] == e_child) else "MISMATCH"
+            # algorithmic bytes of the extension + support stage over the
+            # same patterns (formula in oracle/cpu_gspan.cpp)
+            b = ost["b_ext"] + ost["b_sup"]
+            per_step_ext = ext_s / args.steps
+            roof["algorithmic_bytes_per_step"] = b
+            roof["achieved"] = b / per_step_ext / 1e9
+            roof["frac"] = roof["achieved"] / HBM_PEAK_GBS
+        line["cpu_baseline"] = {"value": ost["e_child"] / ost["seconds"], "unit": "embeddings/s", "cores": 1,
+                                "kind": "port", "sample": sample, "seconds": ost["seconds"],
+                                "patterns": ost["n_patterns"], "cpu": _cpu_model()}
+    line["roofline"] = roof
+    if args.jobs:
+        _batch_section(args.jobs, line)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,84 @@
+"""Build the native library of the miner in-tree with hipcc (gfx950).
+
+Produces gpu_graph_mining_amd/_lib/libgspan_amd.so from csrc/.  Object files
+go to gpu_graph_mining_amd/_build/ (git-ignored).  Run as a module:
+
+    python -m gpu_graph_mining_amd.build
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB_DIR = PKG / "_lib"
+LIB = LIB_DIR / "libgspan_amd.so"
+ARCH = os.environ.get("GM_OFFLOAD_ARCH", "gfx950")
+
+# Packed-FP32 instructions are disabled for all device code.
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-parameter", f"-I{PKG.parent / 'include'}"]
+
+DEVICE_SOURCES = ["kernels.hip"]
+HOST_SOURCES = ["capi.cpp", "miner.cpp", "canonical.cpp", "host_db.cpp", "generators.cpp"]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if proc.returncode != 0:
+        raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
+
+
+def build(verbose: bool = False) -> Path:
+    hipcc = _hipcc()
+    BUILD.mkdir(exist_ok=True)
+    LIB_DIR.mkdir(exist_ok=True)
+    jobs = []
+    objs = []
+    for src in DEVICE_SOURCES:
+        obj = BUILD / (src + ".o")
+        jobs.append([hipcc, f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = BUILD / (src + ".o")
+        # host-only translation units; hipcc supplies the HIP runtime headers
+        jobs.append([hipcc, *COMMON, "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(obj)
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for cmd in jobs:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+        list(ex.map(_run, jobs))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> None:
+    """Compile the CPU oracle (test infrastructure) and, where the reference
+    sources are present, the reference build used to pin it."""
+    root = PKG.parent
+    _run(["make", "-s", "-C", str(root), "-f", "oracle/Makefile"])
+    if Path("/root/reference/src/gspan/gspan.cpp").exists():
+        _run(["make", "-s", "-C", str(root), "-f", "oracle/Makefile.ref"])
+
+
+if __name__ == "__main__":
+    p = build(verbose="-v" in sys.argv)
+    build_oracle()
+    print(p)

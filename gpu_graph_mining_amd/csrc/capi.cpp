@@ -1,0 +1,329 @@
+// capi.cpp -- the extern "C" boundary (include/gspan_amd.h).
+//
+// Every entry point converts C++ exceptions into GM_E_* codes and keeps the
+// message in a thread-local buffer for gm_last_error(); nothing throws
+// across the boundary (the reference throws std::runtime_error and aborts on
+// launch failures, src/cuda_pp/cuda_tools.hpp:10-18, kernel_execution.hpp:63-68).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "../../include/gspan_amd.h"
+#include "gm_internal.hpp"
+#include "miner.hpp"
+
+struct gm_db {
+  gm::HostDB host;
+  std::unique_ptr<gm::DeviceDB> dev;
+  hipStream_t stream = nullptr;
+  ~gm_db() {
+    dev.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+int guard(F &&f) {
+  try {
+    g_last_error.clear();
+    f();
+    return GM_OK;
+  } catch (const gm::Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return GM_E_OOM;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return GM_E_ARG;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return GM_E_ARG;
+  }
+}
+
+void ensure_stream(gm_db *db) {
+  if (!db->stream) gm::hip_check(hipStreamCreateWithFlags(&db->stream, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+void ensure_device(gm_db *db) {
+  if (!db->dev) {
+    std::unique_ptr<gm::DeviceDB> d(new gm::DeviceDB());
+    std::vector<const gm::HostDB *> one{&db->host};
+    gm::PackedDB pk;
+    gm::pack_db(one, *d, pk);  // validates before any HIP call
+    ensure_stream(db);
+    gm::upload_db(pk, *d, db->stream);
+    db->dev = std::move(d);
+  }
+}
+
+gm_db *new_db(gm::HostDB &&h) {
+  gm_db *db = new gm_db();
+  db->host = std::move(h);
+  return db;
+}
+
+void flatten(const std::vector<gm::DFS> &code, int style, std::vector<int32_t> &out) {
+  out.resize(code.size() * 5);
+  for (size_t i = 0; i < code.size(); ++i) {
+    const gm::DFS &d = code[i];
+    out[5 * i + 0] = d.from;
+    out[5 * i + 1] = d.to;
+    out[5 * i + 2] = (style == GM_LABELS_WILDCARD && i > 0) ? -1 : d.fromlabel;
+    out[5 * i + 3] = d.elabel;
+    out[5 * i + 4] = (style == GM_LABELS_WILDCARD && d.from > d.to) ? -1 : d.tolabel;
+  }
+}
+
+gm::MineOptions options(const gm_cfg *cfg) {
+  gm::MineOptions o;
+  if (cfg) {
+    o.max_edges = cfg->max_edges;
+    o.canon_threads = cfg->canon_threads;
+    o.collect_proj = cfg->collect_proj != 0;
+  }
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gm_version(void) { return "gspan_amd 0.1.0 (gfx950)"; }
+
+const char *gm_last_error(void) { return g_last_error.c_str(); }
+
+void gm_cfg_init(gm_cfg *cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->label_style = GM_LABELS_FULL;
+}
+
+int gm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int gm_db_create(int32_t n_graphs, const int32_t *graph_vbegin, const int32_t *vlabels, const int32_t *graph_ebegin,
+                 const int32_t *efrom, const int32_t *eto, const int32_t *elabel, gm_db **out) {
+  return guard([&]() {
+    if (!out || n_graphs < 0 || !graph_vbegin || !graph_ebegin) throw gm::Error(GM_E_ARG, "null argument");
+    *out = nullptr;
+    gm::HostDB h;
+    h.n_graphs = n_graphs;
+    h.vbegin.assign(graph_vbegin, graph_vbegin + n_graphs + 1);
+    h.ebegin.assign(graph_ebegin, graph_ebegin + n_graphs + 1);
+    const int64_t nv = h.vbegin.back(), ne = h.ebegin.back();
+    if (nv < 0 || ne < 0 || (nv > 0 && !vlabels) || (ne > 0 && (!efrom || !eto || !elabel)))
+      throw gm::Error(GM_E_ARG, "null array");
+    h.vlabel.assign(vlabels, vlabels + nv);
+    h.efrom.assign(efrom, efrom + ne);
+    h.eto.assign(eto, eto + ne);
+    h.elabel.assign(elabel, elabel + ne);
+    h.check_consistent();
+    *out = new_db(std::move(h));
+  });
+}
+
+int gm_db_load(const char *path, const char *filetype, gm_db **out) {
+  return guard([&]() {
+    if (!path || !filetype || !out) throw gm::Error(GM_E_ARG, "null argument");
+    *out = nullptr;
+    std::string ft(filetype);
+    if (!ft.empty() && ft[0] == '-') ft = ft.substr(1);
+    gm::HostDB h;
+    if (ft == "txt")
+      gm::read_txt(path, false, h);
+    else if (ft == "fsg")
+      gm::read_txt(path, true, h);
+    else if (ft == "bin")
+      gm::read_bin(path, h);
+    else if (ft == "cudabin")
+      gm::read_cudabin(path, h);
+    else
+      throw gm::Error(GM_E_ARG, "unknown filetype");
+    h.check_consistent();
+    *out = new_db(std::move(h));
+  });
+}
+
+int gm_db_write_txt(const gm_db *db, const char *path) {
+  return guard([&]() {
+    if (!db || !path) throw gm::Error(GM_E_ARG, "null argument");
+    gm::write_txt(db->host, path);
+  });
+}
+
+int gm_db_counts(const gm_db *db, int64_t counts[6]) {
+  return guard([&]() {
+    if (!db || !counts) throw gm::Error(GM_E_ARG, "null argument");
+    const gm::HostDB &h = db->host;
+    std::vector<int32_t> v(h.vlabel), e(h.elabel);
+    std::sort(v.begin(), v.end());
+    std::sort(e.begin(), e.end());
+    counts[0] = h.n_graphs;
+    counts[1] = h.n_vertices();
+    counts[2] = h.n_edges();
+    counts[3] = h.max_vertices();
+    counts[4] = std::unique(v.begin(), v.end()) - v.begin();
+    counts[5] = std::unique(e.begin(), e.end()) - e.begin();
+  });
+}
+
+int gm_db_export(const gm_db *db, int32_t *graph_vbegin, int32_t *vlabels, int32_t *graph_ebegin, int32_t *efrom,
+                 int32_t *eto, int32_t *elabel) {
+  return guard([&]() {
+    if (!db) throw gm::Error(GM_E_ARG, "null argument");
+    const gm::HostDB &h = db->host;
+    if (graph_vbegin) std::copy(h.vbegin.begin(), h.vbegin.end(), graph_vbegin);
+    if (vlabels) std::copy(h.vlabel.begin(), h.vlabel.end(), vlabels);
+    if (graph_ebegin) std::copy(h.ebegin.begin(), h.ebegin.end(), graph_ebegin);
+    if (efrom) std::copy(h.efrom.begin(), h.efrom.end(), efrom);
+    if (eto) std::copy(h.eto.begin(), h.eto.end(), eto);
+    if (elabel) std::copy(h.elabel.begin(), h.elabel.end(), elabel);
+  });
+}
+
+int gm_db_upload(gm_db *db) {
+  return guard([&]() {
+    if (!db) throw gm::Error(GM_E_ARG, "null argument");
+    ensure_device(db);
+  });
+}
+
+void gm_db_destroy(gm_db *db) { delete db; }
+
+int gm_db_subset(const gm_db *db, const int32_t *ids, int32_t n, gm_db **out) {
+  return guard([&]() {
+    if (!db || !out || n < 0 || (n > 0 && !ids)) throw gm::Error(GM_E_ARG, "null argument");
+    *out = nullptr;
+    const gm::HostDB &h = db->host;
+    gm::HostDB r;
+    std::vector<int32_t> lab, ef, et, el;
+    for (int32_t i = 0; i < n; ++i) {
+      int32_t g = ids[i];
+      if (g < 0 || g >= h.n_graphs) throw gm::Error(GM_E_ARG, "graph id out of range");
+      lab.assign(h.vlabel.begin() + h.vbegin[g], h.vlabel.begin() + h.vbegin[g + 1]);
+      ef.assign(h.efrom.begin() + h.ebegin[g], h.efrom.begin() + h.ebegin[g + 1]);
+      et.assign(h.eto.begin() + h.ebegin[g], h.eto.begin() + h.ebegin[g + 1]);
+      el.assign(h.elabel.begin() + h.ebegin[g], h.elabel.begin() + h.ebegin[g + 1]);
+      r.add_graph(lab, ef, et, el);
+    }
+    *out = new_db(std::move(r));
+  });
+}
+
+int gm_mine(gm_db *db, int32_t abs_minsup, const gm_cfg *cfg, gm_pattern_cb cb, void *user, gm_stats *stats) {
+  return guard([&]() {
+    if (!db) throw gm::Error(GM_E_ARG, "null database");
+    if (abs_minsup < 1) throw gm::Error(GM_E_ARG, "absolute min support < 1");
+    ensure_device(db);
+    gm::MineOptions o = options(cfg);
+    o.minsup = {abs_minsup};
+    const int style = cfg ? cfg->label_style : GM_LABELS_FULL;
+    std::vector<int32_t> flat;
+    gm::ReportFn rep = [&](int32_t, const std::vector<gm::DFS> &code, uint32_t sup, uint64_t nproj) {
+      if (!cb) return true;
+      flatten(code, style, flat);
+      return cb(user, flat.data(), (int32_t)code.size(), sup, nproj) == 0;
+    };
+    if (!gm::mine(*db->dev, o, rep, stats, db->stream)) throw gm::Error(GM_E_CALLBACK, "callback stopped mining");
+  });
+}
+
+int gm_mine_batch(int32_t n_jobs, gm_db *const *dbs, const int32_t *abs_minsup, const gm_cfg *cfg,
+                  gm_job_pattern_cb cb, void *user, gm_stats *stats) {
+  return guard([&]() {
+    if (n_jobs < 1 || !dbs || !abs_minsup) throw gm::Error(GM_E_ARG, "null argument");
+    std::vector<const gm::HostDB *> hs;
+    for (int32_t j = 0; j < n_jobs; ++j) {
+      if (!dbs[j]) throw gm::Error(GM_E_ARG, "null database");
+      if (abs_minsup[j] < 1) throw gm::Error(GM_E_ARG, "absolute min support < 1");
+      hs.push_back(&dbs[j]->host);
+    }
+    gm::DeviceDB merged;
+    gm::PackedDB pk;
+    gm::pack_db(hs, merged, pk);
+    ensure_stream(dbs[0]);
+    gm::upload_db(pk, merged, dbs[0]->stream);
+    gm::MineOptions o = options(cfg);
+    o.minsup.assign(abs_minsup, abs_minsup + n_jobs);
+    const int style = cfg ? cfg->label_style : GM_LABELS_FULL;
+    std::vector<int32_t> flat;
+    gm::ReportFn rep = [&](int32_t job, const std::vector<gm::DFS> &code, uint32_t sup, uint64_t nproj) {
+      if (!cb) return true;
+      flatten(code, style, flat);
+      return cb(user, job, flat.data(), (int32_t)code.size(), sup, nproj) == 0;
+    };
+    if (!gm::mine(merged, o, rep, stats, dbs[0]->stream)) throw gm::Error(GM_E_CALLBACK, "callback stopped mining");
+  });
+}
+
+int gm_dfs_code_is_min(const int32_t *dfs5, int32_t n_edges) {
+  int result = 0;
+  int rc = guard([&]() {
+    if (!dfs5 || n_edges < 1) throw gm::Error(GM_E_ARG, "empty code");
+    std::vector<gm::DFS> c(n_edges);
+    std::vector<int32_t> vl(n_edges + 2, -1);
+    for (int32_t i = 0; i < n_edges; ++i) {
+      const int32_t *d = dfs5 + 5 * i;
+      c[i] = gm::DFS{d[0], d[1], d[2], d[3], d[4]};
+      int32_t m = std::max(d[0], d[1]);
+      if (d[0] < 0 || d[1] < 0 || m >= (int32_t)vl.size()) throw gm::Error(GM_E_ARG, "malformed code");
+      if (d[2] != -1) vl[d[0]] = d[2];
+      if (d[4] != -1) vl[d[1]] = d[4];
+    }
+    for (auto &d : c) {  // complete a wildcard code from the labelled tuples
+      d.fromlabel = vl[d.from];
+      d.tolabel = vl[d.to];
+    }
+    result = gm::dfs_code_is_min(c.data(), n_edges) ? 1 : 0;
+  });
+  return rc == GM_OK ? result : rc;
+}
+
+int gm_gen_random(uint64_t seed, int32_t n_graphs, int32_t vmin, int32_t vmax, double edge_factor, int32_t n_vlabels,
+                  int32_t n_elabels, gm_db **out) {
+  return guard([&]() {
+    if (!out) throw gm::Error(GM_E_ARG, "null argument");
+    gm::HostDB h;
+    gm::gen_random(seed, n_graphs, vmin, vmax, edge_factor, n_vlabels, n_elabels, h);
+    *out = new_db(std::move(h));
+  });
+}
+
+int gm_gen_planted(uint64_t seed, int32_t n_graphs, double mean_edges, int32_t n_seed_patterns,
+                   double mean_pattern_edges, int32_t n_vlabels, int32_t n_elabels, double merge_prob, gm_db **out) {
+  return guard([&]() {
+    if (!out) throw gm::Error(GM_E_ARG, "null argument");
+    gm::HostDB h;
+    gm::gen_planted(seed, n_graphs, mean_edges, n_seed_patterns, mean_pattern_edges, n_vlabels, n_elabels, merge_prob,
+                    h);
+    *out = new_db(std::move(h));
+  });
+}
+
+int gm_gen_molecule(uint64_t seed, int32_t n_graphs, double mean_vertices, int32_t max_degree, int32_t n_vlabels,
+                    const double *vlabel_probs, int32_t n_elabels, const double *elabel_probs, gm_db **out) {
+  return guard([&]() {
+    if (!out || !vlabel_probs || !elabel_probs || n_vlabels < 1 || n_elabels < 1)
+      throw gm::Error(GM_E_ARG, "null argument");
+    gm::HostDB h;
+    gm::gen_molecule(seed, n_graphs, mean_vertices, max_degree,
+                     std::vector<double>(vlabel_probs, vlabel_probs + n_vlabels),
+                     std::vector<double>(elabel_probs, elabel_probs + n_elabels), h);
+    *out = new_db(std::move(h));
+  });
+}
+
+}  // extern "C"

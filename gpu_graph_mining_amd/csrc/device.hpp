@@ -1,0 +1,58 @@
+// device.hpp -- device-side layouts shared by the HIP kernels and the host
+// driver of the miner.
+//
+// Database layout in HBM (one copy per gm_db):
+//   gvbase[g]      first global vertex of graph g (graphs are contiguous)
+//   vadj[v]        first adjacency entry of global vertex v; vadj[NV] = 2E
+//   vlab[v]        dense vertex label (uint8)
+//   adj[a]         packed adjacency entry: bits 0-7 neighbour's local id,
+//                  8-15 dense edge label, 16-23 neighbour's dense label
+//   gjob[g]        job index of graph g (0 unless several jobs are batched)
+// Local vertex ids fit a byte, so a graph has at most 256 vertices.  This is
+// the padded-CSR graph_database_cuda of the reference
+// (src/globals/cuda_graph_types.hpp:16-166) without the padding and with the
+// neighbour label folded into the entry, so one 4-byte read answers every
+// question the extension rules ask about a neighbour.
+//
+// Embedding store, one level at a time, graph-major:
+//   grow[g]..grow[g+1]   rows of graph g (int64 offsets)
+//   pid[r]               pattern of row r
+//   vt[r*S .. r*S+S)     the row's vertex tuple: byte i = local id of pattern
+//                        vertex i (DFS discovery order), S = stride in bytes
+// A row is one embedding of its pattern (|Proj(P)| rows per pattern), the
+// counterpart of the reference's column store (embedding_lists.hpp:12-85)
+// with whole tuples instead of back-linked columns.
+#pragma once
+#include <cstdint>
+
+namespace gm {
+
+constexpr int kMaxPatV = 32;  // vertices per pattern (tuple stride <= 32 bytes)
+constexpr int kMaxGraphV = 256;
+
+// Per-pattern facts the extension rules need (src/globals/graph_types.cpp:
+// 305-413 evaluated on the pattern side).  Rightmost-path edge i (i = 0 is
+// the last forward edge, as DFSCode::buildRMPath orders them) is packed in
+// rm[i]: bits 0-7 its from vertex, 8-15 its edge label, 16-23 the label of
+// its to vertex.
+struct alignas(16) PatMeta {
+  int32_t job;
+  uint8_t nv;        // vertex count; the rightmost vertex is nv-1
+  uint8_t nrm;       // rightmost-path edges
+  uint8_t minlabel;  // dense label of vertex 0
+  uint8_t maxlabel;  // dense label of the rightmost vertex
+  uint32_t bwd_mask; // bit t: the pattern already has edge (rightmost, t)
+  uint32_t pad;
+  uint32_t rm[kMaxPatV];
+};
+
+// Extension key of one pattern at a level whose tuple stride is S:
+//   backward to vertex t with edge label e   : t*NEL + e
+//   forward from vertex f, edge e, label l   : S*NEL + (f*NEL + e)*NVL + l
+// K = S*NEL*(NVL+1) keys per pattern; (pattern, key) -> p*K + key.
+struct KeySpace {
+  int32_t S, NEL, NVL;
+  int64_t K() const { return (int64_t)S * NEL * (NVL + 1); }
+};
+
+}  // namespace gm

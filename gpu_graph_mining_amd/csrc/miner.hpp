@@ -1,0 +1,71 @@
+// miner.hpp -- device database and the level-synchronous mining driver.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "../../include/gspan_amd.h"
+#include "device.hpp"
+#include "gm_internal.hpp"
+
+namespace gm {
+
+void hip_check(hipError_t e, const char *what);
+
+// Growable device allocation (grows only; freed with the owner).
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release();
+  void *ensure(size_t bytes);
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+// The packed database of device.hpp on the GPU, for one or more jobs.
+struct DeviceDB {
+  int32_t n_graphs = 0, n_jobs = 1;
+  int64_t n_vertices = 0, n_adj = 0;
+  int32_t NVL = 0, NEL = 0;
+  std::vector<int32_t> vl_values, el_values;  // dense label -> original
+  std::vector<int32_t> job_first_graph;       // n_jobs + 1
+  DevBuf gvbase, vadj, adj, vlab, gjob;
+  size_t bytes = 0;
+};
+
+// Host image of the device layout (device.hpp).
+struct PackedDB {
+  std::vector<int32_t> gvbase, vadj, gjob;
+  std::vector<uint32_t> adj;
+  std::vector<uint8_t> vlab;
+};
+
+// Validate the databases (one job each) for the device path and pack them:
+// dense labels, packed adjacency.  Throws Error(GM_E_UNSUPPORTED) for graphs
+// over 256 vertices, self-loops, parallel edges or more than 255 labels.
+void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk);
+// Copy a packed database to the GPU.
+void upload_db(const PackedDB &pk, DeviceDB &out, hipStream_t s);
+
+// Reported pattern: job, fully labelled code in original labels, support,
+// |Proj| (0 unless collected).
+typedef std::function<bool(int32_t job, const std::vector<DFS> &code, uint32_t sup, uint64_t n_proj)> ReportFn;
+
+struct MineOptions {
+  std::vector<int32_t> minsup;  // per job
+  int32_t max_edges = 0;
+  int32_t canon_threads = 0;
+  bool collect_proj = false;
+};
+
+// Runs the level-synchronous miner; returns false when the report callback
+// asked to stop.
+bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats *stats, hipStream_t s);
+
+}  // namespace gm

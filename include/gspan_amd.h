@@ -1,0 +1,204 @@
+/*
+ * gspan_amd.h -- C ABI of the MI355X-native gSpan miner.
+ *
+ * One shared library (gpu_graph_mining_amd/_lib/libgspan_amd.so) exports
+ * these symbols.  Plain C types only; no exceptions cross the boundary.
+ * Every entry point returns GM_OK (0) or a negative error class; the text of
+ * the last error on the calling thread is available from gm_last_error().
+ *
+ * The reference's drop-in boundary is the C++ class gspan_cuda
+ * (src/gspan_cuda/gspan_cuda.hpp:53-293) plus the graph_output sink
+ * (src/globals/graph_output.hpp:9-52) and the dbio readers
+ * (src/globals/dbio.hpp:8-37).  Each declaration below names the reference
+ * interface it replaces.
+ *
+ * Threading: a gm_db handle is thread-compatible (no concurrent calls on one
+ * handle without external locking).  Pattern callbacks run synchronously on
+ * the calling thread, as graph_output::output_graph does in the reference
+ * (src/gspan_cuda/gspan_cuda_no_sort.cpp:107, 416).
+ */
+#ifndef GSPAN_AMD_H
+#define GSPAN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error classes ---------------------------------------------------- */
+#define GM_OK 0
+#define GM_E_ARG (-1)       /* bad argument / malformed input            */
+#define GM_E_IO (-2)        /* file could not be opened / read / written */
+#define GM_E_OOM (-3)       /* host or device allocation failed          */
+#define GM_E_DEVICE (-4)    /* HIP runtime / kernel launch failure       */
+#define GM_E_UNSUPPORTED (-5) /* input outside the supported envelope    */
+#define GM_E_CALLBACK (-6)  /* the pattern callback asked to stop        */
+
+/* ---- label style of reported DFS codes -------------------------------- */
+/* GM_LABELS_FULL: every DFS tuple fully labelled, as gspan_cuda reports
+ *   (src/gspan_cuda/gspan_cuda_no_sort.cpp:409-416).
+ * GM_LABELS_WILDCARD: fromlabel = -1 on every edge after the first and
+ *   tolabel = -1 on backward edges, as the sequential gSpan reports
+ *   (src/gspan/gspan.cpp:253, 266). */
+#define GM_LABELS_FULL 0
+#define GM_LABELS_WILDCARD 1
+
+typedef struct gm_db gm_db;
+
+/* Mining options.  Zero-initialise and set what you need; gm_cfg_init
+ * fills the defaults.  Replaces cudapp::cuda_configurator
+ * (src/cuda_pp/cuda_configurator.hpp) as the tuning surface. */
+typedef struct gm_cfg {
+  int32_t label_style;      /* GM_LABELS_FULL (default) or GM_LABELS_WILDCARD */
+  int32_t max_edges;        /* stop growing patterns beyond this many edges; 0 = unbounded */
+  int32_t canon_threads;    /* host threads for the canonical check; 0 = auto */
+  int32_t block_threads;    /* threads per workgroup of the level kernels; 0 = auto (256) */
+  int64_t mem_budget_bytes; /* device bytes the embedding store may use; 0 = auto */
+  int32_t collect_proj;     /* 1 = also report |Proj(P)| per pattern (extra pass) */
+  int32_t reserved[7];
+} gm_cfg;
+
+/* Per-run statistics (filled when a non-NULL pointer is passed). */
+typedef struct gm_stats {
+  uint64_t n_patterns;     /* reported frequent, minimal DFS codes        */
+  uint64_t e_child;        /* sum |Proj(P)| over reported P, >= 2 edges   */
+  uint64_t e_root;         /* sum |Proj(P)| over reported 1-edge P        */
+  uint64_t e_cand;         /* extension candidates enumerated             */
+  uint64_t n_candidates;   /* frequent (pattern, extension) keys checked  */
+  uint64_t n_levels;       /* level steps executed                        */
+  double seconds_total;    /* wall time of gm_mine (device upload excluded) */
+  double seconds_device;   /* time inside level kernels (HIP events)      */
+  double seconds_canon;    /* host time in the canonical check            */
+  double seconds_ext_kernel; /* time inside the extension/support kernel  */
+  uint64_t ext_kernel_launches;
+  uint64_t peak_device_bytes;
+  uint64_t reserved[6];
+} gm_stats;
+
+/* Pattern callback: n_edges rows of (from, to, fromlabel, elabel, tolabel),
+ * labels in the caller's original label values, plus the support (number
+ * of distinct database graphs containing the pattern) and, when
+ * cfg.collect_proj is set, |Proj(P)| (else 0).  Return 0 to continue,
+ * non-zero to stop mining (gm_mine then returns GM_E_CALLBACK).
+ * Replaces graph_output::output_graph(const DFSCode&, size_t support)
+ * (src/globals/graph_output.hpp:12). */
+typedef int (*gm_pattern_cb)(void *user, const int32_t *dfs5, int32_t n_edges, uint32_t support,
+                             uint64_t n_proj);
+
+/* Batch variant of the callback: also gets the job index. */
+typedef int (*gm_job_pattern_cb)(void *user, int32_t job, const int32_t *dfs5, int32_t n_edges,
+                                 uint32_t support, uint64_t n_proj);
+
+/* ---- library ---------------------------------------------------------- */
+const char *gm_version(void);
+/* Text of the last error raised on this thread ("" if none). */
+const char *gm_last_error(void);
+/* Fill *cfg with defaults. */
+void gm_cfg_init(gm_cfg *cfg);
+/* Number of visible HIP devices (0 when no GPU); never fails. */
+int gm_device_count(void);
+
+/* ---- graph databases -------------------------------------------------- */
+/* Create a database from per-graph vertex and edge lists (the in-memory
+ * form of types::graph_database_t, src/globals/types.hpp:31).  Graph g owns
+ * vertices [graph_vbegin[g], graph_vbegin[g+1]) with labels vlabels[], and
+ * edges [graph_ebegin[g], graph_ebegin[g+1]) given as local vertex ids
+ * efrom/eto plus elabel (undirected).  Replaces
+ * gspan_cuda::set_database(types::graph_database_t&, bool)
+ * (src/gspan_cuda/gspan_cuda.hpp:284; gspan_cuda.cpp:73-88).  The device
+ * copy is made lazily by the first gm_mine / gm_db_upload. */
+int gm_db_create(int32_t n_graphs, const int32_t *graph_vbegin, const int32_t *vlabels,
+                 const int32_t *graph_ebegin, const int32_t *efrom, const int32_t *eto,
+                 const int32_t *elabel, gm_db **out);
+
+/* Read a database file.  filetype: "txt", "fsg", "bin" or "cudabin", each
+ * optionally prefixed with '-', as dbio::ftype2str accepts
+ * (src/globals/dbio.cpp:33-41).  Replaces dbio::read_database /
+ * read_database_cudabin (src/globals/dbio.hpp:12, 21-22). */
+int gm_db_load(const char *path, const char *filetype, gm_db **out);
+
+/* Write the database as txt ("t # i" / "v id label" / "e from to label"),
+ * the format Graph::read parses (src/globals/graph_types.cpp:66-113). */
+int gm_db_write_txt(const gm_db *db, const char *path);
+
+/* Sizes of the database: counts[0..5] = n_graphs, n_vertices, n_edges,
+ * max vertices per graph, distinct vertex labels, distinct edge labels. */
+int gm_db_counts(const gm_db *db, int64_t counts[6]);
+
+/* Copy the database back out in gm_db_create's array form.  Arrays must
+ * hold n_graphs+1, n_vertices, n_graphs+1, n_edges (x3) entries. */
+int gm_db_export(const gm_db *db, int32_t *graph_vbegin, int32_t *vlabels, int32_t *graph_ebegin,
+                 int32_t *efrom, int32_t *eto, int32_t *elabel);
+
+/* Copy the device layout to the GPU now (otherwise done by gm_mine).
+ * Counterpart of graph_database_cuda::copy_to_device
+ * (src/globals/cuda_graph_types.cpp:130-153). */
+int gm_db_upload(gm_db *db);
+
+/* Release host and device memory.  Replaces
+ * gspan_cuda::delete_database_from_device (src/gspan_cuda/gspan_cuda.hpp:286)
+ * and the destructor (gspan_cuda.cpp:52-64). */
+void gm_db_destroy(gm_db *db);
+
+/* ---- mining ----------------------------------------------------------- */
+/* Mine every frequent, minimal DFS code with support >= abs_minsup on the
+ * GPU and report each through cb.  Replaces the sequence
+ * set_min_support / set_graph_output / run of gspan_cuda
+ * (src/gspan_cuda/gspan_cuda.hpp:289-292; gspan_cuda_no_sort.cpp:20-23,
+ * 68-136, 275-489).  abs_minsup < 1 is GM_E_ARG (the reference CLI exits
+ * with 3, src/gspan_cuda/common_main.cpp:56-70). */
+int gm_mine(gm_db *db, int32_t abs_minsup, const gm_cfg *cfg, gm_pattern_cb cb, void *user,
+            gm_stats *stats);
+
+/* Mine n_jobs independent databases in one batched schedule: the level
+ * steps of every job run in the same kernel launches.  Each job has its own
+ * database and absolute min support; results are identical to mining each
+ * job alone.  No reference counterpart (the reference mines one database
+ * per process); it batches many gm_mine calls. */
+int gm_mine_batch(int32_t n_jobs, gm_db *const *dbs, const int32_t *abs_minsup, const gm_cfg *cfg,
+                  gm_job_pattern_cb cb, void *user, gm_stats *stats);
+
+/* ---- canonical form ---------------------------------------------------- */
+/* Minimum-DFS-code check of one code (n rows of from,to,fromlabel,elabel,
+ * tolabel; either label style).  Returns 1 minimal, 0 not minimal, or a
+ * negative error.  Replaces types::DFSCode::dfs_code_is_min
+ * (src/globals/dfs_ismin.cpp:23-160). */
+int gm_dfs_code_is_min(const int32_t *dfs5, int32_t n_edges);
+
+/* ---- synthetic databases (seeded, deterministic) ---------------------- */
+/* Random connected graphs: n_graphs, vertex count uniform in [vmin,vmax],
+ * a random spanning tree (vertex i attaches to a uniform earlier vertex),
+ * then uniform extra edges without self-loops or multi-edges until
+ * edges = round(edge_factor * vertices); vertex labels uniform over
+ * n_vlabels, edge labels uniform over n_elabels. */
+int gm_gen_random(uint64_t seed, int32_t n_graphs, int32_t vmin, int32_t vmax, double edge_factor,
+                  int32_t n_vlabels, int32_t n_elabels, gm_db **out);
+
+/* Planted-pattern transactions: n_seed_patterns connected seed patterns with
+ * max(1, Poisson(mean_pattern_edges)) edges and Exp(1) weights; each
+ * transaction targets Poisson(mean_edges) edges, inserts weight-sampled
+ * patterns (each pattern vertex merges into an existing same-label vertex
+ * with probability merge_prob), then joins its components with random
+ * labelled edges. */
+int gm_gen_planted(uint64_t seed, int32_t n_graphs, double mean_edges, int32_t n_seed_patterns,
+                   double mean_pattern_edges, int32_t n_vlabels, int32_t n_elabels, double merge_prob,
+                   gm_db **out);
+
+/* Molecule-like graphs: vertex count Poisson(mean_vertices) (at least 2),
+ * degree <= max_degree, a random tree closed by 1-3 rings of size 5-6,
+ * vertex labels from vlabel_probs[n_vlabels], edge labels from
+ * elabel_probs[n_elabels]. */
+int gm_gen_molecule(uint64_t seed, int32_t n_graphs, double mean_vertices, int32_t max_degree,
+                    int32_t n_vlabels, const double *vlabel_probs, int32_t n_elabels,
+                    const double *elabel_probs, gm_db **out);
+
+/* Sub-database of the graphs listed in ids[0..n) (in that order). */
+int gm_db_subset(const gm_db *db, const int32_t *ids, int32_t n, gm_db **out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSPAN_AMD_H */
