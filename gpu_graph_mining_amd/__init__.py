@@ -29,7 +29,7 @@ from typing import Callable, Iterable, List, Optional, Sequence, Tuple
 import numpy as np
 
 _PKG = Path(__file__).resolve().parent
-LIB_PATH = _PKG / "_lib" / "libgspan_amd.so"
+LIB_PATH = Path(os.environ.get("GM_LIB_PATH", _PKG / "_lib" / "libgspan_amd.so"))
 
 GM_OK = 0
 GM_E_ARG = -1

@@ -43,30 +43,33 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, out: Path = LIB, build_dir: Path = BUILD, defines: tuple = ()) -> Path:
+    """Compile csrc/ into the shared library `out` (default _lib/)."""
     hipcc = _hipcc()
-    BUILD.mkdir(exist_ok=True)
-    LIB_DIR.mkdir(exist_ok=True)
+    build_dir.mkdir(parents=True, exist_ok=True)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    extra = [f"-D{d}" for d in defines]
     jobs = []
     objs = []
     for src in DEVICE_SOURCES:
-        obj = BUILD / (src + ".o")
-        jobs.append([hipcc, f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-c", str(CSRC / src), "-o", str(obj)])
+        obj = build_dir / (src + ".o")
+        jobs.append([hipcc, f"--offload-arch={ARCH}", *COMMON, *extra, *DEVICE_FLAGS, "-c", str(CSRC / src), "-o",
+                     str(obj)])
         objs.append(obj)
     for src in HOST_SOURCES:
-        obj = BUILD / (src + ".o")
+        obj = build_dir / (src + ".o")
         # host-only translation units; hipcc supplies the HIP runtime headers
-        jobs.append([hipcc, *COMMON, "-c", str(CSRC / src), "-o", str(obj)])
+        jobs.append([hipcc, *COMMON, *extra, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for cmd in jobs:
             if verbose:
                 print(" ".join(cmd), flush=True)
         list(ex.map(_run, jobs))
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = out.with_suffix(".so.tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def build_oracle() -> None:

@@ -26,7 +26,7 @@
 #include <vector>
 
 #include "gm_internal.hpp"
-#include "../../include/gspan_amd.h"
+#include "gspan_amd.h"
 
 namespace gm {
 namespace {

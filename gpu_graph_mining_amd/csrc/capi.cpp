@@ -11,7 +11,7 @@
 #include <new>
 #include <string>
 
-#include "../../include/gspan_amd.h"
+#include "gspan_amd.h"
 #include "gm_internal.hpp"
 #include "miner.hpp"
 

@@ -18,7 +18,7 @@
 #include <utility>
 
 #include "gm_internal.hpp"
-#include "../../include/gspan_amd.h"
+#include "gspan_amd.h"
 
 namespace gm {
 namespace {

@@ -22,7 +22,7 @@
 #include <sstream>
 
 #include "gm_internal.hpp"
-#include "../../include/gspan_amd.h"
+#include "gspan_amd.h"
 
 namespace gm {
 

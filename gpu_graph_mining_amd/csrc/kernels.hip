@@ -5,112 +5,103 @@
 // extension x") is answered by deduplicating (pattern, key) pairs inside the
 // workgroup (an LDS hash table) and adding 1 per distinct pair to a global
 // counter.  The same enumeration routine drives three kernels:
-//   SUPPORT  count distinct graphs per (pattern, key)         [kernel "A"]
-//   COUNT    count child rows per graph for selected keys      [kernel "B1"]
-//   WRITE    write child rows (pattern id + vertex tuple)      [kernel "B2"]
+//   k_support          distinct-graph support of every (pattern, key)
+//   k_children<false>  child rows per graph for the selected keys (count)
+//   k_children<true>   the child rows themselves (pattern id + tuple)
 // and, for the first level, the same three over 1-edge keys.
 //
 // Extension rules (pattern side precomputed in PatMeta, graph side read
 // from the packed adjacency):
-//   backward  rightmost vertex -> image of rightmost-path vertex rm_from[i],
-//             i >= 1, edge not yet in the pattern, and
-//             rm_el[i] < e || (rm_el[i] == e && rm_tl[i] <= label(rightmost))
+//   backward  rightmost vertex -> image of rightmost-path vertex f = from of
+//             rightmost-path edge i >= 1, edge not yet in the pattern, and
+//             el(i) < e || (el(i) == e && tolabel(i) <= label(rightmost))
 //             (src/globals/graph_types.cpp:389-413, gspan.cpp:220-225);
 //   pure fwd  rightmost vertex -> unvisited w, label(w) >= minlabel
 //             (graph_types.cpp:337-355, gspan.cpp:233-237);
-//   rm fwd    rm_from[i] -> unvisited w, label(w) >= minlabel and
-//             rm_el[i] < e || (rm_el[i] == e && rm_tl[i] <= label(w))
+//   rm fwd    from of rightmost-path edge i -> unvisited w,
+//             label(w) >= minlabel and
+//             el(i) < e || (el(i) == e && tolabel(i) <= label(w))
 //             (graph_types.cpp:305-323, gspan.cpp:240-246);
 //   root      u -> w with label(u) <= label(w) (graph_types.cpp:365-375).
 // Because the database graphs are simple, "edge not in the embedding" is a
 // pattern-level fact (bwd_mask) and "vertex not in the embedding" is a
-// bitmask over the graph's local ids built from the tuple.
+// 256-bit mask over the graph's local ids built from the tuple.
+//
+// Register discipline: a row's vertex tuple lives in a per-thread LDS slot
+// (byte reads), the visited mask in four 64-bit values selected
+// arithmetically, and the graph view is specialised for LDS or global
+// memory, so the kernels use no per-lane scratch and no flat accesses.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "device.hpp"
 #include "kernels.hpp"
-
-#include <algorithm>
 
 namespace gm {
 
 namespace {
 
-struct Tuple {
-  uint64_t w0, w1, w2, w3;
-};
+constexpr int kStageV = 257;   // vertex offsets staged in LDS (<= 256 vertices)
+constexpr int kStageA = 2048;  // adjacency entries staged in LDS
+constexpr int kSlotWords = 9;  // per-thread tuple slot: 36 bytes (odd word stride)
+constexpr uint32_t kEmpty32 = 0xFFFFFFFFu;
 
-__device__ __forceinline__ int tget(const Tuple &t, int i) {
-  uint64_t w = (i < 16) ? ((i < 8) ? t.w0 : t.w1) : ((i < 24) ? t.w2 : t.w3);
-  return (int)((w >> ((i & 7) * 8)) & 0xFF);
-}
-
-__device__ __forceinline__ void tset(Tuple &t, int i, int v) {
-  int q = i >> 3, sh = (i & 7) * 8;
-  uint64_t clr = ~(0xFFull << sh), val = (uint64_t)(v & 0xFF) << sh;
-  t.w0 = q == 0 ? ((t.w0 & clr) | val) : t.w0;
-  t.w1 = q == 1 ? ((t.w1 & clr) | val) : t.w1;
-  t.w2 = q == 2 ? ((t.w2 & clr) | val) : t.w2;
-  t.w3 = q == 3 ? ((t.w3 & clr) | val) : t.w3;
-}
-
-struct VMask {
-  uint64_t m0, m1, m2, m3;
-};
-
-__device__ __forceinline__ void vset(VMask &m, int v) {
-  uint64_t b = 1ull << (v & 63);
-  int q = v >> 6;
-  m.m0 |= q == 0 ? b : 0;
-  m.m1 |= q == 1 ? b : 0;
-  m.m2 |= q == 2 ? b : 0;
-  m.m3 |= q == 3 ? b : 0;
-}
-
-__device__ __forceinline__ bool vhas(const VMask &m, int v) {
-  uint64_t w = (v < 128) ? ((v < 64) ? m.m0 : m.m1) : ((v < 192) ? m.m2 : m.m3);
-  return (w >> (v & 63)) & 1;
-}
-
-__device__ __forceinline__ Tuple load_tuple(const uint8_t *vt, int64_t r, int S) {
-  const uint32_t *p = reinterpret_cast<const uint32_t *>(vt + r * S);
-  const int nw = S >> 2;
-  uint32_t w[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) w[i] = i < nw ? p[i] : 0u;
-  Tuple t;
-  t.w0 = w[0] | ((uint64_t)w[1] << 32);
-  t.w1 = w[2] | ((uint64_t)w[3] << 32);
-  t.w2 = w[4] | ((uint64_t)w[5] << 32);
-  t.w3 = w[6] | ((uint64_t)w[7] << 32);
-  return t;
-}
-
-__device__ __forceinline__ void store_tuple(uint8_t *vt, int64_t r, int S, const Tuple &t) {
-  uint32_t *p = reinterpret_cast<uint32_t *>(vt + r * S);
-  const int nw = S >> 2;
-  uint32_t w[8] = {(uint32_t)t.w0, (uint32_t)(t.w0 >> 32), (uint32_t)t.w1, (uint32_t)(t.w1 >> 32),
-                   (uint32_t)t.w2, (uint32_t)(t.w2 >> 32), (uint32_t)t.w3, (uint32_t)(t.w3 >> 32)};
-#pragma unroll
-  for (int i = 0; i < 8; ++i)
-    if (i < nw) p[i] = w[i];
-}
-
-// Graph adjacency view: vadj[] indexed by local vertex (absolute entry
-// offsets), entries A[a - aoff].  Either LDS-staged or straight from HBM.
+// Graph adjacency: entry offsets vadj[v] are absolute; entries A[a - aoff].
 struct GView {
   const int32_t *vadj;
   const uint32_t *A;
-  int64_t aoff;
+  int32_t aoff;
   __device__ __forceinline__ int32_t begin(int v) const { return vadj[v]; }
   __device__ __forceinline__ int32_t end(int v) const { return vadj[v + 1]; }
   __device__ __forceinline__ uint32_t entry(int32_t a) const { return A[a - aoff]; }
 };
 
-// Enumerate every valid extension of one embedding.  emit(key, w) gets the
-// extension key (see KeySpace) and the new vertex's local id (-1 backward).
+// 256-bit set of graph-local vertex ids held in four 64-bit values.
+struct VSet {
+  uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+  __device__ __forceinline__ void add(int v) {
+    const uint64_t b = 1ull << (v & 63);
+    const int q = v >> 6;
+    m0 |= q == 0 ? b : 0ull;
+    m1 |= q == 1 ? b : 0ull;
+    m2 |= q == 2 ? b : 0ull;
+    m3 |= q == 3 ? b : 0ull;
+  }
+  __device__ __forceinline__ bool has(int v) const {
+    const int q = v >> 6;
+    const uint64_t w = (m0 & (0ull - (uint64_t)(q == 0))) | (m1 & (0ull - (uint64_t)(q == 1))) |
+                       (m2 & (0ull - (uint64_t)(q == 2))) | (m3 & (0ull - (uint64_t)(q == 3)));
+    return (w >> (v & 63)) & 1ull;
+  }
+};
+
+// Copy row r's tuple (S bytes) from HBM into this thread's LDS slot.
+__device__ __forceinline__ void load_slot(uint32_t *slot, const uint8_t *vt, int64_t r, int S) {
+  const uint32_t *p = reinterpret_cast<const uint32_t *>(vt + r * S);
+  const int nw = S >> 2;
+  for (int i = 0; i < nw; ++i) slot[i] = p[i];
+}
+
+__device__ __forceinline__ int slot_get(const uint32_t *slot, int i) {
+  return reinterpret_cast<const uint8_t *>(slot)[i];
+}
+
+__device__ __forceinline__ void slot_set(uint32_t *slot, int i, int v) {
+  reinterpret_cast<uint8_t *>(slot)[i] = (uint8_t)v;
+}
+
+// Write the slot (CS bytes) as child row `pos`.
+__device__ __forceinline__ void store_slot(uint8_t *cvt, int64_t pos, int CS, const uint32_t *slot) {
+  uint32_t *p = reinterpret_cast<uint32_t *>(cvt + pos * CS);
+  const int nw = CS >> 2;
+  for (int i = 0; i < nw; ++i) p[i] = slot[i];
+}
+
+// Enumerate every valid extension of the embedding in `slot`.
+// emit(key, w): extension key (see KeySpace) and new vertex (-1 backward).
 template <class F>
-__device__ __forceinline__ void enumerate_row(const GView &G, const PatMeta *__restrict__ pm, const Tuple &t,
+__device__ __forceinline__ void enumerate_row(const GView &G, const PatMeta *__restrict__ pm, const uint32_t *slot,
                                               int S, int NEL, int NVL, F &&emit) {
   const int nv = pm->nv;
   const int maxtoc = nv - 1;
@@ -119,21 +110,21 @@ __device__ __forceinline__ void enumerate_row(const GView &G, const PatMeta *__r
   const int nrm = pm->nrm;
   const uint32_t bwd_mask = pm->bwd_mask;
   const int fwd_base = S * NEL;
-  VMask vis{0, 0, 0, 0};
-  for (int i = 0; i < nv; ++i) vset(vis, tget(t, i));
-  const int rv = tget(t, maxtoc);
+  VSet vis;
+  for (int i = 0; i < nv; ++i) vis.add(slot_get(slot, i));
+  const int rv = slot_get(slot, maxtoc);
 
   // backward and pure forward: one sweep over the rightmost vertex's edges
   for (int32_t a = G.begin(rv), ae = G.end(rv); a < ae; ++a) {
     const uint32_t e = G.entry(a);
     const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
-    if (!vhas(vis, w)) {
+    if (!vis.has(w)) {
       if (nl >= minlabel) emit(fwd_base + (maxtoc * NEL + el) * NVL + nl, w);
     } else {
       for (int i = 1; i < nrm; ++i) {
         const uint32_t rme = pm->rm[i];
         const int f = rme & 0xFF;
-        if (tget(t, f) != w) continue;
+        if (slot_get(slot, f) != w) continue;
         const int e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
         if (!((bwd_mask >> f) & 1u) && (e1 < el || (e1 == el && l1 <= maxlabel))) emit(f * NEL + el, -1);
         break;
@@ -144,12 +135,12 @@ __device__ __forceinline__ void enumerate_row(const GView &G, const PatMeta *__r
   for (int i = 0; i < nrm; ++i) {
     const uint32_t rme = pm->rm[i];
     const int f = rme & 0xFF;
-    const int u = tget(t, f);
+    const int u = slot_get(slot, f);
     const int e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
     for (int32_t a = G.begin(u), ae = G.end(u); a < ae; ++a) {
       const uint32_t e = G.entry(a);
       const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
-      if (vhas(vis, w) || nl < minlabel) continue;
+      if (vis.has(w) || nl < minlabel) continue;
       if (e1 < el || (e1 == el && l1 <= nl)) emit(fwd_base + (f * NEL + el) * NVL + nl, w);
     }
   }
@@ -165,34 +156,34 @@ __device__ __forceinline__ void enumerate_root(const GView &G, int u, int lu, in
   }
 }
 
-constexpr int kStageV = 257;
-constexpr int kStageA = 2048;
-constexpr uint64_t kEmpty = ~0ull;
-
-struct Shared {
+struct Stage {
   int32_t vadj[kStageV];
   uint32_t adj[kStageA];
 };
 
-// Stage graph g's adjacency in LDS when it fits; returns the view to use.
-__device__ __forceinline__ GView stage_graph(const LevelArgs &a, int g, Shared &sh, int &nv_out) {
+// Whether graph g's adjacency fits the LDS stage (uniform per workgroup).
+__device__ __forceinline__ bool graph_fits(const LevelArgs &a, int g) {
   const int32_t vb = a.gvbase[g], ve = a.gvbase[g + 1];
-  const int nv = ve - vb;
-  nv_out = nv;
+  return ve - vb + 1 <= kStageV && a.vadj[ve] - a.vadj[vb] <= kStageA;
+}
+
+template <bool LDS>
+__device__ __forceinline__ GView graph_view(const LevelArgs &a, int g, Stage &st) {
+  const int32_t vb = a.gvbase[g], ve = a.gvbase[g + 1];
   const int32_t ab = a.vadj[vb], ae = a.vadj[ve];
   GView G;
-  if (nv + 1 <= kStageV && ae - ab <= kStageA) {
-    for (int i = threadIdx.x; i <= nv; i += blockDim.x) sh.vadj[i] = a.vadj[vb + i];
-    for (int i = threadIdx.x; i < ae - ab; i += blockDim.x) sh.adj[i] = a.adj[ab + i];
-    G.vadj = sh.vadj;
-    G.A = sh.adj;
+  if (LDS) {
+    for (int i = threadIdx.x; i <= ve - vb; i += blockDim.x) st.vadj[i] = a.vadj[vb + i];
+    for (int i = threadIdx.x; i < ae - ab; i += blockDim.x) st.adj[i] = a.adj[ab + i];
+    __syncthreads();
+    G.vadj = st.vadj;
+    G.A = st.adj;
     G.aoff = ab;
   } else {
     G.vadj = a.vadj + vb;
     G.A = a.adj;
     G.aoff = 0;
   }
-  __syncthreads();
   return G;
 }
 
@@ -223,87 +214,76 @@ __device__ __forceinline__ int block_exclusive_scan(int x, int &total, int *s_wa
   return r;
 }
 
-__device__ __forceinline__ uint32_t hash_slot(uint64_t k, int bits) {
-  return (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> (64 - bits));
-}
+__device__ __forceinline__ uint32_t hash_slot32(uint32_t k) { return (k * 0x9E3779B1u) >> (32 - kTableBits); }
 
 // ---------------------------------------------------------------- SUPPORT
-template <bool ROOT>
-__global__ void __launch_bounds__(256) k_support(LevelArgs a) {
-  __shared__ Shared sh;
-  __shared__ unsigned long long table[kHashCap];
-  __shared__ int s_fill, s_over, s_top;
-  __shared__ uint32_t s_stack_mod[32], s_stack_res[32];
-  const int g = blockIdx.x;
-  int nv;
-  int64_t r0, r1;
-  if (ROOT) {
-    r0 = 0;
-    r1 = 0;  // rows are the graph's vertices
-  } else {
-    r0 = a.grow[g];
-    r1 = a.grow[g + 1];
-    if (r0 == r1) return;
-  }
-  GView G = stage_graph(a, g, sh, nv);
-  if (ROOT && nv == 0) return;
+// One pass per residue class of pattern ids (p mod `mod` == `res`; root
+// level: of the source vertex label): every candidate key
+// (p - p_lo) * K + key is inserted into an LDS hash table; when the table
+// passes 3/4 full the pass is abandoned and the class split in two (all keys
+// of one pattern stay in one class, so the per-graph dedup stays exact);
+// a completed pass adds 1 per distinct key to the global supports.
+template <bool ROOT, bool LDS>
+__device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, Stage &st,
+                                             uint32_t *table, uint32_t *slot, int *s_misc,
+                                             uint32_t *s_stack_mod, uint32_t *s_stack_res) {
+  const GView G = graph_view<LDS>(a, g, st);
   const int job = ROOT ? a.gjob[g] : 0;
   const int32_t vb = a.gvbase[g];
+  int &s_fill = s_misc[0], &s_over = s_misc[1], &s_top = s_misc[2];
   unsigned long long my_cand = 0;
   if (threadIdx.x == 0) {
     s_top = 1;
     s_stack_mod[0] = 1;
     s_stack_res[0] = 0;
   }
-  __syncthreads();
-  while (true) {
+  for (;;) {
     __syncthreads();
     if (s_top == 0) break;
     const uint32_t mod = s_stack_mod[s_top - 1], res = s_stack_res[s_top - 1];
-    for (int i = threadIdx.x; i < kHashCap; i += blockDim.x) table[i] = kEmpty;
+    for (int i = threadIdx.x; i < kTableCap; i += blockDim.x) table[i] = kEmpty32;
     if (threadIdx.x == 0) {
       s_fill = 0;
       s_over = 0;
     }
     __syncthreads();
     unsigned long long cand = 0;
-    auto insert = [&](uint64_t key) {
+    int over = 0;  // this thread saw the table pass its fill limit
+    auto insert = [&](uint32_t k) {
       ++cand;
-      if (s_over) return;
-      uint32_t h = hash_slot(key, kHashBits);
-      for (int probe = 0; probe < kHashCap; ++probe) {
-        unsigned long long old = atomicCAS(&table[h], kEmpty, (unsigned long long)key);
-        if (old == kEmpty) {
-          if (atomicAdd(&s_fill, 1) >= kHashCap * 3 / 4) s_over = 1;
+      uint32_t h = hash_slot32(k);
+      for (int probe = 0; probe < kTableCap; ++probe) {
+        const uint32_t old = atomicCAS(&table[h], kEmpty32, k);
+        if (old == kEmpty32) {
+          if (atomicAdd(&s_fill, 1) >= kTableFill) over = 1;
           return;
         }
-        if (old == key) return;
-        h = (h + 1) & (kHashCap - 1);
+        if (old == k) return;
+        h = (h + 1) & (kTableCap - 1);
       }
-      s_over = 1;
+      over = 1;
     };
-    if (ROOT) {
-      const uint64_t base = (uint64_t)job * a.K;
-      for (int u = threadIdx.x; u < nv; u += blockDim.x) {
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      if (ROOT) {
+        const int u = (int)r;
         const int lu = a.vlab[vb + u];
         if (mod > 1 && ((uint32_t)lu & (mod - 1)) != res) continue;
-        enumerate_root(G, u, lu, a.NEL, a.NVL, [&](int key, int) { insert(base + (uint64_t)key); });
-      }
-    } else {
-      for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const uint32_t base = (uint32_t)((int64_t)(job - a.p_lo) * a.K);
+        enumerate_root(G, u, lu, a.NEL, a.NVL, [&](int key, int) { insert(base + (uint32_t)key); });
+      } else {
         const int32_t p = a.pid[r];
+        if (p < a.p_lo || p >= a.p_hi) continue;
         if (mod > 1 && ((uint32_t)p & (mod - 1)) != res) continue;
-        const Tuple t = load_tuple(a.vt, r, a.S);
-        const uint64_t base = (uint64_t)p * (uint64_t)a.K;
-        enumerate_row(G, a.meta + p, t, a.S, a.NEL, a.NVL, [&](int key, int) { insert(base + (uint64_t)key); });
+        load_slot(slot, a.vt, r, a.S);
+        const uint32_t base = (uint32_t)((int64_t)(p - a.p_lo) * a.K);
+        enumerate_row(G, a.meta + p, slot, a.S, a.NEL, a.NVL, [&](int key, int) { insert(base + (uint32_t)key); });
       }
     }
+    if (over) atomicOr(&s_over, 1);
     __syncthreads();
-    const bool over = s_over != 0;
+    const bool failed = s_over != 0;
     __syncthreads();
-    if (over) {
-      // Split the residue class: keys of one pattern (or one root label)
-      // never straddle classes, so per-graph dedup stays exact.
+    if (failed) {
       if (threadIdx.x == 0) {
         if (mod >= (1u << 30) || s_top + 1 > 32) {
           atomicOr(a.error_flag, 1);
@@ -319,16 +299,19 @@ __global__ void __launch_bounds__(256) k_support(LevelArgs a) {
       continue;
     }
     my_cand += cand;
-    for (int i = threadIdx.x; i < kHashCap; i += blockDim.x) {
-      const unsigned long long k = table[i];
-      if (k == kEmpty) continue;
-      const int32_t old = atomicAdd(&a.sup[k], 1);
-      const int32_t owner = ROOT ? (int32_t)(k / (uint64_t)a.K) : a.meta[k / (uint64_t)a.K].job;
+    for (int i = threadIdx.x; i < kTableCap; i += blockDim.x) {
+      const uint32_t k = table[i];
+      if (k == kEmpty32) continue;
+      const int64_t gk = (int64_t)a.p_lo * a.K + (int64_t)k;
+      const int32_t old = atomicAdd(&a.sup[gk], 1);
+      const int64_t owner_p = gk / a.K;
+      const int32_t owner = ROOT ? (int32_t)owner_p : a.meta[owner_p].job;
       if (old + 1 == a.minsup_job[owner]) {
-        const uint32_t slot = atomicAdd(a.freq_count, 1u);
-        if (slot < a.freq_cap) a.freq[slot] = k;
+        const uint32_t slot_i = atomicAdd(a.freq_count, 1u);
+        if (slot_i < a.freq_cap) a.freq[slot_i] = (unsigned long long)gk;
       }
     }
+    __syncthreads();
     if (threadIdx.x == 0) s_top -= 1;
   }
   // candidate counter (E_cand): one atomic per wave
@@ -336,32 +319,38 @@ __global__ void __launch_bounds__(256) k_support(LevelArgs a) {
   if ((threadIdx.x & 63) == 0 && my_cand) atomicAdd(a.ecand, my_cand);
 }
 
-// ------------------------------------------------------------ COUNT/WRITE
-template <bool ROOT, bool WRITE>
-__global__ void __launch_bounds__(256) k_children(LevelArgs a) {
-  __shared__ Shared sh;
-  __shared__ int s_waves[17];
+template <bool ROOT>
+__global__ void __launch_bounds__(kBlock) k_support(LevelArgs a) {
+  __shared__ Stage st;
+  __shared__ uint32_t table[kTableCap];
+  __shared__ uint32_t slots[kBlock * kSlotWords];
+  __shared__ int s_misc[4];
+  __shared__ uint32_t s_stack_mod[32], s_stack_res[32];
   const int g = blockIdx.x;
-  int nv;
-  int64_t r0, r1;
+  int64_t r0 = 0, r1;
   if (ROOT) {
-    r0 = 0;
-    r1 = 0;
+    const int job = a.gjob[g];
+    if (job < a.p_lo || job >= a.p_hi) return;
+    r1 = a.gvbase[g + 1] - a.gvbase[g];
   } else {
     r0 = a.grow[g];
     r1 = a.grow[g + 1];
-    if (r0 == r1) {
-      if (!WRITE && threadIdx.x == 0) a.ccount[g] = 0;
-      return;
-    }
   }
-  GView G = stage_graph(a, g, sh, nv);
+  if (r0 == r1) return;
+  uint32_t *slot = slots + threadIdx.x * kSlotWords;
+  if (graph_fits(a, g))
+    support_body<ROOT, true>(a, g, r0, r1, st, table, slot, s_misc, s_stack_mod, s_stack_res);
+  else
+    support_body<ROOT, false>(a, g, r0, r1, st, table, slot, s_misc, s_stack_mod, s_stack_res);
+}
+
+// ------------------------------------------------------------ COUNT/WRITE
+template <bool ROOT, bool WRITE, bool LDS>
+__device__ __forceinline__ void children_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, Stage &st,
+                                              uint32_t *slot, int *s_waves, long long *s_red) {
+  const GView G = graph_view<LDS>(a, g, st);
   const int job = ROOT ? a.gjob[g] : 0;
   const int32_t vb = a.gvbase[g];
-  if (ROOT) {
-    r0 = 0;
-    r1 = nv;
-  }
   const int64_t nrows = r1 - r0;
   int64_t out = WRITE ? a.cgrow[g] : 0;
   long long total_count = 0;
@@ -369,7 +358,6 @@ __global__ void __launch_bounds__(256) k_children(LevelArgs a) {
     const int64_t r = r0 + base + threadIdx.x;
     const bool live = base + threadIdx.x < nrows;
     int cnt = 0;
-    Tuple t{0, 0, 0, 0};
     const PatMeta *pm = a.meta;
     uint64_t kbase = 0;
     int lu = 0;
@@ -383,9 +371,9 @@ __global__ void __launch_bounds__(256) k_children(LevelArgs a) {
       } else {
         const int32_t p = a.pid[r];
         pm = a.meta + p;
-        t = load_tuple(a.vt, r, a.S);
+        load_slot(slot, a.vt, r, a.S);
         kbase = (uint64_t)p * (uint64_t)a.K;
-        enumerate_row(G, pm, t, a.S, a.NEL, a.NVL, [&](int key, int) {
+        enumerate_row(G, pm, slot, a.S, a.NEL, a.NVL, [&](int key, int) {
           if (a.sup[kbase + key] < 0) ++cnt;
         });
       }
@@ -404,22 +392,20 @@ __global__ void __launch_bounds__(256) k_children(LevelArgs a) {
           const int32_t v = a.sup[kbase + key];
           if (v < 0) {
             a.cpid[pos] = -v - 1;
-            Tuple c{0, 0, 0, 0};
-            tset(c, 0, u);
-            tset(c, 1, w);
-            store_tuple(a.cvt, pos, a.CS, c);
+            uint32_t *p = reinterpret_cast<uint32_t *>(a.cvt + pos * a.CS);
+            p[0] = (uint32_t)u | ((uint32_t)w << 8);
+            for (int i = 1; i < (a.CS >> 2); ++i) p[i] = 0;
             ++pos;
           }
         });
       } else {
         const int nvp = pm->nv;
-        enumerate_row(G, pm, t, a.S, a.NEL, a.NVL, [&](int key, int w) {
+        enumerate_row(G, pm, slot, a.S, a.NEL, a.NVL, [&](int key, int w) {
           const int32_t v = a.sup[kbase + key];
           if (v < 0) {
             a.cpid[pos] = -v - 1;
-            Tuple c = t;
-            if (w >= 0) tset(c, nvp, w);
-            store_tuple(a.cvt, pos, a.CS, c);
+            if (w >= 0) slot_set(slot, nvp, w);
+            store_slot(a.cvt, pos, a.CS, slot);
             ++pos;
           }
         });
@@ -428,10 +414,8 @@ __global__ void __launch_bounds__(256) k_children(LevelArgs a) {
     out += tot;
   }
   if (!WRITE) {
-    // block reduction of the per-thread counts
     long long v = total_count;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    __shared__ long long s_red[16];
     if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -440,6 +424,31 @@ __global__ void __launch_bounds__(256) k_children(LevelArgs a) {
       a.ccount[g] = s;
     }
   }
+}
+
+template <bool ROOT, bool WRITE>
+__global__ void __launch_bounds__(kBlock) k_children(LevelArgs a) {
+  __shared__ Stage st;
+  __shared__ uint32_t slots[kBlock * kSlotWords];
+  __shared__ int s_waves[17];
+  __shared__ long long s_red[16];
+  const int g = blockIdx.x;
+  int64_t r0 = 0, r1;
+  if (ROOT)
+    r1 = a.gvbase[g + 1] - a.gvbase[g];
+  else {
+    r0 = a.grow[g];
+    r1 = a.grow[g + 1];
+  }
+  if (r0 == r1) {
+    if (!WRITE && threadIdx.x == 0) a.ccount[g] = 0;
+    return;
+  }
+  uint32_t *slot = slots + threadIdx.x * kSlotWords;
+  if (graph_fits(a, g))
+    children_body<ROOT, WRITE, true>(a, g, r0, r1, st, slot, s_waves, s_red);
+  else
+    children_body<ROOT, WRITE, false>(a, g, r0, r1, st, slot, s_waves, s_red);
 }
 
 // ------------------------------------------------------------ primitives

@@ -9,8 +9,14 @@
 namespace gm {
 
 constexpr int kBlock = 256;      // threads per workgroup of the level kernels
-constexpr int kHashBits = 11;    // LDS dedup table: 2048 x 8 bytes
-constexpr int kHashCap = 1 << kHashBits;
+// Support kernel: LDS dedup table of 2^kTableBits 32-bit keys; a pass whose
+// table passes kTableFill distinct keys is split.
+#ifndef GM_TABLE_BITS
+#define GM_TABLE_BITS 12
+#endif
+constexpr int kTableBits = GM_TABLE_BITS;
+constexpr int kTableCap = 1 << kTableBits;
+constexpr int kTableFill = kTableCap * 3 / 4;
 
 // Everything one level-step kernel reads or writes.  Plain device pointers.
 struct LevelArgs {
@@ -27,6 +33,9 @@ struct LevelArgs {
   int32_t S;  // tuple stride in bytes
   const PatMeta *meta;
   int64_t K;  // keys per pattern (root level: keys per job)
+  // patterns (root level: jobs) [p_lo, p_hi) handled by this launch, so
+  // that (p - p_lo) * K + key fits 32 bits
+  int32_t p_lo, p_hi;
   int32_t NEL, NVL;
   // supports: >= 0 a distinct-graph count; < 0 encodes a selected child
   // as -(child + 1) once the host has chosen the children

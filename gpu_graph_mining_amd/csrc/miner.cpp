@@ -346,8 +346,14 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     a.freq = freq.as<unsigned long long>();
     a.freq_cap = fcap;
 
+    // 32-bit dedup keys: launch over pattern ranges with range * K < 2^32
+    const int64_t chunk = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
     hip_check(hipEventRecord(ev0, s), "hipEventRecord");
-    hip_check(launch_support(a, G, root, s), "k_support");
+    for (int64_t lo = 0; lo < P; lo += chunk) {
+      a.p_lo = (int32_t)lo;
+      a.p_hi = (int32_t)std::min<int64_t>(P, lo + chunk);
+      hip_check(launch_support(a, G, root, s), "k_support");
+    }
     hip_check(hipEventRecord(ev1, s), "hipEventRecord");
     struct {
       uint32_t fcount, pad;

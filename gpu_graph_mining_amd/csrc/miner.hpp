@@ -6,7 +6,7 @@
 #include <memory>
 #include <vector>
 
-#include "../../include/gspan_amd.h"
+#include "gspan_amd.h"
 #include "device.hpp"
 #include "gm_internal.hpp"
 
