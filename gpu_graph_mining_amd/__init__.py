@@ -436,7 +436,7 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
         out.append((code, int(sup), int(nproj)))
         return 0
 
-    ccb = None if count_only else PATTERN_CB(cb)
+    ccb = PATTERN_CB() if count_only else PATTERN_CB(cb)  # PATTERN_CB() is a NULL pointer
     cfg = _make_cfg(label_style, max_edges, canon_threads, collect_proj)
     st = gm_stats()
     _check(lib.gm_mine(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st)))
@@ -444,9 +444,9 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
 
 
 def mine_batch(dbs: Sequence[GraphDB], minsups: Sequence[int], *, label_style: int = GM_LABELS_FULL,
-               collect_proj: bool = False, canon_threads: int = 0):
+               collect_proj: bool = False, canon_threads: int = 0, count_only: bool = False):
     """Mine several independent databases in one batched schedule.  Returns
-    (per_job_patterns, stats)."""
+    (per_job_patterns, stats); with count_only nothing is reported back."""
     lib = load_library()
     res: List[list] = [[] for _ in dbs]
 
@@ -454,7 +454,7 @@ def mine_batch(dbs: Sequence[GraphDB], minsups: Sequence[int], *, label_style: i
         res[job].append((DFSCode(tuple(dfs[5 * i + j] for j in range(5)) for i in range(n)), int(sup), int(nproj)))
         return 0
 
-    ccb = JOB_PATTERN_CB(cb)
+    ccb = JOB_PATTERN_CB() if count_only else JOB_PATTERN_CB(cb)
     handles = (C.c_void_p * len(dbs))(*[d._h.value for d in dbs])
     ms = _i32(minsups)
     cfg = _make_cfg(label_style, 0, canon_threads, collect_proj)
