@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "gm_version", "gm_last_error", "gm_cfg_init", "gm_device_count", "gm_db_create", "gm_db_load",
     "gm_db_write_txt", "gm_db_counts", "gm_db_export", "gm_db_upload", "gm_db_destroy", "gm_mine",
     "gm_mine_batch", "gm_dfs_code_is_min", "gm_gen_random", "gm_gen_planted", "gm_gen_molecule", "gm_db_subset",
+    "gm_batch_create", "gm_batch_mine", "gm_batch_destroy",
 )
 
 
@@ -61,7 +62,8 @@ class gm_stats(C.Structure):
         ("n_patterns", C.c_uint64), ("e_child", C.c_uint64), ("e_root", C.c_uint64), ("e_cand", C.c_uint64),
         ("n_candidates", C.c_uint64), ("n_levels", C.c_uint64), ("seconds_total", C.c_double),
         ("seconds_device", C.c_double), ("seconds_canon", C.c_double), ("seconds_ext_kernel", C.c_double),
-        ("ext_kernel_launches", C.c_uint64), ("peak_device_bytes", C.c_uint64), ("reserved", C.c_uint64 * 6),
+        ("ext_kernel_launches", C.c_uint64), ("peak_device_bytes", C.c_uint64), ("split_passes", C.c_uint64),
+        ("reserved", C.c_uint64 * 5),
     ]
 
     def as_dict(self) -> dict:
@@ -108,6 +110,9 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "gm_gen_molecule": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_int32, C.c_int32, P(C.c_double),
                                       C.c_int32, P(C.c_double), P(dbp)]),
         "gm_db_subset": (C.c_int, [dbp, i32p, C.c_int32, P(dbp)]),
+        "gm_batch_create": (C.c_int, [C.c_int32, P(dbp), P(dbp)]),
+        "gm_batch_mine": (C.c_int, [dbp, i32p, P(gm_cfg), JOB_PATTERN_CB, C.c_void_p, P(gm_stats)]),
+        "gm_batch_destroy": (None, [dbp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -463,6 +468,47 @@ def mine_batch(dbs: Sequence[GraphDB], minsups: Sequence[int], *, label_style: i
     return res, st.as_dict()
 
 
+class Batch:
+    """Databases of several jobs resident on the device in one packed
+    layout (gm_batch_create); mine() runs all jobs in one schedule."""
+
+    def __init__(self, dbs: Sequence[GraphDB]):
+        lib = load_library()
+        handles = (C.c_void_p * len(dbs))(*[d._h.value for d in dbs])
+        h = C.c_void_p()
+        _check(lib.gm_batch_create(len(dbs), handles, C.byref(h)))
+        self._h = h
+        self.n_jobs = len(dbs)
+
+    def mine(self, minsups: Sequence[int], *, label_style: int = GM_LABELS_FULL, collect_proj: bool = False,
+             canon_threads: int = 0, count_only: bool = False):
+        lib = load_library()
+        res: List[list] = [[] for _ in range(self.n_jobs)]
+
+        def cb(_user, job, dfs, n, sup, nproj):
+            res[job].append((DFSCode(tuple(dfs[5 * i + j] for j in range(5)) for i in range(n)), int(sup),
+                             int(nproj)))
+            return 0
+
+        ccb = JOB_PATTERN_CB() if count_only else JOB_PATTERN_CB(cb)
+        ms = _i32(minsups)
+        cfg = _make_cfg(label_style, 0, canon_threads, collect_proj)
+        st = gm_stats()
+        _check(lib.gm_batch_mine(self._h, _ptr(ms), C.byref(cfg), ccb, None, C.byref(st)))
+        return res, st.as_dict()
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            load_library().gm_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class gspan_cuda:
     """The reference's driver class (src/gspan_cuda/gspan_cuda.hpp:53-293)
     over the HIP path: same method names, same call order."""
@@ -513,7 +559,7 @@ class gspan_cuda:
 
 
 __all__ = [
-    "GraphDB", "DFS", "DFSCode", "graph_output", "graph_counting_output", "graph_file_output",
+    "GraphDB", "Batch", "DFS", "DFSCode", "graph_output", "graph_counting_output", "graph_file_output",
     "graph_collecting_output", "gspan_cuda", "mine", "mine_batch", "load_library", "device_count", "version",
     "GMError", "GM_LABELS_FULL", "GM_LABELS_WILDCARD", "EXPORTED_SYMBOLS",
 ]

@@ -25,6 +25,14 @@ struct gm_db {
   }
 };
 
+struct gm_batch {
+  gm::DeviceDB dev;
+  hipStream_t stream = nullptr;
+  ~gm_batch() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -241,21 +249,31 @@ int gm_mine(gm_db *db, int32_t abs_minsup, const gm_cfg *cfg, gm_pattern_cb cb, 
   });
 }
 
-int gm_mine_batch(int32_t n_jobs, gm_db *const *dbs, const int32_t *abs_minsup, const gm_cfg *cfg,
-                  gm_job_pattern_cb cb, void *user, gm_stats *stats) {
+int gm_batch_create(int32_t n_jobs, gm_db *const *dbs, gm_batch **out) {
   return guard([&]() {
-    if (n_jobs < 1 || !dbs || !abs_minsup) throw gm::Error(GM_E_ARG, "null argument");
+    if (n_jobs < 1 || !dbs || !out) throw gm::Error(GM_E_ARG, "null argument");
+    *out = nullptr;
     std::vector<const gm::HostDB *> hs;
     for (int32_t j = 0; j < n_jobs; ++j) {
       if (!dbs[j]) throw gm::Error(GM_E_ARG, "null database");
-      if (abs_minsup[j] < 1) throw gm::Error(GM_E_ARG, "absolute min support < 1");
       hs.push_back(&dbs[j]->host);
     }
-    gm::DeviceDB merged;
+    std::unique_ptr<gm_batch> b(new gm_batch());
     gm::PackedDB pk;
-    gm::pack_db(hs, merged, pk);
-    ensure_stream(dbs[0]);
-    gm::upload_db(pk, merged, dbs[0]->stream);
+    gm::pack_db(hs, b->dev, pk);
+    gm::hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+    gm::upload_db(pk, b->dev, b->stream);
+    *out = b.release();
+  });
+}
+
+int gm_batch_mine(gm_batch *batch, const int32_t *abs_minsup, const gm_cfg *cfg, gm_job_pattern_cb cb, void *user,
+                  gm_stats *stats) {
+  return guard([&]() {
+    if (!batch || !abs_minsup) throw gm::Error(GM_E_ARG, "null argument");
+    const int32_t n_jobs = batch->dev.n_jobs;
+    for (int32_t j = 0; j < n_jobs; ++j)
+      if (abs_minsup[j] < 1) throw gm::Error(GM_E_ARG, "absolute min support < 1");
     gm::MineOptions o = options(cfg);
     o.minsup.assign(abs_minsup, abs_minsup + n_jobs);
     const int style = cfg ? cfg->label_style : GM_LABELS_FULL;
@@ -265,8 +283,21 @@ int gm_mine_batch(int32_t n_jobs, gm_db *const *dbs, const int32_t *abs_minsup, 
       flatten(code, style, flat);
       return cb(user, job, flat.data(), (int32_t)code.size(), sup, nproj) == 0;
     };
-    if (!gm::mine(merged, o, rep, stats, dbs[0]->stream)) throw gm::Error(GM_E_CALLBACK, "callback stopped mining");
+    if (!gm::mine(batch->dev, o, rep, stats, batch->stream))
+      throw gm::Error(GM_E_CALLBACK, "callback stopped mining");
   });
+}
+
+void gm_batch_destroy(gm_batch *batch) { delete batch; }
+
+int gm_mine_batch(int32_t n_jobs, gm_db *const *dbs, const int32_t *abs_minsup, const gm_cfg *cfg,
+                  gm_job_pattern_cb cb, void *user, gm_stats *stats) {
+  gm_batch *b = nullptr;
+  int rc = gm_batch_create(n_jobs, dbs, &b);
+  if (rc != GM_OK) return rc;
+  rc = gm_batch_mine(b, abs_minsup, cfg, cb, user, stats);
+  gm_batch_destroy(b);
+  return rc;
 }
 
 int gm_dfs_code_is_min(const int32_t *dfs5, int32_t n_edges) {

@@ -4,11 +4,11 @@
 // support question ("how many distinct graphs have an embedding of P with
 // extension x") is answered by deduplicating (pattern, key) pairs inside the
 // workgroup (an LDS hash table) and adding 1 per distinct pair to a global
-// counter.  The same enumeration routine drives three kernels:
-//   k_support          distinct-graph support of every (pattern, key)
-//   k_children<false>  child rows per graph for the selected keys (count)
-//   k_children<true>   the child rows themselves (pattern id + tuple)
-// and, for the first level, the same three over 1-edge keys.
+// counter.  The same enumeration routine drives two kernels:
+//   k_support  distinct-graph support and embedding count of every
+//              (pattern, key)
+//   k_write    the child rows (pattern id + tuple) of the selected keys
+// and, for the first level, the same two over 1-edge keys.
 //
 // Extension rules (pattern side precomputed in PatMeta, graph side read
 // from the packed adjacency):
@@ -42,8 +42,6 @@ namespace gm {
 
 namespace {
 
-constexpr int kStageV = 257;   // vertex offsets staged in LDS (<= 256 vertices)
-constexpr int kStageA = 2048;  // adjacency entries staged in LDS
 constexpr int kSlotWords = 9;  // per-thread tuple slot: 36 bytes (odd word stride)
 constexpr uint32_t kEmpty32 = 0xFFFFFFFFu;
 
@@ -156,28 +154,50 @@ __device__ __forceinline__ void enumerate_root(const GView &G, int u, int lu, in
   }
 }
 
-struct Stage {
-  int32_t vadj[kStageV];
-  uint32_t adj[kStageA];
+// Dynamic LDS layout of the level kernels (bytes):
+//   [0, 8*2^tb)                dedup table: keys, then candidate counts
+//                              (k_support only)
+//   then kBlock * 36           per-thread tuple slots
+//   then 4*stage_v             staged vertex offsets
+//   then 4*stage_a             staged adjacency entries
+struct Lds {
+  uint32_t *table;
+  uint32_t *slots;
+  int32_t *vadj;
+  uint32_t *adj;
 };
+
+__device__ __forceinline__ Lds lds_layout(const LevelArgs &a, bool with_table) {
+  extern __shared__ uint32_t dyn[];
+  Lds l;
+  uint32_t *p = dyn;
+  l.table = p;
+  if (with_table) p += (2u << a.table_bits);
+  l.slots = p;
+  p += blockDim.x * kSlotWords;
+  l.vadj = reinterpret_cast<int32_t *>(p);
+  p += a.stage_v;
+  l.adj = p;
+  return l;
+}
 
 // Whether graph g's adjacency fits the LDS stage (uniform per workgroup).
 __device__ __forceinline__ bool graph_fits(const LevelArgs &a, int g) {
   const int32_t vb = a.gvbase[g], ve = a.gvbase[g + 1];
-  return ve - vb + 1 <= kStageV && a.vadj[ve] - a.vadj[vb] <= kStageA;
+  return ve - vb + 1 <= a.stage_v && a.vadj[ve] - a.vadj[vb] <= a.stage_a;
 }
 
 template <bool LDS>
-__device__ __forceinline__ GView graph_view(const LevelArgs &a, int g, Stage &st) {
+__device__ __forceinline__ GView graph_view(const LevelArgs &a, int g, const Lds &l) {
   const int32_t vb = a.gvbase[g], ve = a.gvbase[g + 1];
   const int32_t ab = a.vadj[vb], ae = a.vadj[ve];
   GView G;
   if (LDS) {
-    for (int i = threadIdx.x; i <= ve - vb; i += blockDim.x) st.vadj[i] = a.vadj[vb + i];
-    for (int i = threadIdx.x; i < ae - ab; i += blockDim.x) st.adj[i] = a.adj[ab + i];
+    for (int i = threadIdx.x; i <= ve - vb; i += blockDim.x) l.vadj[i] = a.vadj[vb + i];
+    for (int i = threadIdx.x; i < ae - ab; i += blockDim.x) l.adj[i] = a.adj[ab + i];
     __syncthreads();
-    G.vadj = st.vadj;
-    G.A = st.adj;
+    G.vadj = l.vadj;
+    G.A = l.adj;
     G.aoff = ab;
   } else {
     G.vadj = a.vadj + vb;
@@ -214,22 +234,32 @@ __device__ __forceinline__ int block_exclusive_scan(int x, int &total, int *s_wa
   return r;
 }
 
-__device__ __forceinline__ uint32_t hash_slot32(uint32_t k) { return (k * 0x9E3779B1u) >> (32 - kTableBits); }
+__device__ __forceinline__ uint32_t hash_slot32(uint32_t k, int bits) { return (k * 0x9E3779B1u) >> (32 - bits); }
 
 // ---------------------------------------------------------------- SUPPORT
 // One pass per residue class of pattern ids (p mod `mod` == `res`; root
 // level: of the source vertex label): every candidate key
-// (p - p_lo) * K + key is inserted into an LDS hash table; when the table
-// passes 3/4 full the pass is abandoned and the class split in two (all keys
-// of one pattern stay in one class, so the per-graph dedup stays exact);
-// a completed pass adds 1 per distinct key to the global supports.
+// (p - p_lo) * K + key is inserted into the LDS hash table, whose second
+// half counts the candidates per key; when the table passes 3/4 full the
+// pass is abandoned and the class split in two (all keys of one pattern stay
+// in one class, so the per-graph dedup stays exact).  A completed pass adds
+// 1 per distinct key to the distinct-graph counters and the key's candidate
+// count to the embedding counters of the graph's replica
+// (replica = workgroup id mod a.rep; k_reduce sums the replicas).
 template <bool ROOT, bool LDS>
-__device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, Stage &st,
-                                             uint32_t *table, uint32_t *slot, int *s_misc,
-                                             uint32_t *s_stack_mod, uint32_t *s_stack_res) {
-  const GView G = graph_view<LDS>(a, g, st);
+__device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, const Lds &l,
+                                             int *s_misc, uint32_t *s_stack_mod, uint32_t *s_stack_res) {
+  const GView G = graph_view<LDS>(a, g, l);
   const int job = ROOT ? a.gjob[g] : 0;
   const int32_t vb = a.gvbase[g];
+  const int tb = a.table_bits;
+  const uint32_t cap = 1u << tb, fill_limit = cap - cap / 4;
+  uint32_t *table = l.table;
+  uint32_t *tcnt = l.table + cap;
+  uint32_t *slot = l.slots + threadIdx.x * kSlotWords;
+  const int64_t rbase = (int64_t)(g & (a.rep - 1)) * a.nkeys;
+  int32_t *rep = a.sup_rep + rbase;
+  unsigned long long *repc = a.proj_rep + rbase;
   int &s_fill = s_misc[0], &s_over = s_misc[1], &s_top = s_misc[2];
   unsigned long long my_cand = 0;
   if (threadIdx.x == 0) {
@@ -241,7 +271,10 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
     __syncthreads();
     if (s_top == 0) break;
     const uint32_t mod = s_stack_mod[s_top - 1], res = s_stack_res[s_top - 1];
-    for (int i = threadIdx.x; i < kTableCap; i += blockDim.x) table[i] = kEmpty32;
+    for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
+      table[i] = kEmpty32;
+      tcnt[i] = 0;
+    }
     if (threadIdx.x == 0) {
       s_fill = 0;
       s_over = 0;
@@ -251,15 +284,15 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
     int over = 0;  // this thread saw the table pass its fill limit
     auto insert = [&](uint32_t k) {
       ++cand;
-      uint32_t h = hash_slot32(k);
-      for (int probe = 0; probe < kTableCap; ++probe) {
+      uint32_t h = hash_slot32(k, tb);
+      for (uint32_t probe = 0; probe < cap; ++probe) {
         const uint32_t old = atomicCAS(&table[h], kEmpty32, k);
-        if (old == kEmpty32) {
-          if (atomicAdd(&s_fill, 1) >= kTableFill) over = 1;
+        if (old == kEmpty32 || old == k) {
+          atomicAdd(&tcnt[h], 1u);
+          if (old == kEmpty32 && (uint32_t)atomicAdd(&s_fill, 1) >= fill_limit) over = 1;
           return;
         }
-        if (old == k) return;
-        h = (h + 1) & (kTableCap - 1);
+        h = (h + 1) & (cap - 1);
       }
       over = 1;
     };
@@ -285,6 +318,7 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
     __syncthreads();
     if (failed) {
       if (threadIdx.x == 0) {
+        atomicAdd(a.split_passes, 1ull);
         if (mod >= (1u << 30) || s_top + 1 > 32) {
           atomicOr(a.error_flag, 1);
           s_top = 0;
@@ -299,17 +333,12 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
       continue;
     }
     my_cand += cand;
-    for (int i = threadIdx.x; i < kTableCap; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
       const uint32_t k = table[i];
       if (k == kEmpty32) continue;
       const int64_t gk = (int64_t)a.p_lo * a.K + (int64_t)k;
-      const int32_t old = atomicAdd(&a.sup[gk], 1);
-      const int64_t owner_p = gk / a.K;
-      const int32_t owner = ROOT ? (int32_t)owner_p : a.meta[owner_p].job;
-      if (old + 1 == a.minsup_job[owner]) {
-        const uint32_t slot_i = atomicAdd(a.freq_count, 1u);
-        if (slot_i < a.freq_cap) a.freq[slot_i] = (unsigned long long)gk;
-      }
+      atomicAdd(&rep[gk], 1);
+      atomicAdd(&repc[gk], (unsigned long long)tcnt[i]);
     }
     __syncthreads();
     if (threadIdx.x == 0) s_top -= 1;
@@ -319,75 +348,132 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
   if ((threadIdx.x & 63) == 0 && my_cand) atomicAdd(a.ecand, my_cand);
 }
 
+// Rows of graph g at the current level (root level: its vertices).
 template <bool ROOT>
-__global__ void __launch_bounds__(kBlock) k_support(LevelArgs a) {
-  __shared__ Stage st;
-  __shared__ uint32_t table[kTableCap];
-  __shared__ uint32_t slots[kBlock * kSlotWords];
+__device__ __forceinline__ void graph_rows(const LevelArgs &a, int g, int64_t &r0, int64_t &r1) {
+  if (ROOT) {
+    r0 = 0;
+    r1 = a.gvbase[g + 1] - a.gvbase[g];
+  } else {
+    r0 = a.gstart[g];
+    r1 = r0 + a.gcount[g];
+  }
+}
+
+template <bool ROOT>
+__global__ void __launch_bounds__(kMaxBlock) k_support(LevelArgs a) {
   __shared__ int s_misc[4];
   __shared__ uint32_t s_stack_mod[32], s_stack_res[32];
   const int g = blockIdx.x;
-  int64_t r0 = 0, r1;
   if (ROOT) {
     const int job = a.gjob[g];
     if (job < a.p_lo || job >= a.p_hi) return;
-    r1 = a.gvbase[g + 1] - a.gvbase[g];
-  } else {
-    r0 = a.grow[g];
-    r1 = a.grow[g + 1];
   }
+  int64_t r0, r1;
+  graph_rows<ROOT>(a, g, r0, r1);
   if (r0 == r1) return;
-  uint32_t *slot = slots + threadIdx.x * kSlotWords;
+  const Lds l = lds_layout(a, true);
   if (graph_fits(a, g))
-    support_body<ROOT, true>(a, g, r0, r1, st, table, slot, s_misc, s_stack_mod, s_stack_res);
+    support_body<ROOT, true>(a, g, r0, r1, l, s_misc, s_stack_mod, s_stack_res);
   else
-    support_body<ROOT, false>(a, g, r0, r1, st, table, slot, s_misc, s_stack_mod, s_stack_res);
+    support_body<ROOT, false>(a, g, r0, r1, l, s_misc, s_stack_mod, s_stack_res);
 }
 
-// ------------------------------------------------------------ COUNT/WRITE
-template <bool ROOT, bool WRITE, bool LDS>
-__device__ __forceinline__ void children_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, Stage &st,
-                                              uint32_t *slot, int *s_waves, long long *s_red) {
-  const GView G = graph_view<LDS>(a, g, st);
-  const int job = ROOT ? a.gjob[g] : 0;
-  const int32_t vb = a.gvbase[g];
-  const int64_t nrows = r1 - r0;
-  int64_t out = WRITE ? a.cgrow[g] : 0;
-  long long total_count = 0;
-  for (int64_t base = 0; base < nrows; base += blockDim.x) {
-    const int64_t r = r0 + base + threadIdx.x;
-    const bool live = base + threadIdx.x < nrows;
-    int cnt = 0;
-    const PatMeta *pm = a.meta;
-    uint64_t kbase = 0;
-    int lu = 0;
-    if (live) {
-      if (ROOT) {
-        lu = a.vlab[vb + (int)r];
-        kbase = (uint64_t)job * a.K;
-        enumerate_root(G, (int)r, lu, a.NEL, a.NVL, [&](int key, int) {
-          if (a.sup[kbase + key] < 0) ++cnt;
-        });
-      } else {
-        const int32_t p = a.pid[r];
-        pm = a.meta + p;
-        load_slot(slot, a.vt, r, a.S);
-        kbase = (uint64_t)p * (uint64_t)a.K;
-        enumerate_row(G, pm, slot, a.S, a.NEL, a.NVL, [&](int key, int) {
-          if (a.sup[kbase + key] < 0) ++cnt;
-        });
+// Sum the replicas of every key; keys reaching their job's min support go to
+// the frequent list as (key, support, embeddings).
+template <bool ROOT>
+__global__ void __launch_bounds__(256) k_reduce(LevelArgs a) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nkeys; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t s = 0;
+    unsigned long long c = 0;
+    for (int r = 0; r < a.rep; ++r) {
+      s += a.sup_rep[(int64_t)r * a.nkeys + i];
+      c += a.proj_rep[(int64_t)r * a.nkeys + i];
+    }
+    a.sup[i] = s;
+    const int64_t p = i / a.K;
+    const int32_t owner = ROOT ? (int32_t)p : a.meta[p].job;
+    if (s >= a.minsup_job[owner]) {
+      const uint32_t slot = atomicAdd(a.freq_count, 1u);
+      if (slot < a.freq_cap) {
+        a.freq[3 * slot] = (unsigned long long)i;
+        a.freq[3 * slot + 1] = (unsigned long long)s;
+        a.freq[3 * slot + 2] = c;
       }
     }
-    if (!WRITE) {
-      total_count += cnt;
-      continue;
+  }
+}
+
+// ------------------------------------------------------------------ WRITE
+// Child rows of graph g for the selected keys (sup[key] < 0 encodes child
+// -(sup+1)): pass 1 counts each row's children into rowcnt, one atomic on
+// the level cursor reserves the graph's contiguous output segment, pass 2
+// scans the counts and writes (pattern id, tuple) rows.
+template <bool ROOT, bool LDS>
+__device__ __forceinline__ void write_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, const Lds &l,
+                                           int *s_waves, long long *s_red) {
+  const GView G = graph_view<LDS>(a, g, l);
+  const int job = ROOT ? a.gjob[g] : 0;
+  const int32_t vb = a.gvbase[g];
+  uint32_t *slot = l.slots + threadIdx.x * kSlotWords;
+  const int64_t cbase = ROOT ? (int64_t)vb : 0;  // rowcnt index base
+  // pass 1: count
+  long long mine = 0;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+    int cnt = 0;
+    if (ROOT) {
+      const int lu = a.vlab[vb + (int)r];
+      const uint64_t kbase = (uint64_t)job * a.K;
+      enumerate_root(G, (int)r, lu, a.NEL, a.NVL, [&](int key, int) {
+        if (a.sup[kbase + key] < 0) ++cnt;
+      });
+    } else {
+      const int32_t p = a.pid[r];
+      load_slot(slot, a.vt, r, a.S);
+      const uint64_t kbase = (uint64_t)p * (uint64_t)a.K;
+      enumerate_row(G, a.meta + p, slot, a.S, a.NEL, a.NVL, [&](int key, int) {
+        if (a.sup[kbase + key] < 0) ++cnt;
+      });
     }
+    a.rowcnt[cbase + r] = (uint16_t)cnt;
+    mine += cnt;
+  }
+  for (int o = 32; o > 0; o >>= 1) mine += __shfl_down(mine, o, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long total = 0;
+    for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) total += s_red[i];
+    long long start = 0;
+    if (total > 0) {
+      start = (long long)atomicAdd(a.cursor, (unsigned long long)total);
+      if (start + total > a.ccap) {
+        atomicOr(a.error_flag, 2);
+        total = 0;  // nothing is written; the host reports the overflow
+      }
+    }
+    a.cgstart[g] = start;
+    a.cgcount[g] = total;
+    s_red[0] = start;
+    s_red[1] = total;
+  }
+  __syncthreads();
+  const long long start = s_red[0];
+  if (s_red[1] == 0) return;
+  // pass 2: write
+  int64_t out = start;
+  for (int64_t base = r0; base < r1; base += blockDim.x) {
+    const int64_t r = base + threadIdx.x;
+    const bool live = r < r1;
+    const int cnt = live ? (int)a.rowcnt[cbase + r] : 0;
     int tot;
     const int pre = block_exclusive_scan(cnt, tot, s_waves);
     if (live && cnt) {
       int64_t pos = out + pre;
       if (ROOT) {
         const int u = (int)r;
+        const int lu = a.vlab[vb + u];
+        const uint64_t kbase = (uint64_t)job * a.K;
         enumerate_root(G, u, lu, a.NEL, a.NVL, [&](int key, int w) {
           const int32_t v = a.sup[kbase + key];
           if (v < 0) {
@@ -399,7 +485,11 @@ __device__ __forceinline__ void children_body(const LevelArgs &a, int g, int64_t
           }
         });
       } else {
+        const int32_t p = a.pid[r];
+        const PatMeta *pm = a.meta + p;
         const int nvp = pm->nv;
+        load_slot(slot, a.vt, r, a.S);
+        const uint64_t kbase = (uint64_t)p * (uint64_t)a.K;
         enumerate_row(G, pm, slot, a.S, a.NEL, a.NVL, [&](int key, int w) {
           const int32_t v = a.sup[kbase + key];
           if (v < 0) {
@@ -413,129 +503,30 @@ __device__ __forceinline__ void children_body(const LevelArgs &a, int g, int64_t
     }
     out += tot;
   }
-  if (!WRITE) {
-    long long v = total_count;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      long long s = 0;
-      for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) s += s_red[i];
-      a.ccount[g] = s;
-    }
-  }
 }
 
-template <bool ROOT, bool WRITE>
-__global__ void __launch_bounds__(kBlock) k_children(LevelArgs a) {
-  __shared__ Stage st;
-  __shared__ uint32_t slots[kBlock * kSlotWords];
+template <bool ROOT>
+__global__ void __launch_bounds__(kMaxBlock) k_write(LevelArgs a) {
   __shared__ int s_waves[17];
   __shared__ long long s_red[16];
   const int g = blockIdx.x;
-  int64_t r0 = 0, r1;
-  if (ROOT)
-    r1 = a.gvbase[g + 1] - a.gvbase[g];
-  else {
-    r0 = a.grow[g];
-    r1 = a.grow[g + 1];
-  }
+  int64_t r0, r1;
+  graph_rows<ROOT>(a, g, r0, r1);
   if (r0 == r1) {
-    if (!WRITE && threadIdx.x == 0) a.ccount[g] = 0;
+    if (threadIdx.x == 0) {
+      a.cgstart[g] = 0;
+      a.cgcount[g] = 0;
+    }
     return;
   }
-  uint32_t *slot = slots + threadIdx.x * kSlotWords;
+  const Lds l = lds_layout(a, false);
   if (graph_fits(a, g))
-    children_body<ROOT, WRITE, true>(a, g, r0, r1, st, slot, s_waves, s_red);
+    write_body<ROOT, true>(a, g, r0, r1, l, s_waves, s_red);
   else
-    children_body<ROOT, WRITE, false>(a, g, r0, r1, st, slot, s_waves, s_red);
+    write_body<ROOT, false>(a, g, r0, r1, l, s_waves, s_red);
 }
 
 // ------------------------------------------------------------ primitives
-// Exclusive scan of n int64 values (n+1 outputs; out[n] = total), three
-// phases: per-block sums, one-block scan of the sums, per-block rescan.
-constexpr int kScanBlock = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanBlock * kScanItems;
-
-__device__ __forceinline__ long long block_excl_scan_ll(long long x, long long &total, long long *s) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  long long v = x;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    long long y = __shfl_up(v, o, 64);
-    if (lane >= o) v += y;
-  }
-  if (lane == 63) s[wid] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    long long acc = 0;
-    for (int i = 0; i < kScanBlock / 64; ++i) {
-      long long t = s[i];
-      s[i] = acc;
-      acc += t;
-    }
-    s[kScanBlock / 64] = acc;
-  }
-  __syncthreads();
-  long long r = s[wid] + v - x;
-  total = s[kScanBlock / 64];
-  __syncthreads();
-  return r;
-}
-
-__global__ void __launch_bounds__(kScanBlock) k_scan_sums(const int64_t *in, int64_t n, int64_t *sums) {
-  __shared__ long long s[kScanBlock / 64 + 1];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-  long long acc = 0;
-#pragma unroll
-  for (int i = 0; i < kScanItems; ++i)
-    if (base + i < n) acc += in[base + i];
-  long long tot;
-  block_excl_scan_ll(acc, tot, s);
-  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
-}
-
-__global__ void __launch_bounds__(kScanBlock) k_scan_block_sums(int64_t *sums, int64_t nb) {
-  __shared__ long long s[kScanBlock / 64 + 1];
-  long long carry = 0;
-  for (int64_t base = 0; base < nb; base += kScanBlock) {
-    const int64_t i = base + threadIdx.x;
-    long long x = i < nb ? sums[i] : 0;
-    long long tot;
-    long long ex = block_excl_scan_ll(x, tot, s);
-    if (i < nb) sums[i] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0) sums[nb] = carry;
-}
-
-__global__ void __launch_bounds__(kScanBlock) k_scan_apply(const int64_t *in, int64_t n, const int64_t *sums,
-                                                           int64_t *out) {
-  __shared__ long long s[kScanBlock / 64 + 1];
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-  long long vals[kScanItems];
-  long long acc = 0;
-#pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    vals[i] = base + i < n ? in[base + i] : 0;
-    acc += vals[i];
-  }
-  long long tot;
-  long long ex = block_excl_scan_ll(acc, tot, s) + sums[blockIdx.x];
-#pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    if (base + i < n) out[base + i] = ex;
-    ex += vals[i];
-  }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = sums[gridDim.x];
-}
-
-__global__ void k_gather_i32(const int32_t *src, const uint64_t *idx, int64_t n, int32_t *dst) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[idx[i]];
-}
-
 __global__ void k_scatter_i32(int32_t *dst, const uint64_t *idx, const int32_t *val, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[idx[i]] = val[i];
@@ -546,55 +537,42 @@ __global__ void k_fill_i32(int32_t *dst, int64_t n, int32_t v) {
     dst[i] = v;
 }
 
-__global__ void k_pattern_hist(const int32_t *pid, int64_t n, unsigned long long *hist) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    atomicAdd(&hist[pid[i]], 1ull);
-}
-
 }  // namespace
 
 // ------------------------------------------------------------ launchers
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+size_t level_lds_bytes(const LevelArgs &a, bool with_table) {
+  return 4ull * ((with_table ? (2u << a.table_bits) : 0u) + (size_t)a.block * kSlotWords + a.stage_v + a.stage_a);
+}
+
 hipError_t launch_support(const LevelArgs &a, int n_graphs, bool root, hipStream_t s) {
   if (n_graphs <= 0) return hipSuccess;
+  const size_t lds = level_lds_bytes(a, true);
   if (root)
-    hipLaunchKernelGGL(k_support<true>, dim3(n_graphs), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_support<true>, dim3(n_graphs), dim3(a.block), lds, s, a);
   else
-    hipLaunchKernelGGL(k_support<false>, dim3(n_graphs), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(k_support<false>, dim3(n_graphs), dim3(a.block), lds, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_children(const LevelArgs &a, int n_graphs, bool root, bool write, hipStream_t s) {
+hipError_t launch_reduce(const LevelArgs &a, bool root, hipStream_t s) {
+  if (a.nkeys <= 0) return hipSuccess;
+  const unsigned g = (unsigned)std::min<int64_t>((a.nkeys + 255) / 256, 16384);
+  if (root)
+    hipLaunchKernelGGL(k_reduce<true>, dim3(g), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_reduce<false>, dim3(g), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t s) {
   if (n_graphs <= 0) return hipSuccess;
-  if (root) {
-    if (write)
-      hipLaunchKernelGGL((k_children<true, true>), dim3(n_graphs), dim3(kBlock), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_children<true, false>), dim3(n_graphs), dim3(kBlock), 0, s, a);
-  } else {
-    if (write)
-      hipLaunchKernelGGL((k_children<false, true>), dim3(n_graphs), dim3(kBlock), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_children<false, false>), dim3(n_graphs), dim3(kBlock), 0, s, a);
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_exclusive_scan(const int64_t *in, int64_t n, int64_t *out, int64_t *tmp, hipStream_t s) {
-  // tmp must hold num_tiles + 1 values
-  const int64_t tiles = std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile);
-  hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, in, n, tmp);
-  hipLaunchKernelGGL(k_scan_block_sums, dim3(1), dim3(kScanBlock), 0, s, tmp, tiles);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(kScanBlock), 0, s, in, n, tmp, out);
-  return hipGetLastError();
-}
-
-int64_t scan_tmp_elems(int64_t n) { return std::max<int64_t>(1, (n + kScanTile - 1) / kScanTile) + 1; }
-
-hipError_t launch_gather_i32(const int32_t *src, const uint64_t *idx, int64_t n, int32_t *dst, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_i32, dim3(grid1(n, 256)), dim3(256), 0, s, src, idx, n, dst);
+  const size_t lds = level_lds_bytes(a, false);
+  if (root)
+    hipLaunchKernelGGL(k_write<true>, dim3(n_graphs), dim3(a.block), lds, s, a);
+  else
+    hipLaunchKernelGGL(k_write<false>, dim3(n_graphs), dim3(a.block), lds, s, a);
   return hipGetLastError();
 }
 
@@ -608,13 +586,6 @@ hipError_t launch_fill_i32(int32_t *dst, int64_t n, int32_t v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const unsigned g = (unsigned)std::min<int64_t>(grid1(n, 256), 8192);
   hipLaunchKernelGGL(k_fill_i32, dim3(g), dim3(256), 0, s, dst, n, v);
-  return hipGetLastError();
-}
-
-hipError_t launch_pattern_hist(const int32_t *pid, int64_t n, unsigned long long *hist, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const unsigned g = (unsigned)std::min<int64_t>(grid1(n, 256), 8192);
-  hipLaunchKernelGGL(k_pattern_hist, dim3(g), dim3(256), 0, s, pid, n, hist);
   return hipGetLastError();
 }
 

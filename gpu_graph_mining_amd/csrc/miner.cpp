@@ -16,7 +16,9 @@
 //                    and the list of keys whose support reached min support;
 //   2. host        : decode each frequent key into a child code, keep the
 //                    minimal ones (canonical.cpp), number the children;
-//   3. k_children  : count child rows per graph, scan, write child rows.
+//   3. k_write     : child rows of the chosen keys, each graph's rows
+//                    reserved from a level cursor (sized exactly by the
+//                    embedding counts k_support gathered).
 #include "miner.hpp"
 
 #include <algorithm>
@@ -24,6 +26,9 @@
 #include <chrono>
 #include <cstring>
 #include <map>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 
 #include "kernels.hpp"
@@ -102,6 +107,7 @@ void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk
   std::vector<int32_t> &gvbase = pk.gvbase, &vadj = pk.vadj, &gjob = pk.gjob;
   std::vector<uint32_t> &adj = pk.adj;
   std::vector<uint8_t> &vlab = pk.vlab;
+  pk.max_graph_v = pk.max_graph_adj = 0;
   gvbase.clear();
   vadj.clear();
   gjob.clear();
@@ -146,6 +152,8 @@ void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk
         deg[db->eto[e]]++;
       }
       const int32_t abase = (int32_t)adj.size();
+      pk.max_graph_v = std::max(pk.max_graph_v, n);
+      pk.max_graph_adj = std::max(pk.max_graph_adj, 2 * (e1 - e0));
       fill.assign(n, 0);
       int32_t acc = abase;
       for (int32_t v = 0; v < n; ++v) {
@@ -169,6 +177,8 @@ void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk
   out.n_graphs = (int32_t)ng;
   out.n_vertices = nv;
   out.n_adj = (int64_t)adj.size();
+  out.max_graph_v = pk.max_graph_v;
+  out.max_graph_adj = pk.max_graph_adj;
 }
 
 void upload_db(const PackedDB &pk, DeviceDB &out, hipStream_t s) {
@@ -219,33 +229,85 @@ struct Timer {
   double sec() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
-// Run fn(i) for i in [0, n) on up to `threads` host threads.
-template <class F>
-void parallel_for(int64_t n, int threads, F &&fn) {
-  if (n <= 0) return;
-  if (threads <= 1 || n < 64) {
-    for (int64_t i = 0; i < n; ++i) fn(i);
-    return;
+// Persistent host worker pool for the canonical checks (one per process,
+// created on first use; workers sleep between level steps).
+class Pool {
+ public:
+  static Pool &get() {
+    static Pool p;
+    return p;
   }
-  std::atomic<int64_t> next(0);
-  std::vector<std::thread> pool;
-  std::exception_ptr err;
-  std::atomic<bool> failed(false);
-  for (int t = 0; t < threads; ++t)
-    pool.emplace_back([&]() {
+  // Run fn(i) for i in [0, n) on up to `threads` threads (the caller helps).
+  void run(int64_t n, int threads, const std::function<void(int64_t)> &fn) {
+    if (n <= 0) return;
+    if (threads <= 1 || n < 64) {
+      for (int64_t i = 0; i < n; ++i) fn(i);
+      return;
+    }
+    std::unique_lock<std::mutex> lk(m_);
+    while ((int)workers_.size() < threads - 1) workers_.emplace_back([this]() { loop(); });
+    fn_ = &fn;
+    n_ = n;
+    next_ = 0;
+    err_ = nullptr;
+    active_ = (int)workers_.size();
+    ++gen_;
+    lk.unlock();
+    cv_.notify_all();
+    work();
+    lk.lock();
+    done_.wait(lk, [this]() { return active_ == 0; });
+    fn_ = nullptr;
+    if (err_) std::rethrow_exception(err_);
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : workers_) t.join();
+  }
+
+ private:
+  void work() {
+    for (;;) {
+      const int64_t i = next_.fetch_add(32);
+      if (i >= n_) break;
+      const int64_t e = std::min(n_, i + 32);
       try {
-        for (;;) {
-          int64_t i = next.fetch_add(64);
-          if (i >= n || failed) break;
-          for (int64_t j = i; j < std::min(n, i + 64); ++j) fn(j);
-        }
+        for (int64_t j = i; j < e; ++j) (*fn_)(j);
       } catch (...) {
-        if (!failed.exchange(true)) err = std::current_exception();
+        std::lock_guard<std::mutex> lk(err_m_);
+        if (!err_) err_ = std::current_exception();
+        next_ = n_;
       }
-    });
-  for (auto &th : pool) th.join();
-  if (err) std::rethrow_exception(err);
-}
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    for (;;) {
+      std::unique_lock<std::mutex> lk(m_);
+      cv_.wait(lk, [&]() { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      lk.unlock();
+      work();
+      lk.lock();
+      if (--active_ == 0) done_.notify_all();
+    }
+  }
+  std::mutex m_, err_m_;
+  std::condition_variable cv_, done_;
+  std::vector<std::thread> workers_;
+  const std::function<void(int64_t)> *fn_ = nullptr;
+  std::atomic<int64_t> next_{0};
+  int64_t n_ = 0;
+  int active_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+  std::exception_ptr err_;
+};
 
 }  // namespace
 
@@ -259,17 +321,16 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   std::memset(&st, 0, sizeof(st));
 
   // device state
-  DevBuf minsup_d, sup, freq, counters, meta_d, grow_a, grow_b, pid_a, pid_b, vt_a, vt_b, ccount, scan_tmp,
-      idx_d, val_d, hist_d;
+  DevBuf minsup_d, sup, sup_rep, proj_rep, freq, counters, meta_d, gs_a, gs_b, gc_a, gc_b, pid_a, pid_b, vt_a, vt_b,
+      rowcnt, idx_d, val_d;
   upload(minsup_d, opt.minsup, s);
   counters.ensure(64);
-  ccount.ensure(sizeof(int64_t) * (G + 1));
-  scan_tmp.ensure(sizeof(int64_t) * scan_tmp_elems(G));
-  grow_a.ensure(sizeof(int64_t) * (G + 1));
-  grow_b.ensure(sizeof(int64_t) * (G + 1));
+  for (DevBuf *b : {&gs_a, &gs_b, &gc_a, &gc_b}) b->ensure(sizeof(int64_t) * std::max(G, 1));
   uint32_t *freq_count = reinterpret_cast<uint32_t *>(counters.as<char>());
   unsigned long long *ecand = reinterpret_cast<unsigned long long *>(counters.as<char>() + 8);
   int32_t *error_flag = reinterpret_cast<int32_t *>(counters.as<char>() + 16);
+  unsigned long long *split_passes = reinterpret_cast<unsigned long long *>(counters.as<char>() + 24);
+  unsigned long long *cursor = reinterpret_cast<unsigned long long *>(counters.as<char>() + 32);
 
   hipEvent_t ev0, ev1;
   hip_check(hipEventCreate(&ev0), "hipEventCreate");
@@ -298,14 +359,20 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   a.freq_count = freq_count;
   a.ecand = ecand;
   a.error_flag = error_flag;
-  a.ccount = ccount.as<int64_t>();
+  a.split_passes = split_passes;
+  a.cursor = cursor;
+  // LDS: dedup table of 2^11 keys; stage the largest graph's adjacency when
+  // it fits 16 KB (larger graphs are read from HBM)
+  a.table_bits = 11;
+  a.stage_v = db.max_graph_v + 1;
+  a.stage_a = std::min<int32_t>(db.max_graph_adj, 4096);
 
   std::vector<Pat> pats;      // patterns of the current level
   bool stopped = false;
   int level = 0;              // 0 = root stage
   int S = 0;                  // tuple stride of the current level
   int64_t n_rows = 0;
-  DevBuf *grow_cur = &grow_a, *grow_nxt = &grow_b;
+  DevBuf *gs_cur = &gs_a, *gs_nxt = &gs_b, *gc_cur = &gc_a, *gc_nxt = &gc_b;
   DevBuf *pid_cur = &pid_a, *pid_nxt = &pid_b, *vt_cur = &vt_a, *vt_nxt = &vt_b;
 
   auto to_original = [&](const std::vector<DFS> &c) {
@@ -331,20 +398,40 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       for (int64_t i = 0; i < P; ++i) metas[i] = make_meta(pats[i]);
       upload(meta_d, metas, s);
     }
+    // Support and embedding counters: `rep` replicas (summed by k_reduce)
+    // spread the atomics of small key spaces, where every graph hits the
+    // same keys.  Replicas stay within 256 KB of support counters so the
+    // atomics stay in L2.
+    int rep = 1;
+    while (rep < 64 && (int64_t)rep * 2 * nkeys * 4 <= (int64_t)(256 << 10)) rep *= 2;
     sup.ensure(sizeof(int32_t) * nkeys);
-    hip_check(hipMemsetAsync(sup.p, 0, sizeof(int32_t) * nkeys, s), "memset sup");
+    sup_rep.ensure(sizeof(int32_t) * nkeys * rep);
+    proj_rep.ensure(sizeof(unsigned long long) * nkeys * rep);
+    hip_check(hipMemsetAsync(sup_rep.p, 0, sizeof(int32_t) * nkeys * rep, s), "memset supports");
+    hip_check(hipMemsetAsync(proj_rep.p, 0, sizeof(unsigned long long) * nkeys * rep, s), "memset embeddings");
     hip_check(hipMemsetAsync(counters.p, 0, 64, s), "memset counters");
     uint32_t fcap = (uint32_t)std::min<int64_t>(nkeys, std::max<int64_t>(1 << 20, P * 64));
-    freq.ensure(sizeof(unsigned long long) * fcap);
-    a.grow = root ? nullptr : grow_cur->as<int64_t>();
+    freq.ensure(3 * sizeof(unsigned long long) * fcap);
+    a.gstart = root ? nullptr : gs_cur->as<int64_t>();
+    a.gcount = root ? nullptr : gc_cur->as<int64_t>();
     a.pid = root ? nullptr : pid_cur->as<int32_t>();
     a.vt = root ? nullptr : vt_cur->as<uint8_t>();
     a.S = S;
     a.meta = root ? nullptr : meta_d.as<PatMeta>();
     a.K = K;
     a.sup = sup.as<int32_t>();
+    a.nkeys = nkeys;
+    a.sup_rep = sup_rep.as<int32_t>();
+    a.proj_rep = proj_rep.as<unsigned long long>();
+    a.rep = rep;
     a.freq = freq.as<unsigned long long>();
     a.freq_cap = fcap;
+    // workgroup size: enough lanes for the average graph's rows (or vertices)
+    {
+      const double per_graph =
+          root ? (double)db.n_vertices / std::max(1, G) : (double)n_rows / (double)std::max(1, G);
+      a.block = per_graph > 128 ? 256 : per_graph > 64 ? 128 : 64;
+    }
 
     // 32-bit dedup keys: launch over pattern ranges with range * K < 2^32
     const int64_t chunk = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
@@ -355,10 +442,13 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       hip_check(launch_support(a, G, root, s), "k_support");
     }
     hip_check(hipEventRecord(ev1, s), "hipEventRecord");
+    hip_check(launch_reduce(a, root, s), "k_reduce");
     struct {
       uint32_t fcount, pad;
       unsigned long long ecand;
       int32_t err, pad2;
+      unsigned long long splits;
+      unsigned long long cursor;
     } cnt;
     hip_check(hipMemcpyAsync(&cnt, counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s), "download counters");
     hip_check(hipStreamSynchronize(s), "k_support");
@@ -369,32 +459,32 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       ext_launches++;
       st.e_cand += cnt.ecand;
     }
+    st.split_passes += cnt.splits;
     if (cnt.err) throw Error(GM_E_UNSUPPORTED, "per-graph deduplication table overflow");
     if (cnt.fcount > fcap)
       throw Error(GM_E_OOM, "frequent-key list overflow (" + std::to_string(cnt.fcount) + " keys)");
-    std::vector<unsigned long long> keys;
-    download(keys, freq.p, cnt.fcount, s);
-    std::sort(keys.begin(), keys.end());
-    std::vector<int32_t> sups;
-    if (!keys.empty()) {
-      upload(idx_d, keys, s);
-      val_d.ensure(sizeof(int32_t) * keys.size());
-      hip_check(launch_gather_i32(sup.as<int32_t>(), idx_d.as<uint64_t>(), (int64_t)keys.size(), val_d.as<int32_t>(), s),
-                "gather");
-      download(sups, val_d.p, keys.size(), s);
+    struct Freq {
+      unsigned long long key, sup, proj;
+    };
+    std::vector<Freq> fr(cnt.fcount);
+    if (cnt.fcount) {
+      hip_check(hipMemcpyAsync(fr.data(), freq.p, sizeof(Freq) * cnt.fcount, hipMemcpyDeviceToHost, s),
+                "download frequent keys");
+      hip_check(hipStreamSynchronize(s), "download frequent keys");
     }
+    std::sort(fr.begin(), fr.end(), [](const Freq &x, const Freq &y) { return x.key < y.key; });
     dev_sec += tdev.sec();
 
     // decode frequent keys into child codes; keep the minimal ones
     Timer tcan;
-    const int64_t F = (int64_t)keys.size();
+    const int64_t F = (int64_t)fr.size();
     std::vector<Pat> kids(F);
     std::vector<char> keep(F, 0);
     bool too_big = false;
-    parallel_for(F, threads, [&](int64_t i) {
-      const uint64_t key = keys[i];
+    Pool::get().run(F, threads, [&](int64_t i) {
+      const uint64_t key = fr[i].key;
       Pat &c = kids[i];
-      c.sup = (uint32_t)sups[i];
+      c.sup = (uint32_t)fr[i].sup;
       if (root) {
         const int64_t j = key / K, x = key % K;
         const int fl = (int)(x / (NEL * NVL)), el = (int)((x / NVL) % NEL), tl = (int)(x % NVL);
@@ -434,17 +524,21 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     std::vector<Pat> next;
     std::vector<unsigned long long> sel_keys;
     std::vector<int32_t> sel_vals;
+    std::vector<uint64_t> nproj;
+    int64_t child_rows = 0;
     for (int64_t i = 0; i < F; ++i) {
       if (!keep[i]) continue;
-      sel_keys.push_back(keys[i]);
+      sel_keys.push_back(fr[i].key);
       sel_vals.push_back(-((int32_t)next.size() + 1));
+      nproj.push_back(fr[i].proj);
+      child_rows += (int64_t)fr[i].proj;
       next.push_back(std::move(kids[i]));
     }
     const int64_t C = (int64_t)next.size();
-    std::vector<uint64_t> nproj(C, 0);
-    int64_t child_rows = 0;
     int CS = 4;
     if (C > 0) {
+      // The embedding counts size the child level exactly: one write kernel
+      // reserves each graph's rows from a cursor and fills them.
       Timer tdev2;
       int max_nv = 0;
       for (const Pat &c : next) max_nv = std::max(max_nv, (int)c.vlab.size());
@@ -452,33 +546,28 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       upload(idx_d, sel_keys, s);
       upload(val_d, sel_vals, s);
       hip_check(launch_scatter_i32(sup.as<int32_t>(), idx_d.as<uint64_t>(), val_d.as<int32_t>(), C, s), "scatter");
-      hip_check(launch_children(a, G, root, false, s), "k_children(count)");
-      hip_check(launch_exclusive_scan(ccount.as<int64_t>(), G, grow_nxt->as<int64_t>(), scan_tmp.as<int64_t>(), s),
-                "scan");
-      int64_t tot = 0;
-      hip_check(hipMemcpyAsync(&tot, grow_nxt->as<int64_t>() + G, sizeof(int64_t), hipMemcpyDeviceToHost, s),
-                "download total");
-      hip_check(hipStreamSynchronize(s), "k_children(count)");
-      child_rows = tot;
-      pid_nxt->ensure(sizeof(int32_t) * std::max<int64_t>(tot, 1));
-      vt_nxt->ensure((size_t)CS * std::max<int64_t>(tot, 1));
-      a.cgrow = grow_nxt->as<int64_t>();
+      pid_nxt->ensure(sizeof(int32_t) * std::max<int64_t>(child_rows, 1));
+      vt_nxt->ensure((size_t)CS * std::max<int64_t>(child_rows, 1));
+      rowcnt.ensure(sizeof(uint16_t) * std::max<int64_t>(root ? db.n_vertices : n_rows, 1));
+      a.rowcnt = rowcnt.as<uint16_t>();
+      a.ccap = child_rows;
+      a.cgstart = gs_nxt->as<int64_t>();
+      a.cgcount = gc_nxt->as<int64_t>();
       a.cpid = pid_nxt->as<int32_t>();
       a.cvt = vt_nxt->as<uint8_t>();
       a.CS = CS;
-      hip_check(launch_children(a, G, root, true, s), "k_children(write)");
-      if (opt.collect_proj) {
-        hist_d.ensure(sizeof(unsigned long long) * C);
-        hip_check(hipMemsetAsync(hist_d.p, 0, sizeof(unsigned long long) * C, s), "memset hist");
-        hip_check(launch_pattern_hist(pid_nxt->as<int32_t>(), tot, hist_d.as<unsigned long long>(), s), "hist");
-        download(nproj, hist_d.p, C, s);
-      } else {
-        hip_check(hipStreamSynchronize(s), "k_children(write)");
-      }
+      hip_check(launch_write(a, G, root, s), "k_write");
+      hip_check(hipMemcpyAsync(&cnt, counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s), "download counters");
+      hip_check(hipStreamSynchronize(s), "k_write");
+      if (cnt.err || cnt.cursor != (unsigned long long)child_rows)
+        throw Error(GM_E_DEVICE, "child rows written (" + std::to_string(cnt.cursor) +
+                                     ") differ from the embedding count (" + std::to_string(child_rows) + ")");
       dev_sec += tdev2.sec();
-      size_t live = db.bytes + sup.cap + pid_a.cap + pid_b.cap + vt_a.cap + vt_b.cap + freq.cap + meta_d.cap;
+      size_t live = db.bytes + sup.cap + sup_rep.cap + proj_rep.cap + pid_a.cap + pid_b.cap + vt_a.cap + vt_b.cap +
+                    freq.cap + meta_d.cap + rowcnt.cap;
       peak = std::max(peak, live);
     }
+    if (!opt.collect_proj) std::fill(nproj.begin(), nproj.end(), 0);
     if (root)
       st.e_root += child_rows;
     else
@@ -490,7 +579,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     st.n_levels++;
     if (stopped) break;
     pats.swap(next);
-    std::swap(grow_cur, grow_nxt);
+    std::swap(gs_cur, gs_nxt);
+    std::swap(gc_cur, gc_nxt);
     std::swap(pid_cur, pid_nxt);
     std::swap(vt_cur, vt_nxt);
     S = CS;

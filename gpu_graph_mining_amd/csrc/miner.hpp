@@ -32,6 +32,7 @@ struct DevBuf {
 struct DeviceDB {
   int32_t n_graphs = 0, n_jobs = 1;
   int64_t n_vertices = 0, n_adj = 0;
+  int32_t max_graph_v = 0, max_graph_adj = 0;  // largest graph: vertices, adjacency entries
   int32_t NVL = 0, NEL = 0;
   std::vector<int32_t> vl_values, el_values;  // dense label -> original
   std::vector<int32_t> job_first_graph;       // n_jobs + 1
@@ -41,6 +42,7 @@ struct DeviceDB {
 
 // Host image of the device layout (device.hpp).
 struct PackedDB {
+  int32_t max_graph_v = 0, max_graph_adj = 0;
   std::vector<int32_t> gvbase, vadj, gjob;
   std::vector<uint32_t> adj;
   std::vector<uint8_t> vlab;

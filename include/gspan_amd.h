@@ -54,9 +54,9 @@ typedef struct gm_cfg {
   int32_t label_style;      /* GM_LABELS_FULL (default) or GM_LABELS_WILDCARD */
   int32_t max_edges;        /* stop growing patterns beyond this many edges; 0 = unbounded */
   int32_t canon_threads;    /* host threads for the canonical check; 0 = auto */
-  int32_t block_threads;    /* threads per workgroup of the level kernels; 0 = auto (256) */
+  int32_t block_threads;    /* reserved: the level kernels run one wave64 per graph */
   int64_t mem_budget_bytes; /* device bytes the embedding store may use; 0 = auto */
-  int32_t collect_proj;     /* 1 = also report |Proj(P)| per pattern (extra pass) */
+  int32_t collect_proj;     /* 1 = also report |Proj(P)| per pattern (no extra pass) */
   int32_t reserved[7];
 } gm_cfg;
 
@@ -74,7 +74,8 @@ typedef struct gm_stats {
   double seconds_ext_kernel; /* time inside the extension/support kernel  */
   uint64_t ext_kernel_launches;
   uint64_t peak_device_bytes;
-  uint64_t reserved[6];
+  uint64_t split_passes;   /* per-graph dedup passes redone in smaller parts */
+  uint64_t reserved[5];
 } gm_stats;
 
 /* Pattern callback: n_edges rows of (from, to, fromlabel, elabel, tolabel),
@@ -159,6 +160,18 @@ int gm_mine(gm_db *db, int32_t abs_minsup, const gm_cfg *cfg, gm_pattern_cb cb, 
  * per process); it batches many gm_mine calls. */
 int gm_mine_batch(int32_t n_jobs, gm_db *const *dbs, const int32_t *abs_minsup, const gm_cfg *cfg,
                   gm_job_pattern_cb cb, void *user, gm_stats *stats);
+
+/* A resident batch: the databases of n_jobs jobs packed into one device
+ * layout once, so that repeated batched mining (or serving) does not upload
+ * them again.  No reference counterpart (the reference mines one database
+ * per process). */
+typedef struct gm_batch gm_batch;
+int gm_batch_create(int32_t n_jobs, gm_db *const *dbs, gm_batch **out);
+/* Mine every job of the batch in one schedule with per-job min supports;
+ * identical per-job results to gm_mine on each database. */
+int gm_batch_mine(gm_batch *batch, const int32_t *abs_minsup, const gm_cfg *cfg, gm_job_pattern_cb cb, void *user,
+                  gm_stats *stats);
+void gm_batch_destroy(gm_batch *batch);
 
 /* ---- canonical form ---------------------------------------------------- */
 /* Minimum-DFS-code check of one code (n rows of from,to,fromlabel,elabel,
