@@ -105,6 +105,30 @@ def _batch_section(n_jobs: int, out: dict) -> None:
                     "batched_patterns": int(st["n_patterns"])}
 
 
+def _copy_calibration(mib: int) -> None:
+    """One device-to-device hipMemcpy of mib MiB, which the HIP runtime runs
+    as a blit kernel (__amd_rocclr_copyBuffer) that reads and writes exactly
+    mib MiB: its FETCH_SIZE and WRITE_SIZE readings give the counters' scale
+    on this device."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, ctypes.c_size_t(n)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3) != 0 or hip.hipDeviceSynchronize() != 0:  # 3: D2D
+            raise RuntimeError("hipMemcpy failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    print(f"# copy calibration: {mib} MiB read + {mib} MiB written by one device-to-device copy", file=sys.stderr,
+          flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +137,9 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--jobs", type=int, default=0, help="also run the first N config-3 jobs batched vs sequential")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, make one device-to-device copy of MIB MiB (a blit kernel that reads and "
+                         "writes MIB MiB): the reference that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single GPU; "
@@ -175,6 +202,8 @@ def main() -> int:
         "canon_seconds_per_step": canon_s / args.steps,
         "ext_kernel_seconds_per_step": ext_s / args.steps,
         "split_passes": st["split_passes"],
+        "max_graph_rows": st["max_graph_rows"],
+        "max_table_fill": st["max_table_fill"],
     }
 
     roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
@@ -208,6 +237,8 @@ def main() -> int:
     line["roofline"] = roof
     if args.jobs:
         _batch_section(args.jobs, line)
+    if args.calibrate_copy:
+        _copy_calibration(args.calibrate_copy)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -4,11 +4,16 @@
 // support question ("how many distinct graphs have an embedding of P with
 // extension x") is answered by deduplicating (pattern, key) pairs inside the
 // workgroup (an LDS hash table) and adding 1 per distinct pair to a global
-// counter.  The same enumeration routine drives two kernels:
-//   k_support  distinct-graph support and embedding count of every
-//              (pattern, key)
-//   k_write    the child rows (pattern id + tuple) of the selected keys
-// and, for the first level, the same two over 1-edge keys.
+// counter.  Two kernels per level:
+//   k_support  enumerates every extension candidate of every row once:
+//              distinct-graph support, embedding count and child candidate
+//              bound per (pattern, key), plus each row's candidate list;
+//   k_children the child rows (pattern id + tuple) of the selected keys,
+//              read from the candidate lists (or, for a graph whose lists
+//              were not kept, from a fresh enumeration): a count pass, a
+//              one-workgroup scan (k_scan) and a write pass, which also lay
+//              out the next level's candidate lists.
+// The first level runs the same two kernels over 1-edge keys.
 //
 // Extension rules (pattern side precomputed in PatMeta, graph side read
 // from the packed adjacency):
@@ -27,6 +32,17 @@
 // pattern-level fact (bwd_mask) and "vertex not in the embedding" is a
 // 256-bit mask over the graph's local ids built from the tuple.
 //
+// Candidate bound of a row: every candidate of an embedding is one
+// adjacency entry of its rightmost vertex or of a rightmost-path source, so
+// bound = deg(rightmost) + sum of deg(rightmost-path sources).  The
+// enumeration derives it for each child from the parent's path:
+//   backward           the parent's bound (same path, same rightmost);
+//   pure fwd to w      parent sources + parent rightmost, plus deg(w);
+//   fwd from source i  sources i..root, plus deg(w);
+//   root edge (u, w)   deg(u) + deg(w).
+// The count pass sums the bounds of a graph's children; the scan turns the
+// sums into the regions that the next k_support fills with their lists.
+//
 // Register discipline: a row's vertex tuple lives in a per-thread LDS slot
 // (byte reads), the visited mask in four 64-bit values selected
 // arithmetically, and the graph view is specialised for LDS or global
@@ -42,8 +58,9 @@ namespace gm {
 
 namespace {
 
-constexpr int kSlotWords = 9;  // per-thread tuple slot: 36 bytes (odd word stride)
 constexpr uint32_t kEmpty32 = 0xFFFFFFFFu;
+constexpr int kPackShift = 40;  // packed scans: count << 40 | bound sum
+constexpr long long kPackMask = (1ll << kPackShift) - 1;
 
 // Graph adjacency: entry offsets vadj[v] are absolute; entries A[a - aoff].
 struct GView {
@@ -52,6 +69,7 @@ struct GView {
   int32_t aoff;
   __device__ __forceinline__ int32_t begin(int v) const { return vadj[v]; }
   __device__ __forceinline__ int32_t end(int v) const { return vadj[v + 1]; }
+  __device__ __forceinline__ int deg(int v) const { return vadj[v + 1] - vadj[v]; }
   __device__ __forceinline__ uint32_t entry(int32_t a) const { return A[a - aoff]; }
 };
 
@@ -96,70 +114,111 @@ __device__ __forceinline__ void store_slot(uint8_t *cvt, int64_t pos, int CS, co
   for (int i = 0; i < nw; ++i) p[i] = slot[i];
 }
 
-// Enumerate every valid extension of the embedding in `slot`.
-// emit(key, w): extension key (see KeySpace) and new vertex (-1 backward).
+// Copy the pattern's rightmost-path words into this thread's LDS slot
+// (rounded up to a multiple of 4 words; 16-byte loads, all independent).
+__device__ __forceinline__ void stage_rm(const PatMeta *__restrict__ pm, uint32_t *rmv) {
+  const int nrm = pm->nrm;
+  const uint4 *src = reinterpret_cast<const uint4 *>(pm->rm);
+  for (int i = 0; i < nrm; i += 4) {
+    const uint4 v = src[i >> 2];
+    rmv[i] = v.x;
+    rmv[i + 1] = v.y;
+    rmv[i + 2] = v.z;
+    rmv[i + 3] = v.w;
+  }
+}
+
+// Enumerate every valid extension of the embedding in `slot` (tuple bytes;
+// the pattern's rightmost path staged at rmv by stage_rm).
+// emit(key, w, bound): extension key, new vertex (-1 backward) and the
+// child's candidate bound.
 template <class F>
 __device__ __forceinline__ void enumerate_row(const GView &G, const PatMeta *__restrict__ pm, const uint32_t *slot,
-                                              int S, int NEL, int NVL, F &&emit) {
-  const int nv = pm->nv;
+                                              const uint32_t *rmv, int S, int NEL, int NVL, F &&emit) {
+  const uint4 head = *reinterpret_cast<const uint4 *>(pm);
+  const int nv = (int)(head.y & 0xFF);
+  const int nrm = (int)((head.y >> 8) & 0xFF);
+  const int minlabel = (int)((head.y >> 16) & 0xFF);
+  const int maxlabel = (int)(head.y >> 24);
+  const uint32_t bwd_mask = head.z;
   const int maxtoc = nv - 1;
-  const int minlabel = pm->minlabel;
-  const int maxlabel = pm->maxlabel;
-  const int nrm = pm->nrm;
-  const uint32_t bwd_mask = pm->bwd_mask;
   const int fwd_base = S * NEL;
   VSet vis;
   for (int i = 0; i < nv; ++i) vis.add(slot_get(slot, i));
   const int rv = slot_get(slot, maxtoc);
+  int rm_deg = 0;  // degrees of the rightmost-path sources
+  for (int i = 0; i < nrm; ++i) rm_deg += G.deg(slot_get(slot, rmv[i] & 0xFF));
+  const int own = rm_deg + G.deg(rv);  // this row's own bound
 
   // backward and pure forward: one sweep over the rightmost vertex's edges
   for (int32_t a = G.begin(rv), ae = G.end(rv); a < ae; ++a) {
     const uint32_t e = G.entry(a);
     const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
     if (!vis.has(w)) {
-      if (nl >= minlabel) emit(fwd_base + (maxtoc * NEL + el) * NVL + nl, w);
+      if (nl >= minlabel) emit(fwd_base + (maxtoc * NEL + el) * NVL + nl, w, own + G.deg(w));
     } else {
       for (int i = 1; i < nrm; ++i) {
-        const uint32_t rme = pm->rm[i];
+        const uint32_t rme = rmv[i];
         const int f = rme & 0xFF;
         if (slot_get(slot, f) != w) continue;
         const int e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
-        if (!((bwd_mask >> f) & 1u) && (e1 < el || (e1 == el && l1 <= maxlabel))) emit(f * NEL + el, -1);
+        if (!((bwd_mask >> f) & 1u) && (e1 < el || (e1 == el && l1 <= maxlabel))) emit(f * NEL + el, -1, own);
         break;
       }
     }
   }
   // forward from the rightmost path
+  int below = 0;  // degrees of the sources deeper than i
   for (int i = 0; i < nrm; ++i) {
-    const uint32_t rme = pm->rm[i];
+    const uint32_t rme = rmv[i];
     const int f = rme & 0xFF;
     const int u = slot_get(slot, f);
     const int e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
+    const int keep = rm_deg - below;  // sources i .. root
     for (int32_t a = G.begin(u), ae = G.end(u); a < ae; ++a) {
       const uint32_t e = G.entry(a);
       const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
       if (vis.has(w) || nl < minlabel) continue;
-      if (e1 < el || (e1 == el && l1 <= nl)) emit(fwd_base + (f * NEL + el) * NVL + nl, w);
+      if (e1 < el || (e1 == el && l1 <= nl)) emit(fwd_base + (f * NEL + el) * NVL + nl, w, keep + G.deg(w));
     }
+    below += G.deg(u);
   }
 }
 
 // Root extensions of vertex u (1-edge codes with fromlabel <= tolabel).
 template <class F>
 __device__ __forceinline__ void enumerate_root(const GView &G, int u, int lu, int NEL, int NVL, F &&emit) {
+  const int du = G.deg(u);
   for (int32_t a = G.begin(u), ae = G.end(u); a < ae; ++a) {
     const uint32_t e = G.entry(a);
     const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
-    if (lu <= nl) emit((lu * NEL + el) * NVL + nl, w);
+    if (lu <= nl) emit((lu * NEL + el) * NVL + nl, w, du + G.deg(w));
   }
 }
 
-// Dynamic LDS layout of the level kernels (bytes):
-//   [0, 8*2^tb)                dedup table: keys, then candidate counts
-//                              (k_support only)
-//   then kBlock * 36           per-thread tuple slots
-//   then 4*stage_v             staged vertex offsets
-//   then 4*stage_a             staged adjacency entries
+// Candidate list entries: {code, child bound}.
+//   root codes   key | w << 24          (key < 255^3 < 2^24)
+//   later codes  key | (w + 1) << 23    (key < 32 * 255 * 256 < 2^23,
+//                                        w = -1 for backward edges)
+__device__ __forceinline__ uint32_t cand_code(bool root, int key, int w) {
+  return root ? ((uint32_t)key | ((uint32_t)w << 24)) : ((uint32_t)key | ((uint32_t)(w + 1) << 23));
+}
+__device__ __forceinline__ void cand_decode(bool root, uint32_t c, int &key, int &w) {
+  if (root) {
+    key = (int)(c & 0xFFFFFFu);
+    w = (int)(c >> 24);
+  } else {
+    key = (int)(c & 0x7FFFFFu);
+    w = (int)(c >> 23) - 1;
+  }
+}
+
+// Dynamic LDS layout of the level kernels (32-bit words):
+//   [0, 2 * 2^tb)          dedup table (k_support): 2^tb keys, 2^tb counts
+//   blockDim * slot_words  per-thread slots: the row's tuple (tuple_words)
+//                          then its rightmost path
+//   stage_v                staged vertex offsets
+//   stage_a                staged adjacency entries
 struct Lds {
   uint32_t *table;
   uint32_t *slots;
@@ -167,14 +226,14 @@ struct Lds {
   uint32_t *adj;
 };
 
-__device__ __forceinline__ Lds lds_layout(const LevelArgs &a, bool with_table) {
+__device__ __forceinline__ Lds lds_layout(const LevelArgs &a, bool support) {
   extern __shared__ uint32_t dyn[];
   Lds l;
   uint32_t *p = dyn;
   l.table = p;
-  if (with_table) p += (2u << a.table_bits);
+  if (support) p += 2u << a.table_bits;
   l.slots = p;
-  p += blockDim.x * kSlotWords;
+  p += blockDim.x * a.slot_words;
   l.vadj = reinterpret_cast<int32_t *>(p);
   p += a.stage_v;
   l.adj = p;
@@ -207,146 +266,48 @@ __device__ __forceinline__ GView graph_view(const LevelArgs &a, int g, const Lds
   return G;
 }
 
-// Block-wide exclusive scan of one int per thread (wave64 shuffles + LDS).
-__device__ __forceinline__ int block_exclusive_scan(int x, int &total, int *s_waves) {
+// Block-wide exclusive scan of one value per thread (wave64 shuffles + LDS;
+// s holds blockDim / 64 + 1 values).
+template <class T>
+__device__ __forceinline__ T block_exclusive_scan(T x, T &total, T *s) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  int v = x;
+  T v = x;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    int y = __shfl_up(v, o, 64);
+    T y = __shfl_up(v, o, 64);
     if (lane >= o) v += y;
   }
-  if (lane == 63) s_waves[wid] = v;
+  if (lane == 63) s[wid] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-    int acc = 0;
+    T acc = 0;
     for (int i = 0; i < nw; ++i) {
-      int t = s_waves[i];
-      s_waves[i] = acc;
+      T t = s[i];
+      s[i] = acc;
       acc += t;
     }
-    s_waves[nw] = acc;
+    s[nw] = acc;
   }
   __syncthreads();
-  int r = s_waves[wid] + v - x;
-  total = s_waves[nw];
+  T r = s[wid] + v - x;
+  total = s[nw];
   __syncthreads();
   return r;
 }
 
-__device__ __forceinline__ uint32_t hash_slot32(uint32_t k, int bits) { return (k * 0x9E3779B1u) >> (32 - bits); }
-
-// ---------------------------------------------------------------- SUPPORT
-// One pass per residue class of pattern ids (p mod `mod` == `res`; root
-// level: of the source vertex label): every candidate key
-// (p - p_lo) * K + key is inserted into the LDS hash table, whose second
-// half counts the candidates per key; when the table passes 3/4 full the
-// pass is abandoned and the class split in two (all keys of one pattern stay
-// in one class, so the per-graph dedup stays exact).  A completed pass adds
-// 1 per distinct key to the distinct-graph counters and the key's candidate
-// count to the embedding counters of the graph's replica
-// (replica = workgroup id mod a.rep; k_reduce sums the replicas).
-template <bool ROOT, bool LDS>
-__device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, const Lds &l,
-                                             int *s_misc, uint32_t *s_stack_mod, uint32_t *s_stack_res) {
-  const GView G = graph_view<LDS>(a, g, l);
-  const int job = ROOT ? a.gjob[g] : 0;
-  const int32_t vb = a.gvbase[g];
-  const int tb = a.table_bits;
-  const uint32_t cap = 1u << tb, fill_limit = cap - cap / 4;
-  uint32_t *table = l.table;
-  uint32_t *tcnt = l.table + cap;
-  uint32_t *slot = l.slots + threadIdx.x * kSlotWords;
-  const int64_t rbase = (int64_t)(g & (a.rep - 1)) * a.nkeys;
-  int32_t *rep = a.sup_rep + rbase;
-  unsigned long long *repc = a.proj_rep + rbase;
-  int &s_fill = s_misc[0], &s_over = s_misc[1], &s_top = s_misc[2];
-  unsigned long long my_cand = 0;
-  if (threadIdx.x == 0) {
-    s_top = 1;
-    s_stack_mod[0] = 1;
-    s_stack_res[0] = 0;
-  }
-  for (;;) {
-    __syncthreads();
-    if (s_top == 0) break;
-    const uint32_t mod = s_stack_mod[s_top - 1], res = s_stack_res[s_top - 1];
-    for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
-      table[i] = kEmpty32;
-      tcnt[i] = 0;
-    }
-    if (threadIdx.x == 0) {
-      s_fill = 0;
-      s_over = 0;
-    }
-    __syncthreads();
-    unsigned long long cand = 0;
-    int over = 0;  // this thread saw the table pass its fill limit
-    auto insert = [&](uint32_t k) {
-      ++cand;
-      uint32_t h = hash_slot32(k, tb);
-      for (uint32_t probe = 0; probe < cap; ++probe) {
-        const uint32_t old = atomicCAS(&table[h], kEmpty32, k);
-        if (old == kEmpty32 || old == k) {
-          atomicAdd(&tcnt[h], 1u);
-          if (old == kEmpty32 && (uint32_t)atomicAdd(&s_fill, 1) >= fill_limit) over = 1;
-          return;
-        }
-        h = (h + 1) & (cap - 1);
-      }
-      over = 1;
-    };
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-      if (ROOT) {
-        const int u = (int)r;
-        const int lu = a.vlab[vb + u];
-        if (mod > 1 && ((uint32_t)lu & (mod - 1)) != res) continue;
-        const uint32_t base = (uint32_t)((int64_t)(job - a.p_lo) * a.K);
-        enumerate_root(G, u, lu, a.NEL, a.NVL, [&](int key, int) { insert(base + (uint32_t)key); });
-      } else {
-        const int32_t p = a.pid[r];
-        if (p < a.p_lo || p >= a.p_hi) continue;
-        if (mod > 1 && ((uint32_t)p & (mod - 1)) != res) continue;
-        load_slot(slot, a.vt, r, a.S);
-        const uint32_t base = (uint32_t)((int64_t)(p - a.p_lo) * a.K);
-        enumerate_row(G, a.meta + p, slot, a.S, a.NEL, a.NVL, [&](int key, int) { insert(base + (uint32_t)key); });
-      }
-    }
-    if (over) atomicOr(&s_over, 1);
-    __syncthreads();
-    const bool failed = s_over != 0;
-    __syncthreads();
-    if (failed) {
-      if (threadIdx.x == 0) {
-        atomicAdd(a.split_passes, 1ull);
-        if (mod >= (1u << 30) || s_top + 1 > 32) {
-          atomicOr(a.error_flag, 1);
-          s_top = 0;
-        } else {
-          s_stack_mod[s_top - 1] = mod * 2;
-          s_stack_res[s_top - 1] = res + mod;
-          s_stack_mod[s_top] = mod * 2;
-          s_stack_res[s_top] = res;
-          s_top += 1;
-        }
-      }
-      continue;
-    }
-    my_cand += cand;
-    for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) {
-      const uint32_t k = table[i];
-      if (k == kEmpty32) continue;
-      const int64_t gk = (int64_t)a.p_lo * a.K + (int64_t)k;
-      atomicAdd(&rep[gk], 1);
-      atomicAdd(&repc[gk], (unsigned long long)tcnt[i]);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) s_top -= 1;
-  }
-  // candidate counter (E_cand): one atomic per wave
-  for (int o = 32; o > 0; o >>= 1) my_cand += __shfl_down(my_cand, o, 64);
-  if ((threadIdx.x & 63) == 0 && my_cand) atomicAdd(a.ecand, my_cand);
+// Block-wide sum (the result in every thread; s holds blockDim / 64 values).
+template <class T>
+__device__ __forceinline__ T block_sum(T x, T *s) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = x;
+  __syncthreads();
+  T t = 0;
+  for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) t += s[i];
+  __syncthreads();
+  return t;
 }
+
+__device__ __forceinline__ uint32_t hash_slot32(uint32_t k, int bits) { return (k * 0x9E3779B1u) >> (32 - bits); }
 
 // Rows of graph g at the current level (root level: its vertices).
 template <bool ROOT>
@@ -360,10 +321,234 @@ __device__ __forceinline__ void graph_rows(const LevelArgs &a, int g, int64_t &r
   }
 }
 
+// ---------------------------------------------------------------- SUPPORT
+// One lane per row: the lane stages the row's tuple and rightmost path in
+// its LDS slot, enumerates the row's extensions (enumerate_row) and inserts
+// each candidate's key (p - p_lo) * K + key into the LDS hash table, which
+// counts the candidates per key.  With candidate lists on, the lane also
+// appends each candidate to the row's list (its region holds the row's
+// bound, the most candidates the row can have).
+//
+// Passes and splits: one pass per residue class of pattern ids (root level:
+// of the vertex label).  When the table passes 3/4 full the pass is
+// abandoned and the class split in two; all keys of one pattern stay in one
+// class, so the per-graph dedup stays exact.  The first pass writes every
+// row's list even when it fails, so with lists on the split passes read the
+// lists back instead of enumerating again.  A completed pass adds
+// (1 << 40 | count) per distinct key to the graph's replica of the packed
+// counters (replica = workgroup id mod a.rep; k_reduce sums the replicas).
+
+constexpr int kSupShift = 40;  // packed counters: support << 40 | embeddings
+
+struct SupportShared {
+  int pre[kMaxBlock];  // list replay: prefix of the chunk's list lengths
+  int ired[5];
+  unsigned long long red[5];
+  int fill, over, top, replay;
+  uint32_t stack_mod[32], stack_res[32];
+};
+
+// Largest k in [0, n) with pre[k] <= j (pre non-decreasing, pre[0] = 0).
+__device__ __forceinline__ int find_row(const int *pre, int j, int n) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= j) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <bool ROOT, bool LDS>
+__device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, const Lds &l,
+                                             SupportShared &sh) {
+  const GView G = graph_view<LDS>(a, g, l);
+  const int job = ROOT ? a.gjob[g] : 0;
+  const int32_t vb = a.gvbase[g];
+  uint2 *lists = nullptr;
+  if (a.use_lists) {
+    if (ROOT) {
+      lists = a.cand;
+    } else {
+      const long long b = a.gcbase[g];
+      if (b >= 0) lists = a.cand + b;
+    }
+  }
+  const int B = blockDim.x;
+  const int tb = a.table_bits;
+  const uint32_t cap = 1u << tb, fill_limit = cap - cap / 4;
+  uint32_t *table = l.table;
+  uint32_t *tcnt = l.table + cap;
+  uint32_t *slot = l.slots + threadIdx.x * a.slot_words;
+  uint32_t *rmv = slot + a.tuple_words;
+  unsigned long long *rep = a.acc_rep + (int64_t)(g & (a.rep - 1)) * a.nkeys;
+  unsigned long long my_cand = 0, my_bound = 0;
+  if (threadIdx.x == 0) {
+    sh.top = 1;
+    sh.stack_mod[0] = 1;
+    sh.stack_res[0] = 0;
+    sh.replay = 0;
+  }
+  for (;;) {
+    __syncthreads();
+    if (sh.top == 0) break;
+    const uint32_t mod = sh.stack_mod[sh.top - 1], res = sh.stack_res[sh.top - 1];
+    for (uint32_t i = threadIdx.x; i < cap; i += B) {
+      table[i] = kEmpty32;
+      tcnt[i] = 0;
+    }
+    if (threadIdx.x == 0) {
+      sh.fill = 0;
+      sh.over = 0;
+    }
+    __syncthreads();
+    unsigned long long cand = 0, bound_sum = 0;
+    int over = 0;  // this thread saw the table pass its fill limit
+    // once the pass has failed (flag published in LDS) nothing more is
+    // inserted: a nearly full table would make every probe sequence long
+    volatile int *failed_flag = &sh.over;
+    auto insert = [&](uint32_t k) {
+      if (over || *failed_flag) return;
+      uint32_t h = hash_slot32(k, tb);
+      for (uint32_t probe = 0; probe < cap; ++probe) {
+        const uint32_t old = atomicCAS(&table[h], kEmpty32, k);
+        if (old == kEmpty32 || old == k) {
+          atomicAdd(&tcnt[h], 1u);
+          if (old == kEmpty32 && (uint32_t)atomicAdd(&sh.fill, 1) >= fill_limit) {
+            over = 1;
+            *failed_flag = 1;
+          }
+          return;
+        }
+        h = (h + 1) & (cap - 1);
+      }
+      over = 1;
+      *failed_flag = 1;
+    };
+    if (!sh.replay) {
+      for (int64_t r = r0 + threadIdx.x; r < r1; r += B) {
+        int nc = 0;
+        if (ROOT) {
+          const int u = (int)r;
+          const int lu = a.vlab[vb + u];
+          if (mod > 1 && ((uint32_t)lu & (mod - 1)) != res) continue;
+          const uint32_t base = (uint32_t)((int64_t)(job - a.p_lo) * a.K);
+          uint2 *out = lists ? lists + G.begin(u) : nullptr;
+          enumerate_root(G, u, lu, a.NEL, a.NVL, [&](int key, int w, int bound) {
+            insert(base + (uint32_t)key);
+            ++cand;
+            bound_sum += (unsigned long long)bound;
+            if (out) out[nc++] = make_uint2(cand_code(true, key, w), (uint32_t)bound);
+          });
+          if (out) a.rccnt[vb + u] = (uint16_t)nc;
+        } else {
+          const int32_t p = a.pid[r];
+          if (p < a.p_lo || p >= a.p_hi) continue;
+          if (mod > 1 && ((uint32_t)p & (mod - 1)) != res) continue;
+          const PatMeta *pm = a.meta + p;
+          load_slot(slot, a.vt, r, a.S);
+          stage_rm(pm, rmv);
+          const uint32_t base = (uint32_t)((int64_t)(p - a.p_lo) * a.K);
+          uint2 *out = lists ? lists + a.rcoff[r] : nullptr;
+          enumerate_row(G, pm, slot, rmv, a.S, a.NEL, a.NVL, [&](int key, int w, int bound) {
+            insert(base + (uint32_t)key);
+            ++cand;
+            bound_sum += (unsigned long long)bound;
+            if (out) out[nc++] = make_uint2(cand_code(false, key, w), (uint32_t)bound);
+          });
+          if (out) a.rccnt[r] = (uint16_t)nc;
+        }
+      }
+    } else {
+      // read the lists back, lane-parallel: rows blockDim at a time, each
+      // lane takes list entries found by binary search over the prefix
+      for (int64_t base = r0; base < r1; base += B) {
+        const int64_t r = base + threadIdx.x;
+        int n = 0;
+        if (r < r1) {
+          if (ROOT) {
+            const int u = (int)r;
+            const int lu = a.vlab[vb + u];
+            if (!(mod > 1 && ((uint32_t)lu & (mod - 1)) != res)) {
+              n = a.rccnt[vb + u];
+              slot[0] = (uint32_t)G.begin(u);
+              slot[1] = (uint32_t)((int64_t)(job - a.p_lo) * a.K);
+            }
+          } else {
+            const int32_t p = a.pid[r];
+            if (p >= a.p_lo && p < a.p_hi && !(mod > 1 && ((uint32_t)p & (mod - 1)) != res)) {
+              n = a.rccnt[r];
+              slot[0] = a.rcoff[r];
+              slot[1] = (uint32_t)((int64_t)(p - a.p_lo) * a.K);
+            }
+          }
+        }
+        int T;
+        sh.pre[threadIdx.x] = block_exclusive_scan<int>(n, T, sh.ired);
+        __syncthreads();
+        const int nrow = (int)min<int64_t>(B, r1 - base);
+        for (int j = threadIdx.x; j < T; j += B) {
+          const int k = find_row(sh.pre, j, nrow);
+          const uint32_t *rec = l.slots + k * a.slot_words;
+          const uint2 c = lists[rec[0] + (j - sh.pre[k])];
+          int key, w;
+          cand_decode(ROOT, c.x, key, w);
+          insert(rec[1] + (uint32_t)key);
+          ++cand;
+          bound_sum += c.y;
+        }
+        __syncthreads();  // the next chunk reuses the slots
+      }
+    }
+    if (over) atomicOr(&sh.over, 1);
+    __syncthreads();
+    const bool failed = sh.over != 0;
+    __syncthreads();
+    if (failed) {
+      if (threadIdx.x == 0) {
+        atomicAdd(a.split_passes, 1ull);
+        if (lists) sh.replay = 1;  // every row's list is complete
+        // the next level wants a larger table
+        atomicMax(&a.spread[(g & (kSpread - 1)) * kSpreadStride + kSpreadFill], 2ull * cap);
+        if (mod >= (1u << 30) || sh.top + 1 > 32) {
+          atomicOr(a.error_flag, 1);
+          sh.top = 0;
+        } else {
+          sh.stack_mod[sh.top - 1] = mod * 2;
+          sh.stack_res[sh.top - 1] = res + mod;
+          sh.stack_mod[sh.top] = mod * 2;
+          sh.stack_res[sh.top] = res;
+          sh.top += 1;
+        }
+      }
+      continue;
+    }
+    my_cand += cand;
+    my_bound += bound_sum;
+    for (uint32_t i = threadIdx.x; i < cap; i += B) {
+      const uint32_t k = table[i];
+      if (k == kEmpty32) continue;
+      atomicAdd(&rep[(int64_t)a.p_lo * a.K + (int64_t)k], (1ull << kSupShift) | (unsigned long long)tcnt[i]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      atomicMax(&a.spread[(g & (kSpread - 1)) * kSpreadStride + kSpreadFill], (unsigned long long)sh.fill);
+      sh.top -= 1;
+    }
+  }
+  // statistics: one atomic per workgroup each, spread over kSpread lines
+  my_cand = block_sum<unsigned long long>(my_cand, sh.red);
+  my_bound = block_sum<unsigned long long>(my_bound, sh.red);
+  if (threadIdx.x == 0) {
+    unsigned long long *line = a.spread + (g & (kSpread - 1)) * kSpreadStride;
+    if (my_cand) atomicAdd(&line[kSpreadCand], my_cand);
+    if (my_bound) atomicAdd(&line[kSpreadBound], my_bound);
+  }
+}
+
 template <bool ROOT>
 __global__ void __launch_bounds__(kMaxBlock) k_support(LevelArgs a) {
-  __shared__ int s_misc[4];
-  __shared__ uint32_t s_stack_mod[32], s_stack_res[32];
+  __shared__ SupportShared sh;
   const int g = blockIdx.x;
   if (ROOT) {
     const int job = a.gjob[g];
@@ -374,156 +559,329 @@ __global__ void __launch_bounds__(kMaxBlock) k_support(LevelArgs a) {
   if (r0 == r1) return;
   const Lds l = lds_layout(a, true);
   if (graph_fits(a, g))
-    support_body<ROOT, true>(a, g, r0, r1, l, s_misc, s_stack_mod, s_stack_res);
+    support_body<ROOT, true>(a, g, r0, r1, l, sh);
   else
-    support_body<ROOT, false>(a, g, r0, r1, l, s_misc, s_stack_mod, s_stack_res);
+    support_body<ROOT, false>(a, g, r0, r1, l, sh);
 }
 
-// Sum the replicas of every key; keys reaching their job's min support go to
-// the frequent list as (key, support, embeddings).
+// Sum the replicas of every key (and clear them for the next level); keys
+// reaching their job's min support go to the frequent list as
+// (key, support, embeddings).
 template <bool ROOT>
 __global__ void __launch_bounds__(256) k_reduce(LevelArgs a) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nkeys; i += (int64_t)gridDim.x * blockDim.x) {
-    int32_t s = 0;
-    unsigned long long c = 0;
+    unsigned long long v = 0;
     for (int r = 0; r < a.rep; ++r) {
-      s += a.sup_rep[(int64_t)r * a.nkeys + i];
-      c += a.proj_rep[(int64_t)r * a.nkeys + i];
+      const int64_t j = (int64_t)r * a.nkeys + i;
+      v += a.acc_rep[j];
+      a.acc_rep[j] = 0;
     }
+    const int32_t s = (int32_t)(v >> kSupShift);
+    const unsigned long long c = v & ((1ull << kSupShift) - 1);
     a.sup[i] = s;
     const int64_t p = i / a.K;
     const int32_t owner = ROOT ? (int32_t)p : a.meta[p].job;
-    if (s >= a.minsup_job[owner]) {
-      const uint32_t slot = atomicAdd(a.freq_count, 1u);
+    const bool hit = s >= a.minsup_job[owner];
+    // one atomic per wave for its frequent keys
+    const unsigned long long mask = __ballot(hit);
+    const int lane = threadIdx.x & 63;
+    uint32_t first = 0;
+    const int leader = mask ? __ffsll((long long)mask) - 1 : 0;
+    if (mask && lane == leader) first = atomicAdd(a.freq_count, (uint32_t)__popcll(mask));
+    first = __shfl(first, leader, 64);
+    if (hit) {
+      const uint32_t slot = first + (uint32_t)__popcll(mask & ((1ull << lane) - 1));
       if (slot < a.freq_cap) {
-        a.freq[3 * slot] = (unsigned long long)i;
-        a.freq[3 * slot + 1] = (unsigned long long)s;
-        a.freq[3 * slot + 2] = c;
+        unsigned long long *f = a.freq + 3 * (int64_t)slot;
+        f[0] = (unsigned long long)i;
+        f[1] = (unsigned long long)s;
+        f[2] = c;
       }
     }
   }
 }
 
 // ------------------------------------------------------------------ WRITE
-// Child rows of graph g for the selected keys (sup[key] < 0 encodes child
-// -(sup+1)): pass 1 counts each row's children into rowcnt, one atomic on
-// the level cursor reserves the graph's contiguous output segment, pass 2
-// scans the counts and writes (pattern id, tuple) rows.
-template <bool ROOT, bool LDS>
-__device__ __forceinline__ void write_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, const Lds &l,
-                                           int *s_waves, long long *s_red) {
-  const GView G = graph_view<LDS>(a, g, l);
+// Child rows of the selected keys (sup[key] < 0 encodes child -(sup+1)),
+// in three launches with no contended atomics:
+//   k_count  per graph: children and the sum of their candidate bounds
+//            (enumeration mode also keeps both per row);
+//   k_scan   one workgroup: exclusive scans of both over the graphs give each
+//            graph's first child row and its next-level list region;
+//   k_write  per graph: the rows (pattern id, tuple) and each child's list
+//            offset within its graph's region.
+// Child rows are graph-major in graph order, so the layout is the same on
+// every run.
+
+struct WriteShared {
+  long long red[5];          // block sums and scans (<= 4 waves + total)
+  int ired[5];
+  int pre[kMaxBlock];        // list mode: prefix of the chunk's list lengths
+  long long off[kMaxBlock];  // list mode: list offset of each chunk row
+};
+
+// Write one child row at `pos` (parent tuple in slot; root: vertex u).
+template <bool ROOT>
+__device__ __forceinline__ void put_child(const LevelArgs &a, int64_t pos, int32_t child, int u, int w, int nvp,
+                                          uint32_t *slot, bool keep_lists, uint32_t off) {
+  a.cpid[pos] = child;
+  if (ROOT) {
+    uint32_t *p = reinterpret_cast<uint32_t *>(a.cvt + pos * a.CS);
+    p[0] = (uint32_t)u | ((uint32_t)w << 8);
+    for (int i = 1; i < (a.CS >> 2); ++i) p[i] = 0;
+  } else {
+    if (w >= 0) slot_set(slot, nvp, w);
+    store_slot(a.cvt, pos, a.CS, slot);
+  }
+  if (keep_lists) a.nrcoff[pos] = off;
+}
+
+// List mode: the graph's candidates are processed lane-parallel.  Rows are
+// taken blockDim at a time; their list lengths are scanned into LDS and each
+// lane finds the row of its flat candidate index by binary search.
+template <bool ROOT, bool WRITE>
+__device__ __forceinline__ void children_lists(const LevelArgs &a, int g, int64_t r0, int64_t r1, const Lds &l,
+                                               WriteShared &sh) {
+  const int B = blockDim.x;
   const int job = ROOT ? a.gjob[g] : 0;
   const int32_t vb = a.gvbase[g];
-  uint32_t *slot = l.slots + threadIdx.x * kSlotWords;
-  const int64_t cbase = ROOT ? (int64_t)vb : 0;  // rowcnt index base
-  // pass 1: count
-  long long mine = 0;
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-    int cnt = 0;
-    if (ROOT) {
-      const int lu = a.vlab[vb + (int)r];
-      const uint64_t kbase = (uint64_t)job * a.K;
-      enumerate_root(G, (int)r, lu, a.NEL, a.NVL, [&](int key, int) {
-        if (a.sup[kbase + key] < 0) ++cnt;
-      });
-    } else {
-      const int32_t p = a.pid[r];
-      load_slot(slot, a.vt, r, a.S);
-      const uint64_t kbase = (uint64_t)p * (uint64_t)a.K;
-      enumerate_row(G, a.meta + p, slot, a.S, a.NEL, a.NVL, [&](int key, int) {
-        if (a.sup[kbase + key] < 0) ++cnt;
-      });
-    }
-    a.rowcnt[cbase + r] = (uint16_t)cnt;
-    mine += cnt;
-  }
-  for (int o = 32; o > 0; o >>= 1) mine += __shfl_down(mine, o, 64);
-  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = mine;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    long long total = 0;
-    for (int i = 0; i < (int)((blockDim.x + 63) >> 6); ++i) total += s_red[i];
-    long long start = 0;
-    if (total > 0) {
-      start = (long long)atomicAdd(a.cursor, (unsigned long long)total);
-      if (start + total > a.ccap) {
-        atomicOr(a.error_flag, 2);
-        total = 0;  // nothing is written; the host reports the overflow
-      }
-    }
-    a.cgstart[g] = start;
-    a.cgcount[g] = total;
-    s_red[0] = start;
-    s_red[1] = total;
-  }
-  __syncthreads();
-  const long long start = s_red[0];
-  if (s_red[1] == 0) return;
-  // pass 2: write
-  int64_t out = start;
-  for (int64_t base = r0; base < r1; base += blockDim.x) {
+  const uint2 *lists = ROOT ? a.cand : a.cand + a.gcbase[g];
+  uint32_t *slot = l.slots + threadIdx.x * a.slot_words;
+  // rows [base, base + B): list prefix and offsets into sh; returns the
+  // chunk's candidate total
+  auto chunk = [&](int64_t base) -> int {
     const int64_t r = base + threadIdx.x;
     const bool live = r < r1;
-    const int cnt = live ? (int)a.rowcnt[cbase + r] : 0;
-    int tot;
-    const int pre = block_exclusive_scan(cnt, tot, s_waves);
-    if (live && cnt) {
-      int64_t pos = out + pre;
-      if (ROOT) {
-        const int u = (int)r;
-        const int lu = a.vlab[vb + u];
-        const uint64_t kbase = (uint64_t)job * a.K;
-        enumerate_root(G, u, lu, a.NEL, a.NVL, [&](int key, int w) {
-          const int32_t v = a.sup[kbase + key];
-          if (v < 0) {
-            a.cpid[pos] = -v - 1;
-            uint32_t *p = reinterpret_cast<uint32_t *>(a.cvt + pos * a.CS);
-            p[0] = (uint32_t)u | ((uint32_t)w << 8);
-            for (int i = 1; i < (a.CS >> 2); ++i) p[i] = 0;
-            ++pos;
-          }
-        });
-      } else {
-        const int32_t p = a.pid[r];
-        const PatMeta *pm = a.meta + p;
-        const int nvp = pm->nv;
-        load_slot(slot, a.vt, r, a.S);
-        const uint64_t kbase = (uint64_t)p * (uint64_t)a.K;
-        enumerate_row(G, pm, slot, a.S, a.NEL, a.NVL, [&](int key, int w) {
-          const int32_t v = a.sup[kbase + key];
-          if (v < 0) {
-            a.cpid[pos] = -v - 1;
-            if (w >= 0) slot_set(slot, nvp, w);
-            store_slot(a.cvt, pos, a.CS, slot);
-            ++pos;
-          }
-        });
-      }
+    const int n = live ? (int)a.rccnt[ROOT ? vb + r : r] : 0;
+    int T;
+    const int pre = block_exclusive_scan<int>(n, T, sh.ired);
+    sh.pre[threadIdx.x] = pre;
+    sh.off[threadIdx.x] = live ? (ROOT ? (long long)a.vadj[vb + r] : (long long)a.rcoff[r]) : 0;
+    __syncthreads();
+    return T;
+  };
+  auto find = [&](int j, int nrow) {
+    int lo = 0, hi = nrow;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sh.pre[mid] <= j) lo = mid;
+      else hi = mid;
     }
-    out += tot;
+    return lo;
+  };
+  if (!WRITE) {
+    long long cnt = 0, bsum = 0;
+    for (int64_t base = r0; base < r1; base += B) {
+      const int T = chunk(base);
+      const int nrow = (int)min<int64_t>(B, r1 - base);
+      for (int j = threadIdx.x; j < T; j += B) {
+        const int k = find(j, nrow);
+        const uint2 c = lists[sh.off[k] + (j - sh.pre[k])];
+        int key, w;
+        cand_decode(ROOT, c.x, key, w);
+        const uint64_t kbase = ROOT ? (uint64_t)job * a.K : (uint64_t)a.pid[base + k] * (uint64_t)a.K;
+        if (a.sup[kbase + key] < 0) {
+          ++cnt;
+          bsum += c.y;
+        }
+      }
+      __syncthreads();
+    }
+    cnt = block_sum<long long>(cnt, sh.red);
+    bsum = block_sum<long long>(bsum, sh.red);
+    if (threadIdx.x == 0) {
+      a.cgcount[g] = cnt;
+      a.gbsum[g] = (unsigned long long)bsum;
+    }
+    return;
+  }
+  const long long nb = a.next_lists ? a.ngcbase[g] : -1;
+  int64_t out = a.cgstart[g];
+  long long boff = 0;
+  for (int64_t base = r0; base < r1; base += B) {
+    const int T = chunk(base);
+    const int nrow = (int)min<int64_t>(B, r1 - base);
+    for (int jb = 0; jb < T; jb += B) {
+      const int j = jb + threadIdx.x;
+      int32_t v = 0;
+      int key = 0, w = 0, k = 0;
+      uint32_t bound = 0;
+      if (j < T) {
+        k = find(j, nrow);
+        const uint2 c = lists[sh.off[k] + (j - sh.pre[k])];
+        cand_decode(ROOT, c.x, key, w);
+        bound = c.y;
+        const uint64_t kbase = ROOT ? (uint64_t)job * a.K : (uint64_t)a.pid[base + k] * (uint64_t)a.K;
+        v = a.sup[kbase + key];
+      }
+      const long long x = v < 0 ? ((1ll << kPackShift) | (long long)bound) : 0;
+      long long tot;
+      const long long pre = block_exclusive_scan<long long>(x, tot, sh.red);
+      if (v < 0) {
+        const int64_t r = base + k;
+        int nvp = 0;
+        if (!ROOT) {
+          nvp = a.meta[a.pid[r]].nv;
+          load_slot(slot, a.vt, r, a.S);
+        }
+        put_child<ROOT>(a, out + (pre >> kPackShift), -v - 1, (int)r, w, nvp, slot, nb >= 0,
+                        (uint32_t)(boff + (pre & kPackMask)));
+      }
+      out += tot >> kPackShift;
+      boff += tot & kPackMask;
+    }
+    __syncthreads();
   }
 }
 
-template <bool ROOT>
-__global__ void __launch_bounds__(kMaxBlock) k_write(LevelArgs a) {
-  __shared__ int s_waves[17];
-  __shared__ long long s_red[16];
+// Enumeration mode: rows are enumerated again (adjacency staged in LDS or
+// read from HBM), one row per lane.
+template <bool ROOT, bool LDS, bool WRITE>
+__device__ __forceinline__ void children_enum(const LevelArgs &a, int g, int64_t r0, int64_t r1, const Lds &l,
+                                              WriteShared &sh) {
+  const GView G = graph_view<LDS>(a, g, l);
+  const int job = ROOT ? a.gjob[g] : 0;
+  const int32_t vb = a.gvbase[g];
+  uint32_t *slot = l.slots + threadIdx.x * a.slot_words;
+  const int64_t cbase = ROOT ? (int64_t)vb : 0;  // rowcnt / rowbsum index base
+  uint32_t *rmv = slot + a.tuple_words;
+  // every candidate of row r (tuple already in the slot)
+  auto candidates = [&](int64_t r, auto &&f) {
+    if (ROOT) {
+      enumerate_root(G, (int)r, a.vlab[vb + (int)r], a.NEL, a.NVL, f);
+    } else {
+      const PatMeta *pm = a.meta + a.pid[r];
+      stage_rm(pm, rmv);
+      enumerate_row(G, pm, slot, rmv, a.S, a.NEL, a.NVL, f);
+    }
+  };
+  if (!WRITE) {
+    long long cnt = 0, bsum = 0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      int n = 0;
+      uint32_t b = 0;
+      const uint64_t kbase = ROOT ? (uint64_t)job * a.K : (uint64_t)a.pid[r] * (uint64_t)a.K;
+      if (!ROOT) load_slot(slot, a.vt, r, a.S);
+      candidates(r, [&](int key, int, int bound) {
+        if (a.sup[kbase + key] < 0) {
+          ++n;
+          b += (uint32_t)bound;
+        }
+      });
+      a.rowcnt[cbase + r] = (uint16_t)n;
+      a.rowbsum[cbase + r] = b;
+      cnt += n;
+      bsum += b;
+    }
+    cnt = block_sum<long long>(cnt, sh.red);
+    bsum = block_sum<long long>(bsum, sh.red);
+    if (threadIdx.x == 0) {
+      a.cgcount[g] = cnt;
+      a.gbsum[g] = (unsigned long long)bsum;
+    }
+    return;
+  }
+  const long long nb = a.next_lists ? a.ngcbase[g] : -1;
+  int64_t out = a.cgstart[g];
+  long long boff = 0;
+  for (int64_t base = r0; base < r1; base += blockDim.x) {
+    const int64_t r = base + threadIdx.x;
+    const bool live = r < r1;
+    const int n = live ? (int)a.rowcnt[cbase + r] : 0;
+    const long long x = live ? (((long long)n << kPackShift) | (long long)a.rowbsum[cbase + r]) : 0;
+    long long tot;
+    const long long pre = block_exclusive_scan<long long>(x, tot, sh.red);
+    if (n) {
+      int64_t pos = out + (pre >> kPackShift);
+      long long off = boff + (pre & kPackMask);
+      const uint64_t kbase = ROOT ? (uint64_t)job * a.K : (uint64_t)a.pid[r] * (uint64_t)a.K;
+      const int nvp = ROOT ? 0 : a.meta[a.pid[r]].nv;
+      if (!ROOT) load_slot(slot, a.vt, r, a.S);
+      candidates(r, [&](int key, int w, int bound) {
+        const int32_t v = a.sup[kbase + key];
+        if (v < 0) {
+          put_child<ROOT>(a, pos, -v - 1, (int)r, w, nvp, slot, nb >= 0, (uint32_t)off);
+          ++pos;
+          off += bound;
+        }
+      });
+    }
+    out += tot >> kPackShift;
+    boff += tot & kPackMask;
+  }
+}
+
+// WRITE = false: k_count; WRITE = true: k_write.
+template <bool ROOT, bool WRITE>
+__global__ void __launch_bounds__(kMaxBlock) k_children(LevelArgs a) {
+  __shared__ WriteShared sh;
   const int g = blockIdx.x;
   int64_t r0, r1;
   graph_rows<ROOT>(a, g, r0, r1);
   if (r0 == r1) {
-    if (threadIdx.x == 0) {
-      a.cgstart[g] = 0;
+    if (!WRITE && threadIdx.x == 0) {
       a.cgcount[g] = 0;
+      a.gbsum[g] = 0;
     }
     return;
   }
+  if (WRITE && a.cgcount[g] == 0) return;
   const Lds l = lds_layout(a, false);
-  if (graph_fits(a, g))
-    write_body<ROOT, true>(a, g, r0, r1, l, s_waves, s_red);
+  const bool lists = a.use_lists && (ROOT || a.gcbase[g] >= 0);
+  if (lists)
+    children_lists<ROOT, WRITE>(a, g, r0, r1, l, sh);
+  else if (graph_fits(a, g))
+    children_enum<ROOT, true, WRITE>(a, g, r0, r1, l, sh);
   else
-    write_body<ROOT, false>(a, g, r0, r1, l, s_waves, s_red);
+    children_enum<ROOT, false, WRITE>(a, g, r0, r1, l, sh);
+}
+
+// One workgroup of kScanBlock threads: cgstart = exclusive scan of cgcount,
+// ngcbase = exclusive scan of gbsum (or -1 where the region would pass the
+// store or the 32-bit offsets), totals into *cursor / *ncand_cursor.  A
+// graph whose rows would pass ccap gets no rows (the host compares the
+// total with the embedding count and fails the call).
+constexpr int kScanBlock = 1024;
+
+__global__ void __launch_bounds__(kScanBlock) k_scan(LevelArgs a, int n) {
+  __shared__ long long s[kScanBlock / 64 + 1];
+  __shared__ unsigned long long su[kScanBlock / 64 + 1];
+  long long rows = 0;
+  unsigned long long ents = 0;
+  const int per = (n + kScanBlock - 1) / kScanBlock;  // graphs per thread, contiguous
+  const int g0 = min(n, (int)threadIdx.x * per), g1 = min(n, g0 + per);
+  long long my_rows = 0;
+  unsigned long long my_ents = 0;
+  for (int g = g0; g < g1; ++g) {
+    my_rows += a.cgcount[g];
+    my_ents += a.gbsum[g];
+  }
+  long long trows;
+  unsigned long long tents;
+  rows = block_exclusive_scan<long long>(my_rows, trows, s);
+  ents = block_exclusive_scan<unsigned long long>(my_ents, tents, su);
+  for (int g = g0; g < g1; ++g) {
+    a.cgstart[g] = rows;
+    const long long c = a.cgcount[g];
+    if (rows + c > a.ccap) a.cgcount[g] = 0;  // past the allocation: not written; the host sees the total
+    rows += c;
+    const unsigned long long b = a.gbsum[g];
+    long long nb = -1;
+    if (a.next_lists && b > 0 && b <= 0xFFFFFFFFull && ents + b <= (unsigned long long)a.ncand_cap) nb = (long long)ents;
+    a.ngcbase[g] = nb;
+    ents += b;
+  }
+  long long my_max = 0;
+  for (int g = g0; g < g1; ++g) my_max = max(my_max, (long long)a.cgcount[g]);
+  for (int o = 32; o > 0; o >>= 1) my_max = max(my_max, (long long)__shfl_down(my_max, o, 64));
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = my_max;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kScanBlock / 64; ++i) my_max = max(my_max, s[i]);
+    *a.cursor = (unsigned long long)trows;
+    *a.ncand_cursor = tents;
+    a.cursor[2] = (unsigned long long)my_max;  // largest graph's child rows
+  }
 }
 
 // ------------------------------------------------------------ primitives
@@ -542,8 +900,8 @@ __global__ void k_fill_i32(int32_t *dst, int64_t n, int32_t v) {
 // ------------------------------------------------------------ launchers
 static inline unsigned grid1(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
-size_t level_lds_bytes(const LevelArgs &a, bool with_table) {
-  return 4ull * ((with_table ? (2u << a.table_bits) : 0u) + (size_t)a.block * kSlotWords + a.stage_v + a.stage_a);
+size_t level_lds_bytes(const LevelArgs &a, bool support) {
+  return 4ull * ((support ? (2u << a.table_bits) : 0u) + (size_t)a.block * a.slot_words + a.stage_v + a.stage_a);
 }
 
 hipError_t launch_support(const LevelArgs &a, int n_graphs, bool root, hipStream_t s) {
@@ -570,9 +928,14 @@ hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t
   if (n_graphs <= 0) return hipSuccess;
   const size_t lds = level_lds_bytes(a, false);
   if (root)
-    hipLaunchKernelGGL(k_write<true>, dim3(n_graphs), dim3(a.block), lds, s, a);
+    hipLaunchKernelGGL((k_children<true, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
   else
-    hipLaunchKernelGGL(k_write<false>, dim3(n_graphs), dim3(a.block), lds, s, a);
+    hipLaunchKernelGGL((k_children<false, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, s, a, n_graphs);
+  if (root)
+    hipLaunchKernelGGL((k_children<true, true>), dim3(n_graphs), dim3(a.block), lds, s, a);
+  else
+    hipLaunchKernelGGL((k_children<false, true>), dim3(n_graphs), dim3(a.block), lds, s, a);
   return hipGetLastError();
 }
 

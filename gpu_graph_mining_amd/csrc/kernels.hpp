@@ -13,6 +13,14 @@ namespace gm {
 // 65535 (uint16 counts): a row extends by at most 32 * 256 edges.
 constexpr int kMaxBlock = 256;
 
+// Statistics counters that every workgroup bumps are spread over kSpread
+// cache lines: atomics on one address serialise across the device.
+constexpr int kSpread = 64;
+constexpr int kSpreadStride = 16;  // 64-bit words per line
+constexpr int kSpreadCand = 0;     // extension candidates enumerated
+constexpr int kSpreadFill = 1;     // largest dedup table fill (atomicMax)
+constexpr int kSpreadBound = 2;    // sum of the candidates' child bounds
+
 // Everything one level-step kernel reads or writes.  Plain device pointers.
 struct LevelArgs {
   // database (device.hpp layout)
@@ -36,35 +44,59 @@ struct LevelArgs {
   // supports: >= 0 a distinct-graph count; < 0 encodes a selected child
   // as -(child + 1) once the host has chosen the children
   int32_t *sup;
-  int64_t nkeys;      // P * K
-  int32_t *sup_rep;   // rep replicas of nkeys distinct-graph counters
-  unsigned long long *proj_rep;  // rep replicas of nkeys embedding counters
-  int32_t rep;        // power of two
+  int64_t nkeys;                 // P * K
+  // rep replicas of nkeys packed counters: distinct graphs << 40 | embeddings
+  unsigned long long *acc_rep;
+  int32_t rep;                   // power of two
   const int32_t *minsup_job;
-  // (key, support, embeddings) triples of the keys whose support reached the
-  // job's min support
+  // (key, support, embeddings) of the keys whose support reached the job's
+  // min support
   unsigned long long *freq;
   uint32_t *freq_count;
   uint32_t freq_cap;
-  unsigned long long *ecand;  // extension candidates enumerated
-  int32_t *error_flag;        // set when a graph cannot be deduplicated
-  // children: graph g's child rows are [cgstart[g], cgstart[g] + cgcount[g]),
-  // reserved from *cursor; ccap rows are allocated
-  uint16_t *rowcnt;  // children per current row (root level: per vertex)
-  unsigned long long *cursor;
+  // statistics spread over kSpread cache lines (slot i of line j at
+  // spread[j * kSpreadStride + i]): candidates enumerated, largest table
+  // fill, sum of the candidates' child bounds
+  unsigned long long *spread;
+  int32_t *error_flag;           // set when a graph cannot be deduplicated
+  unsigned long long *split_passes;  // dedup passes abandoned for a split
+  // candidate lists of this level ({code, child bound} entries, one per
+  // item, code kNoCand where the item extends nothing): row r's list is
+  // cand[gcbase[g] + rcoff[r] ...] with rccnt[r] entries (root level:
+  // cand[vadj[v] ...] for vertex v); gcbase[g] < 0: no lists
+  int32_t use_lists;
+  uint2 *cand;
+  const int64_t *gcbase;
+  const uint32_t *rcoff;
+  uint16_t *rccnt;
+  // children: graph g's child rows are [cgstart[g], cgstart[g] + cgcount[g])
+  // (counted by k_children<.., false>, placed by k_scan); ccap rows are
+  // allocated
+  uint16_t *rowcnt;   // enumeration mode: children per row (root: per vertex)
+  uint32_t *rowbsum;  // enumeration mode: children's bound sum per row
+  unsigned long long *gbsum;   // children's bound sum per graph
+  unsigned long long *cursor;  // total child rows (written by k_scan)
   int64_t ccap;
   int64_t *cgstart;
   int64_t *cgcount;
   int32_t *cpid;
   uint8_t *cvt;
   int32_t CS;  // child tuple stride
-  // dynamic LDS: dedup table bits, staged vertex-offset and adjacency words
-  int32_t table_bits, stage_v, stage_a;
-  int32_t block;                       // threads per workgroup
-  unsigned long long *split_passes;    // dedup passes abandoned for a split
+  // next level's candidate lists: per-graph region base ngcbase (-1: none)
+  // within ncand_cap entries, per-child offset nrcoff; *ncand_cursor gets
+  // the entries all graphs asked for
+  int32_t next_lists;
+  int64_t ncand_cap;
+  unsigned long long *ncand_cursor;
+  int64_t *ngcbase;
+  uint32_t *nrcoff;
+  // dynamic LDS: dedup table bits, per-thread slot words (tuple_words of
+  // tuple, then the rightmost path), staged vertex-offset and adjacency words
+  int32_t table_bits, slot_words, tuple_words, stage_v, stage_a;
+  int32_t block;  // threads per workgroup
 };
 
-size_t level_lds_bytes(const LevelArgs &a, bool with_table);
+size_t level_lds_bytes(const LevelArgs &a, bool support);
 hipError_t launch_reduce(const LevelArgs &a, bool root, hipStream_t s);
 
 hipError_t launch_support(const LevelArgs &a, int n_graphs, bool root, hipStream_t s);

@@ -5,9 +5,9 @@
 // (src/gspan_cuda/gspan_cuda_no_sort.cpp:275-489; kernel_execution.hpp:61-71).
 // This driver grows it level by level instead: one level step extends the
 // embeddings of *every* pattern of the current size in the same launches,
-// so each step costs a fixed number of kernels and two small host round
-// trips (the frequent-key list down, the chosen children up) no matter how
-// many patterns the level holds.  Patterns are independent subtrees, so the
+// so each step costs a fixed number of kernels and one host round trip
+// (the frequent-key list down, the chosen children up) no matter how many
+// patterns the level holds.  Patterns are independent subtrees, so the
 // set of reported (code, support) pairs is the one the depth-first oracle
 // reports (src/gspan/gspan.cpp:172-274); only the emission order differs.
 //
@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstddef>
 #include <cstring>
 #include <map>
 #include <condition_variable>
@@ -50,9 +51,24 @@ void DevBuf::release() {
 
 void *DevBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return p;
+  const size_t want = std::max<size_t>({bytes, cap + cap / 2, 256});
   release();
-  size_t want = std::max<size_t>(bytes, 256);
   hip_check(hipMalloc(&p, want), "hipMalloc");
+  cap = want;
+  return p;
+}
+
+HostBuf::~HostBuf() {
+  if (p) (void)hipHostFree(p);
+}
+
+void *HostBuf::ensure(size_t bytes) {
+  if (bytes <= cap && p) return p;
+  const size_t want = std::max<size_t>({bytes, cap + cap / 2, 4096});
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+  hip_check(hipHostMalloc(&p, want, hipHostMallocDefault), "hipHostMalloc");
   cap = want;
   return p;
 }
@@ -63,13 +79,6 @@ template <class T>
 void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
   b.ensure(sizeof(T) * std::max<size_t>(v.size(), 1));
   if (!v.empty()) hip_check(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s), "upload");
-}
-
-template <class T>
-void download(std::vector<T> &v, const void *src, size_t n, hipStream_t s) {
-  v.resize(n);
-  if (n) hip_check(hipMemcpyAsync(v.data(), src, sizeof(T) * n, hipMemcpyDeviceToHost, s), "download");
-  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
 int64_t round4(int64_t x) { return (x + 3) & ~int64_t(3); }
@@ -91,6 +100,7 @@ void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk
   }
   if (ng > INT32_MAX - 1 || nv > INT32_MAX - 1 || 2 * ne > INT32_MAX - 1)
     throw Error(GM_E_UNSUPPORTED, "database too large for 32-bit offsets");
+  if (ng >= (1ll << 24)) throw Error(GM_E_UNSUPPORTED, "more than 16777215 graphs (24-bit support counters)");
   std::sort(vls.begin(), vls.end());
   vls.erase(std::unique(vls.begin(), vls.end()), vls.end());
   std::sort(els.begin(), els.end());
@@ -309,9 +319,60 @@ class Pool {
   std::exception_ptr err_;
 };
 
+// Device-side counters at the head of the frequent-key buffer (bytes):
+//   0 frequent keys (u32)   8 dedup error (i32)   16 split passes (u64)
+//   24 child rows placed (u64)   32 next-level list entries asked (u64)
+// then kSpread lines of spread statistics (candidates, largest table fill).
+// The level-start clear covers all but the child-row words, which k_scan
+// overwrites and the next level's download reads back.
+struct Counters {
+  uint32_t fcount, pad0;
+  int32_t err, pad1;
+  unsigned long long splits, wcursor, ncursor, max_rows;  // max_rows: largest graph's child rows (k_scan)
+  unsigned long long pad[2];
+  unsigned long long spread[kSpread * kSpreadStride];
+};
+static_assert(sizeof(Counters) == 64 + 8 * kSpread * kSpreadStride, "counter block layout");
+
+// The write kernel must have produced exactly the rows the embedding
+// counts announced.
+void check_write(const Counters &c, int64_t rows) {
+  if (c.wcursor != (unsigned long long)rows)
+    throw Error(GM_E_DEVICE, "child rows written (" + std::to_string(c.wcursor) +
+                                 ") differ from the embedding count (" + std::to_string(rows) + ")");
+}
+
+// Grow a workspace buffer; the stream is drained first because the old
+// allocation may still be in use by queued kernels.
+void *grow(DevBuf &b, size_t bytes, hipStream_t s) {
+  if (bytes <= b.cap && b.p) return b.p;
+  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  return b.ensure(bytes);
+}
+
+// Grow a replica buffer and clear it (its zero invariant covers all of it).
+void grow_zero(DevBuf &b, size_t bytes, hipStream_t s) {
+  if (bytes <= b.cap && b.p) return;
+  grow(b, bytes, s);
+  hip_check(hipMemsetAsync(b.p, 0, b.cap, s), "memset replicas");
+}
+
+// Dedup table bits for an expected number of distinct keys per graph: the
+// table passes are split when 3/4 full.
+int table_bits_for(uint64_t keys) {
+  int b = 8;
+  while (b < 12 && ((1ull << b) * 3) / 4 < keys) ++b;
+  return b;
+}
+
+// Candidate-list entries (8 bytes) a level may keep before the write kernel
+// falls back to enumeration.
+constexpr uint64_t kMaxListEntries = 1ull << 32;
+
 }  // namespace
 
 bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats *stats, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(db.mu);
   Timer total;
   const int32_t G = db.n_graphs;
   const int32_t NEL = db.NEL, NVL = db.NVL;
@@ -319,18 +380,23 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
                                       : (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
   gm_stats st;
   std::memset(&st, 0, sizeof(st));
+  if (!db.ws) db.ws.reset(new Workspace());
+  Workspace &w = *db.ws;
+  if (!w.replicas_zero) {
+    if (w.acc_rep.p) hip_check(hipMemsetAsync(w.acc_rep.p, 0, w.acc_rep.cap, s), "memset replicas");
+  }
+  w.replicas_zero = false;  // until this call ends normally
 
-  // device state
-  DevBuf minsup_d, sup, sup_rep, proj_rep, freq, counters, meta_d, gs_a, gs_b, gc_a, gc_b, pid_a, pid_b, vt_a, vt_b,
-      rowcnt, idx_d, val_d;
-  upload(minsup_d, opt.minsup, s);
-  counters.ensure(64);
-  for (DevBuf *b : {&gs_a, &gs_b, &gc_a, &gc_b}) b->ensure(sizeof(int64_t) * std::max(G, 1));
-  uint32_t *freq_count = reinterpret_cast<uint32_t *>(counters.as<char>());
-  unsigned long long *ecand = reinterpret_cast<unsigned long long *>(counters.as<char>() + 8);
-  int32_t *error_flag = reinterpret_cast<int32_t *>(counters.as<char>() + 16);
-  unsigned long long *split_passes = reinterpret_cast<unsigned long long *>(counters.as<char>() + 24);
-  unsigned long long *cursor = reinterpret_cast<unsigned long long *>(counters.as<char>() + 32);
+  grow(w.minsup, sizeof(int32_t) * opt.minsup.size(), s);
+  hip_check(hipMemcpyAsync(w.minsup.p, opt.minsup.data(), sizeof(int32_t) * opt.minsup.size(),
+                           hipMemcpyHostToDevice, s), "upload min supports");
+  for (int i = 0; i < 2; ++i) {
+    grow(w.gs[i], sizeof(int64_t) * std::max(G, 1), s);
+    grow(w.gc[i], sizeof(int64_t) * std::max(G, 1), s);
+    grow(w.gcb[i], sizeof(int64_t) * std::max(G, 1), s);
+  }
+  // the root level's lists sit at the adjacency offsets
+  if (opt.candidate_lists) grow(w.cand[0], sizeof(uint2) * std::max<int64_t>(db.n_adj, 1), s);
 
   hipEvent_t ev0, ev1;
   hip_check(hipEventCreate(&ev0), "hipEventCreate");
@@ -355,25 +421,25 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   a.gjob = db.gjob.as<int32_t>();
   a.NEL = NEL;
   a.NVL = NVL;
-  a.minsup_job = minsup_d.as<int32_t>();
-  a.freq_count = freq_count;
-  a.ecand = ecand;
-  a.error_flag = error_flag;
-  a.split_passes = split_passes;
-  a.cursor = cursor;
-  // LDS: dedup table of 2^11 keys; stage the largest graph's adjacency when
-  // it fits 16 KB (larger graphs are read from HBM)
-  a.table_bits = 11;
+  // LDS: stage the largest graph's adjacency when it fits 16 KB (larger
+  // graphs are read from HBM)
   a.stage_v = db.max_graph_v + 1;
   a.stage_a = std::min<int32_t>(db.max_graph_adj, 4096);
+  int dev = 0, lds_max = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
 
-  std::vector<Pat> pats;      // patterns of the current level
+  std::vector<Pat> pats;  // patterns of the current level
   bool stopped = false;
-  int level = 0;              // 0 = root stage
-  int S = 0;                  // tuple stride of the current level
+  int level = 0;          // 0 = root stage
+  int S = 0;              // tuple stride of the current level
   int64_t n_rows = 0;
-  DevBuf *gs_cur = &gs_a, *gs_nxt = &gs_b, *gc_cur = &gc_a, *gc_nxt = &gc_b;
-  DevBuf *pid_cur = &pid_a, *pid_nxt = &pid_b, *vt_cur = &vt_a, *vt_nxt = &vt_b;
+  int cur = 0;                // which of the double buffers holds the current level
+  int64_t pending_rows = -1;  // child rows the last write must have produced
+  uint32_t last_fcount = 0;
+  uint32_t last_fill = 0;     // largest dedup table fill of the last level
+  int max_nrm = 0;            // longest rightmost path among this level's patterns
+  bool lists_now = opt.candidate_lists;  // this level's rows have candidate lists
 
   auto to_original = [&](const std::vector<DFS> &c) {
     std::vector<DFS> o(c);
@@ -392,46 +458,75 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const int64_t P = root ? db.n_jobs : (int64_t)pats.size();
     const int64_t K = root ? (int64_t)NVL * NEL * NVL : (int64_t)S * NEL * (NVL + 1);
     const int64_t nkeys = P * K;
-    std::vector<PatMeta> metas;
-    if (!root) {
-      metas.resize(P);
-      for (int64_t i = 0; i < P; ++i) metas[i] = make_meta(pats[i]);
-      upload(meta_d, metas, s);
-    }
-    // Support and embedding counters: `rep` replicas (summed by k_reduce)
-    // spread the atomics of small key spaces, where every graph hits the
-    // same keys.  Replicas stay within 256 KB of support counters so the
-    // atomics stay in L2.
+    // Packed support / embedding counters: `rep` replicas (summed and
+    // cleared by k_reduce) spread the atomics of small key spaces, where
+    // every graph hits the same keys; the replicas stay within 512 KB.
     int rep = 1;
-    while (rep < 64 && (int64_t)rep * 2 * nkeys * 4 <= (int64_t)(256 << 10)) rep *= 2;
-    sup.ensure(sizeof(int32_t) * nkeys);
-    sup_rep.ensure(sizeof(int32_t) * nkeys * rep);
-    proj_rep.ensure(sizeof(unsigned long long) * nkeys * rep);
-    hip_check(hipMemsetAsync(sup_rep.p, 0, sizeof(int32_t) * nkeys * rep, s), "memset supports");
-    hip_check(hipMemsetAsync(proj_rep.p, 0, sizeof(unsigned long long) * nkeys * rep, s), "memset embeddings");
-    hip_check(hipMemsetAsync(counters.p, 0, 64, s), "memset counters");
-    uint32_t fcap = (uint32_t)std::min<int64_t>(nkeys, std::max<int64_t>(1 << 20, P * 64));
-    freq.ensure(3 * sizeof(unsigned long long) * fcap);
-    a.gstart = root ? nullptr : gs_cur->as<int64_t>();
-    a.gcount = root ? nullptr : gc_cur->as<int64_t>();
-    a.pid = root ? nullptr : pid_cur->as<int32_t>();
-    a.vt = root ? nullptr : vt_cur->as<uint8_t>();
+    while (rep < 64 && (int64_t)rep * 2 * nkeys * 8 <= (int64_t)(512 << 10)) rep *= 2;
+    grow(w.sup, sizeof(int32_t) * nkeys, s);
+    grow_zero(w.acc_rep, sizeof(unsigned long long) * nkeys * rep, s);
+    const uint32_t fcap = (uint32_t)std::min<int64_t>(nkeys, std::max<int64_t>(1 << 20, P * 64));
+    const size_t freq_bytes = sizeof(Counters) + 3 * sizeof(unsigned long long) * fcap;
+    if (w.freq.p && freq_bytes > w.freq.cap && pending_rows >= 0) {
+      // the counter block moves: settle the last write's check first
+      Counters c;
+      hip_check(hipMemcpyAsync(&c, w.freq.p, sizeof(c), hipMemcpyDeviceToHost, s), "download counters");
+      hip_check(hipStreamSynchronize(s), "k_write");
+      check_write(c, pending_rows);
+      pending_rows = -1;
+    }
+    grow(w.freq, freq_bytes, s);
+    Counters *dctr = w.freq.as<Counters>();
+    // clear all but the child-row words (k_scan of the last level wrote them)
+    hip_check(hipMemsetAsync(dctr, 0, offsetof(Counters, wcursor), s), "memset counters");
+    hip_check(hipMemsetAsync(dctr->spread, 0, sizeof(dctr->spread), s), "memset counters");
+    const int64_t list_rows = root ? db.n_vertices : n_rows;
+    a.use_lists = lists_now ? 1 : 0;
+    if (a.use_lists) grow(w.rccnt, sizeof(uint16_t) * std::max<int64_t>(list_rows, 1), s);
+    a.cand = w.cand[cur].as<uint2>();
+    a.gcbase = w.gcb[cur].as<int64_t>();
+    a.rcoff = w.rcoff[cur].as<uint32_t>();
+    a.rccnt = w.rccnt.as<uint16_t>();
+    a.gstart = root ? nullptr : w.gs[cur].as<int64_t>();
+    a.gcount = root ? nullptr : w.gc[cur].as<int64_t>();
+    a.pid = root ? nullptr : w.pid[cur].as<int32_t>();
+    a.vt = root ? nullptr : w.vt[cur].as<uint8_t>();
     a.S = S;
-    a.meta = root ? nullptr : meta_d.as<PatMeta>();
+    a.meta = root ? nullptr : reinterpret_cast<const PatMeta *>(w.upl[cur].as<char>());
     a.K = K;
-    a.sup = sup.as<int32_t>();
+    a.sup = w.sup.as<int32_t>();
     a.nkeys = nkeys;
-    a.sup_rep = sup_rep.as<int32_t>();
-    a.proj_rep = proj_rep.as<unsigned long long>();
+    a.acc_rep = w.acc_rep.as<unsigned long long>();
     a.rep = rep;
-    a.freq = freq.as<unsigned long long>();
+    a.minsup_job = w.minsup.as<int32_t>();
+    a.freq = reinterpret_cast<unsigned long long *>(dctr + 1);
     a.freq_cap = fcap;
+    a.freq_count = &dctr->fcount;
+    a.spread = dctr->spread;
+    a.error_flag = &dctr->err;
+    a.split_passes = &dctr->splits;
+    a.cursor = &dctr->wcursor;
+    a.ncand_cursor = &dctr->ncursor;
     // workgroup size: enough lanes for the average graph's rows (or vertices)
-    {
+    if (opt.block_threads > 0) {
+      a.block = opt.block_threads;
+    } else {
       const double per_graph =
           root ? (double)db.n_vertices / std::max(1, G) : (double)n_rows / (double)std::max(1, G);
       a.block = per_graph > 128 ? 256 : per_graph > 64 ? 128 : 64;
     }
+    // dedup table: sized for twice the last level's fullest graph (root
+    // level: the largest adjacency), at most 2^10 entries so that many
+    // workgroups fit a CU's LDS; larger graphs split their passes
+    a.table_bits = std::min(10, table_bits_for(root ? (uint64_t)db.max_graph_adj : 2ull * last_fill));
+    // per-thread slots: the tuple, then the row's rightmost path (at least
+    // 2 words for the list replay; odd word stride against bank conflicts)
+    const int rm_words = (int)round4(max_nrm);
+    a.tuple_words = S / 4;
+    a.slot_words = std::max(2, a.tuple_words + rm_words) | 1;
+    while (level_lds_bytes(a, true) > (size_t)lds_max && a.table_bits > 6) --a.table_bits;
+    if (level_lds_bytes(a, true) > (size_t)lds_max)
+      throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
 
     // 32-bit dedup keys: launch over pattern ranges with range * K < 2^32
     const int64_t chunk = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
@@ -443,34 +538,53 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     }
     hip_check(hipEventRecord(ev1, s), "hipEventRecord");
     hip_check(launch_reduce(a, root, s), "k_reduce");
-    struct {
-      uint32_t fcount, pad;
-      unsigned long long ecand;
-      int32_t err, pad2;
-      unsigned long long splits;
-      unsigned long long cursor;
-    } cnt;
-    hip_check(hipMemcpyAsync(&cnt, counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s), "download counters");
+    // counters and (most often all of) the frequent keys in one copy
+    const uint32_t guess = std::min<uint32_t>(fcap, std::max<uint32_t>(4096, 2 * last_fcount));
+    const size_t head = sizeof(Counters) + 3 * sizeof(unsigned long long) * guess;
+    char *hbuf = static_cast<char *>(w.down.ensure(head));
+    hip_check(hipMemcpyAsync(hbuf, dctr, head, hipMemcpyDeviceToHost, s), "download counters");
     hip_check(hipStreamSynchronize(s), "k_support");
+    Counters cnt;
+    std::memcpy(&cnt, hbuf, sizeof(cnt));
+    unsigned long long ecand = 0, max_fill = 0, cand_bound = 0;
+    for (int i = 0; i < kSpread; ++i) {
+      ecand += cnt.spread[i * kSpreadStride + kSpreadCand];
+      max_fill = std::max(max_fill, cnt.spread[i * kSpreadStride + kSpreadFill]);
+      cand_bound += cnt.spread[i * kSpreadStride + kSpreadBound];
+    }
     float ms = 0;
     hip_check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
     if (!root) {
       ext_ms += ms;
       ext_launches++;
-      st.e_cand += cnt.ecand;
+      st.e_cand += ecand;
     }
     st.split_passes += cnt.splits;
+    if (pending_rows >= 0) {
+      check_write(cnt, pending_rows);
+      st.max_graph_rows = std::max<uint64_t>(st.max_graph_rows, cnt.max_rows);
+    }
+    st.max_table_fill = std::max<uint64_t>(st.max_table_fill, max_fill);
     if (cnt.err) throw Error(GM_E_UNSUPPORTED, "per-graph deduplication table overflow");
     if (cnt.fcount > fcap)
       throw Error(GM_E_OOM, "frequent-key list overflow (" + std::to_string(cnt.fcount) + " keys)");
+    last_fcount = cnt.fcount;
+    last_fill = (uint32_t)max_fill;
     struct Freq {
       unsigned long long key, sup, proj;
     };
     std::vector<Freq> fr(cnt.fcount);
     if (cnt.fcount) {
-      hip_check(hipMemcpyAsync(fr.data(), freq.p, sizeof(Freq) * cnt.fcount, hipMemcpyDeviceToHost, s),
-                "download frequent keys");
-      hip_check(hipStreamSynchronize(s), "download frequent keys");
+      const uint32_t have = std::min(cnt.fcount, guess);
+      std::memcpy(fr.data(), hbuf + sizeof(Counters), sizeof(Freq) * have);
+      if (cnt.fcount > have) {
+        const size_t rest = sizeof(Freq) * (cnt.fcount - have);
+        char *h2 = static_cast<char *>(w.down.ensure(rest));
+        hip_check(hipMemcpyAsync(h2, reinterpret_cast<const char *>(a.freq) + sizeof(Freq) * have, rest,
+                                 hipMemcpyDeviceToHost, s), "download frequent keys");
+        hip_check(hipStreamSynchronize(s), "download frequent keys");
+        std::memcpy(fr.data() + have, h2, rest);
+      }
     }
     std::sort(fr.begin(), fr.end(), [](const Freq &x, const Freq &y) { return x.key < y.key; });
     dev_sec += tdev.sec();
@@ -536,37 +650,75 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     }
     const int64_t C = (int64_t)next.size();
     int CS = 4;
+    pending_rows = -1;
+    bool lists_next = false;
     if (C > 0) {
-      // The embedding counts size the child level exactly: one write kernel
-      // reserves each graph's rows from a cursor and fills them.
+      // The embedding counts size the child level exactly; the write kernel
+      // reserves each graph's rows from a cursor.  One upload carries the
+      // selected keys and the children's metadata (the next level's).
       Timer tdev2;
       int max_nv = 0;
       for (const Pat &c : next) max_nv = std::max(max_nv, (int)c.vlab.size());
       CS = (int)round4(max_nv);
-      upload(idx_d, sel_keys, s);
-      upload(val_d, sel_vals, s);
-      hip_check(launch_scatter_i32(sup.as<int32_t>(), idx_d.as<uint64_t>(), val_d.as<int32_t>(), C, s), "scatter");
-      pid_nxt->ensure(sizeof(int32_t) * std::max<int64_t>(child_rows, 1));
-      vt_nxt->ensure((size_t)CS * std::max<int64_t>(child_rows, 1));
-      rowcnt.ensure(sizeof(uint16_t) * std::max<int64_t>(root ? db.n_vertices : n_rows, 1));
-      a.rowcnt = rowcnt.as<uint16_t>();
+      const int nxt = cur ^ 1;
+      const size_t meta_bytes = sizeof(PatMeta) * C;
+      const size_t keys_off = (meta_bytes + 7) & ~size_t(7);
+      const size_t vals_off = keys_off + sizeof(unsigned long long) * C;
+      const size_t up_bytes = vals_off + sizeof(int32_t) * C;
+      char *hu = static_cast<char *>(w.up.ensure(up_bytes));
+      PatMeta *hm = reinterpret_cast<PatMeta *>(hu);
+      int next_nrm = 0;
+      for (int64_t i = 0; i < C; ++i) {
+        hm[i] = make_meta(next[i]);
+        next_nrm = std::max(next_nrm, (int)hm[i].nrm);
+      }
+      std::memcpy(hu + keys_off, sel_keys.data(), sizeof(unsigned long long) * C);
+      std::memcpy(hu + vals_off, sel_vals.data(), sizeof(int32_t) * C);
+      char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
+      hip_check(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, s), "upload children");
+      hip_check(launch_scatter_i32(w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(du + keys_off),
+                                   reinterpret_cast<const int32_t *>(du + vals_off), C, s), "scatter");
+      grow(w.pid[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
+      grow(w.vt[nxt], (size_t)CS * std::max<int64_t>(child_rows, 1), s);
+      grow(w.rowcnt, sizeof(uint16_t) * std::max<int64_t>(list_rows, 1), s);
+      grow(w.rowbsum, sizeof(uint32_t) * std::max<int64_t>(list_rows, 1), s);
+      // the children's candidate lists: at most the bound sum of all this
+      // level's candidates (the selected ones need less)
+      lists_next = opt.candidate_lists && cand_bound > 0 && cand_bound < kMaxListEntries;
+      if (lists_next) {
+        grow(w.cand[nxt], sizeof(uint2) * cand_bound, s);
+        grow(w.rcoff[nxt], sizeof(uint32_t) * std::max<int64_t>(child_rows, 1), s);
+      }
+      grow(w.gbsum, sizeof(unsigned long long) * std::max(G, 1), s);
+      a.rowcnt = w.rowcnt.as<uint16_t>();
+      a.rowbsum = w.rowbsum.as<uint32_t>();
+      a.gbsum = w.gbsum.as<unsigned long long>();
       a.ccap = child_rows;
-      a.cgstart = gs_nxt->as<int64_t>();
-      a.cgcount = gc_nxt->as<int64_t>();
-      a.cpid = pid_nxt->as<int32_t>();
-      a.cvt = vt_nxt->as<uint8_t>();
+      a.cgstart = w.gs[nxt].as<int64_t>();
+      a.cgcount = w.gc[nxt].as<int64_t>();
+      a.cpid = w.pid[nxt].as<int32_t>();
+      a.cvt = w.vt[nxt].as<uint8_t>();
       a.CS = CS;
+      a.next_lists = lists_next ? 1 : 0;
+      a.tuple_words = std::max(S, CS) / 4;
+      a.slot_words = (a.tuple_words + rm_words) | 1;
+      if (level_lds_bytes(a, false) > (size_t)lds_max)
+        throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
+      a.ncand_cap = lists_next ? (int64_t)(w.cand[nxt].cap / sizeof(uint2)) : 0;
+      a.ngcbase = w.gcb[nxt].as<int64_t>();
+      a.nrcoff = w.rcoff[nxt].as<uint32_t>();
       hip_check(launch_write(a, G, root, s), "k_write");
-      hip_check(hipMemcpyAsync(&cnt, counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s), "download counters");
-      hip_check(hipStreamSynchronize(s), "k_write");
-      if (cnt.err || cnt.cursor != (unsigned long long)child_rows)
-        throw Error(GM_E_DEVICE, "child rows written (" + std::to_string(cnt.cursor) +
-                                     ") differ from the embedding count (" + std::to_string(child_rows) + ")");
+      pending_rows = child_rows;
       dev_sec += tdev2.sec();
-      size_t live = db.bytes + sup.cap + sup_rep.cap + proj_rep.cap + pid_a.cap + pid_b.cap + vt_a.cap + vt_b.cap +
-                    freq.cap + meta_d.cap + rowcnt.cap;
+      size_t live = db.bytes + w.sup.cap + w.acc_rep.cap + w.freq.cap +
+                    w.rowcnt.cap + w.rowbsum.cap + w.rccnt.cap;
+      for (int i = 0; i < 2; ++i)
+        live += w.pid[i].cap + w.vt[i].cap + w.upl[i].cap + w.cand[i].cap + w.rcoff[i].cap;
       peak = std::max(peak, live);
+      cur = nxt;
+      max_nrm = next_nrm;
     }
+    lists_now = lists_next;
     if (!opt.collect_proj) std::fill(nproj.begin(), nproj.end(), 0);
     if (root)
       st.e_root += child_rows;
@@ -579,15 +731,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     st.n_levels++;
     if (stopped) break;
     pats.swap(next);
-    std::swap(gs_cur, gs_nxt);
-    std::swap(gc_cur, gc_nxt);
-    std::swap(pid_cur, pid_nxt);
-    std::swap(vt_cur, vt_nxt);
     S = CS;
     n_rows = child_rows;
     ++level;
   }
-  (void)n_rows;
+  hip_check(hipStreamSynchronize(s), "mining");
+  w.replicas_zero = true;
   st.seconds_total = total.sec();
   st.seconds_device = dev_sec;
   st.seconds_canon = canon_sec;

@@ -4,6 +4,7 @@
 
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "gspan_amd.h"
@@ -28,6 +29,32 @@ struct DevBuf {
   T *as() const { return static_cast<T *>(p); }
 };
 
+// Growable pinned host allocation (staging for stream-ordered copies).
+struct HostBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
+  ~HostBuf();
+  void *ensure(size_t bytes);
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+// Device buffers of the level steps, kept across mining calls on one
+// database so that repeated calls allocate nothing once warm.
+struct Workspace {
+  DevBuf minsup, sup, acc_rep, freq, rowcnt, rowbsum, gbsum, rccnt;
+  // double-buffered by level: rows (gs/gc/pid/vt), pattern metadata and
+  // selected keys (upl), candidate lists (cand/gcb/rcoff)
+  DevBuf gs[2], gc[2], pid[2], vt[2], upl[2], cand[2], gcb[2], rcoff[2];
+  HostBuf up, down;
+  // the replica counters hold zeros outside a level step (k_reduce clears
+  // what it reads); false after a call that ended early
+  bool replicas_zero = false;
+};
+
 // The packed database of device.hpp on the GPU, for one or more jobs.
 struct DeviceDB {
   int32_t n_graphs = 0, n_jobs = 1;
@@ -38,6 +65,9 @@ struct DeviceDB {
   std::vector<int32_t> job_first_graph;       // n_jobs + 1
   DevBuf gvbase, vadj, adj, vlab, gjob;
   size_t bytes = 0;
+  // mining state; one mining call at a time per database
+  std::unique_ptr<Workspace> ws;
+  std::mutex mu;
 };
 
 // Host image of the device layout (device.hpp).
@@ -63,7 +93,9 @@ struct MineOptions {
   std::vector<int32_t> minsup;  // per job
   int32_t max_edges = 0;
   int32_t canon_threads = 0;
+  int32_t block_threads = 0;   // 0 = by rows per graph
   bool collect_proj = false;
+  bool candidate_lists = true;  // k_support stores candidates for k_write
 };
 
 // Runs the level-synchronous miner; returns false when the report callback

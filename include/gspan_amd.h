@@ -54,10 +54,12 @@ typedef struct gm_cfg {
   int32_t label_style;      /* GM_LABELS_FULL (default) or GM_LABELS_WILDCARD */
   int32_t max_edges;        /* stop growing patterns beyond this many edges; 0 = unbounded */
   int32_t canon_threads;    /* host threads for the canonical check; 0 = auto */
-  int32_t block_threads;    /* reserved: the level kernels run one wave64 per graph */
+  int32_t block_threads;    /* threads per graph workgroup: 0 = by rows per graph, or 64/128/256 */
   int64_t mem_budget_bytes; /* device bytes the embedding store may use; 0 = auto */
   int32_t collect_proj;     /* 1 = also report |Proj(P)| per pattern (no extra pass) */
-  int32_t reserved[7];
+  int32_t candidate_lists;  /* 0 = the support pass keeps each embedding's extension candidates for
+                               the write pass (default); 1 = the write pass enumerates again */
+  int32_t reserved[6];
 } gm_cfg;
 
 /* Per-run statistics (filled when a non-NULL pointer is passed). */
@@ -75,7 +77,9 @@ typedef struct gm_stats {
   uint64_t ext_kernel_launches;
   uint64_t peak_device_bytes;
   uint64_t split_passes;   /* per-graph dedup passes redone in smaller parts */
-  uint64_t reserved[5];
+  uint64_t max_graph_rows; /* most embeddings one graph held at one level   */
+  uint64_t max_table_fill; /* most distinct extensions one graph produced in one pass */
+  uint64_t reserved[3];
 } gm_stats;
 
 /* Pattern callback: n_edges rows of (from, to, fromlabel, elabel, tolabel),

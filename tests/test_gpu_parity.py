@@ -142,3 +142,31 @@ def test_gpu_driver_class_api(golden, tmp_path):
     lines = (tmp_path / "out.txt").read_text().splitlines()
     assert sorted(lines) == sorted(l.split("\t")[0] for l in case["lines"])
     assert out.get_count() == case["n_patterns"]
+
+
+@pytest.mark.parametrize("name", ["config1_1k", "chem_34", "config4_2k"])
+def test_gpu_write_paths_agree(golden, name):
+    """The write kernel's two sources of children -- the candidate lists the
+    support kernel stored, and a fresh enumeration -- give the same patterns,
+    supports and |Proj|; so do the fixed workgroup sizes.  The first call on
+    a database runs with a cold candidate store (some graphs fall back to
+    enumeration), the later calls with a warm one."""
+    case = golden[name]
+    db = case_db(case)
+    runs = [gm.mine(db, case["minsup"], collect_proj=True),  # cold store
+            gm.mine(db, case["minsup"], collect_proj=True),  # warm store
+            gm.mine(db, case["minsup"], collect_proj=True, candidate_lists=False)]
+    runs += [gm.mine(db, case["minsup"], collect_proj=True, block_threads=b) for b in (64, 128, 256)]
+    want = _by_code(runs[0][0])
+    check_listing(listing(runs[0][0]), case)
+    for pats, st in runs[1:]:
+        assert _by_code(pats) == want
+        assert (st["e_child"], st["e_cand"], st["e_root"]) == (runs[0][1]["e_child"], runs[0][1]["e_cand"],
+                                                               runs[0][1]["e_root"])
+
+
+def test_gpu_bad_block_threads_rejected(golden):
+    db = case_db(golden["chem_17"])
+    with pytest.raises(gm.GMError) as e:
+        gm.mine(db, 5, block_threads=96)
+    assert e.value.code == gm.GM_E_ARG
