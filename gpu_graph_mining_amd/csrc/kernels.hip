@@ -359,10 +359,185 @@ __device__ __forceinline__ int find_row(const int *pre, int j, int n) {
   return lo;
 }
 
+// ----------------------------------------------------------------- EXPAND
+// Later levels with candidate lists: the extensions of every row are listed
+// before any deduplication, fully in parallel over the level's rows: one
+// 64-lane workgroup (a single wave) per 64 consecutive rows, whatever their
+// graphs.  Setup: lane t stages row t's record (tuple, rightmost path, item
+// prefix per source, row facts) in LDS and its visited mask.  Items: the
+// wave's items -- one per adjacency entry of each row's sources (the
+// rightmost vertex, then the rightmost-path sources) -- are taken 64 at a
+// time, one per lane (row by binary search over the item prefix); each lane
+// applies the extension rule to its adjacency entry.  A row's candidates
+// are appended to its list in item order, positions from a ballot over the
+// window; lane t keeps row t's running count.  The adjacency is read from
+// HBM (rows of one wave may belong to different graphs).
+//
+// Row record (32-bit words): [0, TW) tuple bytes; [TW, TW + RMW) rightmost
+// path; [TW + RMW, + SPW) item prefix per source: 0, deg(rv), + deg(rm 0),
+// ...; then pattern id, PatMeta word 1 (nv, nrm, minlabel, maxlabel),
+// bwd_mask, the graph's first global vertex, list index (2 words).
+constexpr int kExpandRows = 64;
+
+__global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
+  __shared__ int s_pre[kExpandRows];
+  extern __shared__ uint32_t dyn[];
+  const int t = threadIdx.x;
+  const int TW = a.tuple_words, RMW = a.rm_words, SPW = a.src_words;
+  uint32_t *recs = dyn;
+  unsigned long long *vis = reinterpret_cast<unsigned long long *>(dyn + kExpandRows * a.slot_words);
+  const int64_t r0 = (int64_t)blockIdx.x * kExpandRows;
+  const int64_t r = r0 + t;
+  const int nrow = (int)min<int64_t>(kExpandRows, a.n_rows - r0);
+  const int NEL = a.NEL, NVL = a.NVL;
+  const int fwd_base = a.S * NEL;
+  uint32_t *mine = recs + t * a.slot_words;
+  int n = 0;
+  bool has_list = false;  // row t's graph kept lists (else k_support enumerates it)
+  if (t < nrow) {
+    const int g = a.rgraph[r];
+    const long long gb = a.gcbase[g];
+    if (gb >= 0) {
+      has_list = true;
+      const int32_t p = a.pid[r];
+      const PatMeta *pm = a.meta + p;
+      const uint4 head = *reinterpret_cast<const uint4 *>(pm);
+      const int nv = (int)(head.y & 0xFF), nrm = (int)((head.y >> 8) & 0xFF);
+      const int32_t vb = a.gvbase[g];
+      GView G;
+      G.vadj = a.vadj + vb;
+      G.A = a.adj;
+      G.aoff = 0;
+      load_slot(mine, a.vt, r, a.S);
+      uint32_t *rmv = mine + TW;
+      stage_rm(pm, rmv);
+      VSet vs;
+      for (int i = 0; i < nv; ++i) vs.add(slot_get(mine, i));
+      vis[t * 4 + 0] = vs.m0;
+      vis[t * 4 + 1] = vs.m1;
+      vis[t * 4 + 2] = vs.m2;
+      vis[t * 4 + 3] = vs.m3;
+      uint32_t *sp = rmv + RMW;
+      int acc = G.deg(slot_get(mine, nv - 1));
+      sp[0] = 0;
+      sp[1] = (uint32_t)acc;
+      for (int i = 0; i < nrm; ++i) {
+        acc += G.deg(slot_get(mine, rmv[i] & 0xFF));
+        sp[i + 2] = (uint32_t)acc;
+      }
+      uint32_t *misc = sp + SPW;
+      const unsigned long long li = (unsigned long long)gb + a.rcoff[r];
+      misc[0] = (uint32_t)p;
+      misc[1] = head.y;
+      misc[2] = head.z;
+      misc[3] = (uint32_t)vb;
+      misc[4] = (uint32_t)li;
+      misc[5] = (uint32_t)(li >> 32);
+      n = acc;
+    }
+  }
+  int T;
+  const int pre = block_exclusive_scan<int>(n, T, s_pre);  // one wave: cheap barriers
+  __syncthreads();
+  s_pre[t] = pre;
+  __syncthreads();
+  int cnt = 0;  // candidates of row t listed so far
+  unsigned long long cand = 0, bound_sum = 0;
+  for (int jb = 0; jb < T; jb += kExpandRows) {
+    const int j = jb + t;
+    int k = 0, key = -1, w = 0, bound = 0;
+    if (j < T) {
+      k = find_row(s_pre, j, nrow);
+      const int i = j - s_pre[k];
+      const uint32_t *rec = recs + k * a.slot_words;
+      const uint32_t *rmv = rec + TW;
+      const uint32_t *sp = rmv + RMW;
+      const uint32_t *misc = sp + SPW;
+      const uint32_t hy = misc[1];
+      const int nv = (int)(hy & 0xFF), nrm = (int)((hy >> 8) & 0xFF);
+      const int minlabel = (int)((hy >> 16) & 0xFF), maxlabel = (int)(hy >> 24);
+      const int maxtoc = nv - 1;
+      const unsigned long long *vrow = vis + k * 4;
+      GView G;
+      G.vadj = a.vadj + misc[3];
+      G.A = a.adj;
+      G.aoff = 0;
+      const int own = (int)sp[nrm + 1];
+      int src = 0;  // 0: the rightmost vertex; s >= 1: rightmost-path source s - 1
+      while ((int)sp[src + 1] <= i) ++src;
+      if (src == 0) {
+        const int rv = slot_get(rec, maxtoc);
+        const uint32_t e = G.entry(G.begin(rv) + i);
+        w = e & 0xFF;
+        const int el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
+        if (!((vrow[w >> 6] >> (w & 63)) & 1ull)) {
+          if (nl >= minlabel) {
+            key = fwd_base + (maxtoc * NEL + el) * NVL + nl;
+            bound = own + G.deg(w);
+          }
+        } else {
+          const uint32_t bwd_mask = misc[2];
+          for (int q = 1; q < nrm; ++q) {
+            const uint32_t rme = rmv[q];
+            const int f = rme & 0xFF;
+            if (slot_get(rec, f) != w) continue;
+            const int e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
+            if (!((bwd_mask >> f) & 1u) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
+              key = f * NEL + el;
+              bound = own;
+            }
+            break;
+          }
+          w = -1;
+        }
+      } else {
+        const uint32_t rme = rmv[src - 1];
+        const int f = rme & 0xFF;
+        const int u = slot_get(rec, f);
+        const int e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
+        const uint32_t e = G.entry(G.begin(u) + (i - (int)sp[src]));
+        w = e & 0xFF;
+        const int el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
+        if (!((vrow[w >> 6] >> (w & 63)) & 1ull) && nl >= minlabel && (e1 < el || (e1 == el && l1 <= nl))) {
+          key = fwd_base + (f * NEL + el) * NVL + nl;
+          bound = own - (int)sp[src] + G.deg(w);  // sources src - 1 .. root, plus w
+        }
+      }
+    }
+    const bool emit = key >= 0;
+    const unsigned long long mask = __ballot(emit);
+    const int first = j < T ? max(s_pre[k], jb) - jb : 0;  // lane of row k's first item in the window
+    const unsigned long long below = mask & ((1ull << t) - 1) & ~((1ull << first) - 1);
+    const int listed = __shfl(cnt, k, kExpandRows);
+    if (emit) {
+      const uint32_t *misc = recs + k * a.slot_words + TW + RMW + SPW;
+      const unsigned long long li = (unsigned long long)misc[4] | ((unsigned long long)misc[5] << 32);
+      a.cand[li + listed + __popcll(below)] = make_uint2(cand_code(false, key, w), (uint32_t)bound);
+      ++cand;
+      bound_sum += (unsigned long long)bound;
+    }
+    // row t's items in this window: lanes [lo, hi)
+    const int lo = max(pre, jb) - jb, hi = min(pre + n, jb + kExpandRows) - jb;
+    if (hi > lo) {
+      const unsigned long long range = (hi - lo == 64 ? ~0ull : (((1ull << (hi - lo)) - 1) << lo));
+      cnt += __popcll(mask & range);
+    }
+  }
+  if (has_list) a.rccnt[r] = (uint16_t)cnt;
+  // statistics: one atomic per workgroup each, spread over kSpread lines
+  __shared__ unsigned long long s_red[2];
+  cand = block_sum<unsigned long long>(cand, s_red);
+  bound_sum = block_sum<unsigned long long>(bound_sum, s_red);
+  if (t == 0) {
+    unsigned long long *line = a.spread + (blockIdx.x & (kSpread - 1)) * kSpreadStride;
+    if (cand) atomicAdd(&line[kSpreadCand], cand);
+    if (bound_sum) atomicAdd(&line[kSpreadBound], bound_sum);
+  }
+}
+
 template <bool ROOT, bool LDS>
 __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, const Lds &l,
                                              SupportShared &sh) {
-  const GView G = graph_view<LDS>(a, g, l);
   const int job = ROOT ? a.gjob[g] : 0;
   const int32_t vb = a.gvbase[g];
   uint2 *lists = nullptr;
@@ -374,8 +549,26 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
       if (b >= 0) lists = a.cand + b;
     }
   }
+  // k_expand already listed this graph's candidates (and counted them):
+  // only the dedup passes remain, reading the lists
+  const bool listed = !ROOT && a.lists_ready && lists != nullptr;
+  GView G{};
+  if (!listed) G = graph_view<LDS>(a, g, l);
   const int B = blockDim.x;
-  const int tb = a.table_bits;
+  int tb = a.table_bits;
+  if (listed) {
+    // the graph's candidates bound its distinct keys: a table just large
+    // enough for them is cheaper to clear and to commit
+    unsigned long long ents = 0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      const int32_t p = a.pid[r];
+      if (p >= a.p_lo && p < a.p_hi) ents += a.rccnt[r];
+    }
+    ents = block_sum<unsigned long long>(ents, sh.red);
+    int want = 6;
+    while (want < tb && ((1ull << want) * 3) / 4 < ents) ++want;
+    tb = want;
+  }
   const uint32_t cap = 1u << tb, fill_limit = cap - cap / 4;
   uint32_t *table = l.table;
   uint32_t *tcnt = l.table + cap;
@@ -387,7 +580,7 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
     sh.top = 1;
     sh.stack_mod[0] = 1;
     sh.stack_res[0] = 0;
-    sh.replay = 0;
+    sh.replay = listed ? 1 : 0;
   }
   for (;;) {
     __syncthreads();
@@ -406,9 +599,10 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
     int over = 0;  // this thread saw the table pass its fill limit
     // once the pass has failed (flag published in LDS) nothing more is
     // inserted: a nearly full table would make every probe sequence long
-    volatile int *failed_flag = &sh.over;
+    auto failed_now = [&]() { return __hip_atomic_load(&sh.over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
+    auto set_failed = [&]() { __hip_atomic_store(&sh.over, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); };
     auto insert = [&](uint32_t k) {
-      if (over || *failed_flag) return;
+      if (over || failed_now()) return;
       uint32_t h = hash_slot32(k, tb);
       for (uint32_t probe = 0; probe < cap; ++probe) {
         const uint32_t old = atomicCAS(&table[h], kEmpty32, k);
@@ -416,14 +610,14 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
           atomicAdd(&tcnt[h], 1u);
           if (old == kEmpty32 && (uint32_t)atomicAdd(&sh.fill, 1) >= fill_limit) {
             over = 1;
-            *failed_flag = 1;
+            set_failed();
           }
           return;
         }
         h = (h + 1) & (cap - 1);
       }
       over = 1;
-      *failed_flag = 1;
+      set_failed();
     };
     if (!sh.replay) {
       for (int64_t r = r0 + threadIdx.x; r < r1; r += B) {
@@ -494,8 +688,10 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
           int key, w;
           cand_decode(ROOT, c.x, key, w);
           insert(rec[1] + (uint32_t)key);
-          ++cand;
-          bound_sum += c.y;
+          if (!listed) {
+            ++cand;
+            bound_sum += c.y;
+          }
         }
         __syncthreads();  // the next chunk reuses the slots
       }
@@ -546,10 +742,20 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
   }
 }
 
-template <bool ROOT>
+// Graph of this workgroup: the small-graph launch covers every graph id and
+// leaves graphs of more than big_rows rows to the big-graph launch, which
+// runs over the list k_scan built (-1: past the list's end).
+template <bool BIG>
+__device__ __forceinline__ int pick_graph(const LevelArgs &a) {
+  if (!BIG) return blockIdx.x;
+  return blockIdx.x < (unsigned)*a.nbig ? a.big[blockIdx.x] : -1;
+}
+
+template <bool ROOT, bool BIG>
 __global__ void __launch_bounds__(kMaxBlock) k_support(LevelArgs a) {
   __shared__ SupportShared sh;
-  const int g = blockIdx.x;
+  const int g = pick_graph<BIG>(a);
+  if (g < 0) return;
   if (ROOT) {
     const int job = a.gjob[g];
     if (job < a.p_lo || job >= a.p_hi) return;
@@ -557,6 +763,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_support(LevelArgs a) {
   int64_t r0, r1;
   graph_rows<ROOT>(a, g, r0, r1);
   if (r0 == r1) return;
+  if (!ROOT && !BIG && r1 - r0 > a.big_rows) return;
   const Lds l = lds_layout(a, true);
   if (graph_fits(a, g))
     support_body<ROOT, true>(a, g, r0, r1, l, sh);
@@ -622,9 +829,10 @@ struct WriteShared {
 
 // Write one child row at `pos` (parent tuple in slot; root: vertex u).
 template <bool ROOT>
-__device__ __forceinline__ void put_child(const LevelArgs &a, int64_t pos, int32_t child, int u, int w, int nvp,
-                                          uint32_t *slot, bool keep_lists, uint32_t off) {
+__device__ __forceinline__ void put_child(const LevelArgs &a, int g, int64_t pos, int32_t child, int u, int w,
+                                          int nvp, uint32_t *slot, bool keep_lists, uint32_t off) {
   a.cpid[pos] = child;
+  a.crgraph[pos] = g;
   if (ROOT) {
     uint32_t *p = reinterpret_cast<uint32_t *>(a.cvt + pos * a.CS);
     p[0] = (uint32_t)u | ((uint32_t)w << 8);
@@ -724,7 +932,7 @@ __device__ __forceinline__ void children_lists(const LevelArgs &a, int g, int64_
           nvp = a.meta[a.pid[r]].nv;
           load_slot(slot, a.vt, r, a.S);
         }
-        put_child<ROOT>(a, out + (pre >> kPackShift), -v - 1, (int)r, w, nvp, slot, nb >= 0,
+        put_child<ROOT>(a, g, out + (pre >> kPackShift), -v - 1, (int)r, w, nvp, slot, nb >= 0,
                         (uint32_t)(boff + (pre & kPackMask)));
       }
       out += tot >> kPackShift;
@@ -800,7 +1008,7 @@ __device__ __forceinline__ void children_enum(const LevelArgs &a, int g, int64_t
       candidates(r, [&](int key, int w, int bound) {
         const int32_t v = a.sup[kbase + key];
         if (v < 0) {
-          put_child<ROOT>(a, pos, -v - 1, (int)r, w, nvp, slot, nb >= 0, (uint32_t)off);
+          put_child<ROOT>(a, g, pos, -v - 1, (int)r, w, nvp, slot, nb >= 0, (uint32_t)off);
           ++pos;
           off += bound;
         }
@@ -812,12 +1020,14 @@ __device__ __forceinline__ void children_enum(const LevelArgs &a, int g, int64_t
 }
 
 // WRITE = false: k_count; WRITE = true: k_write.
-template <bool ROOT, bool WRITE>
+template <bool ROOT, bool WRITE, bool BIG>
 __global__ void __launch_bounds__(kMaxBlock) k_children(LevelArgs a) {
   __shared__ WriteShared sh;
-  const int g = blockIdx.x;
+  const int g = pick_graph<BIG>(a);
+  if (g < 0) return;
   int64_t r0, r1;
   graph_rows<ROOT>(a, g, r0, r1);
+  if (!ROOT && !BIG && r1 - r0 > a.big_rows) return;
   if (r0 == r1) {
     if (!WRITE && threadIdx.x == 0) {
       a.cgcount[g] = 0;
@@ -871,6 +1081,14 @@ __global__ void __launch_bounds__(kScanBlock) k_scan(LevelArgs a, int n) {
     a.ngcbase[g] = nb;
     ents += b;
   }
+  // the next level's big graphs, in graph order
+  int my_big = 0;
+  for (int g = g0; g < g1; ++g) my_big += a.cgcount[g] > a.big_rows ? 1 : 0;
+  int tbig;
+  int at = block_exclusive_scan<int>(my_big, tbig, reinterpret_cast<int *>(s));
+  for (int g = g0; g < g1; ++g)
+    if (a.cgcount[g] > a.big_rows) a.nbig_list[at++] = g;
+  if (threadIdx.x == 0) *a.nnbig = (unsigned long long)tbig;
   long long my_max = 0;
   for (int g = g0; g < g1; ++g) my_max = max(my_max, (long long)a.cgcount[g]);
   for (int o = 32; o > 0; o >>= 1) my_max = max(my_max, (long long)__shfl_down(my_max, o, 64));
@@ -906,11 +1124,25 @@ size_t level_lds_bytes(const LevelArgs &a, bool support) {
 
 hipError_t launch_support(const LevelArgs &a, int n_graphs, bool root, hipStream_t s) {
   if (n_graphs <= 0) return hipSuccess;
-  const size_t lds = level_lds_bytes(a, true);
-  if (root)
-    hipLaunchKernelGGL(k_support<true>, dim3(n_graphs), dim3(a.block), lds, s, a);
-  else
-    hipLaunchKernelGGL(k_support<false>, dim3(n_graphs), dim3(a.block), lds, s, a);
+  if (root) {
+    hipLaunchKernelGGL((k_support<true, false>), dim3(n_graphs), dim3(a.block), level_lds_bytes(a, true), s, a);
+  } else {
+    hipLaunchKernelGGL((k_support<false, false>), dim3(n_graphs), dim3(a.block), level_lds_bytes(a, true), s, a);
+    if (a.big_grid > 0) {
+      LevelArgs b = a;
+      b.block = a.big_block;
+      hipLaunchKernelGGL((k_support<false, true>), dim3(a.big_grid), dim3(b.block), level_lds_bytes(b, true), s, b);
+    }
+  }
+  return hipGetLastError();
+}
+
+size_t expand_lds_bytes(const LevelArgs &a) { return 4ull * kExpandRows * a.slot_words + 32ull * kExpandRows; }
+
+hipError_t launch_expand(const LevelArgs &a, hipStream_t s) {
+  if (a.n_rows <= 0) return hipSuccess;
+  const unsigned grid = (unsigned)((a.n_rows + kExpandRows - 1) / kExpandRows);
+  hipLaunchKernelGGL(k_expand, dim3(grid), dim3(kExpandRows), expand_lds_bytes(a), s, a);
   return hipGetLastError();
 }
 
@@ -927,15 +1159,23 @@ hipError_t launch_reduce(const LevelArgs &a, bool root, hipStream_t s) {
 hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t s) {
   if (n_graphs <= 0) return hipSuccess;
   const size_t lds = level_lds_bytes(a, false);
-  if (root)
-    hipLaunchKernelGGL((k_children<true, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
-  else
-    hipLaunchKernelGGL((k_children<false, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
+  LevelArgs b = a;
+  b.block = a.big_block;
+  const size_t lds_big = level_lds_bytes(b, false);
+  const bool big = !root && a.big_grid > 0;
+  if (root) {
+    hipLaunchKernelGGL((k_children<true, false, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
+  } else {
+    hipLaunchKernelGGL((k_children<false, false, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
+    if (big) hipLaunchKernelGGL((k_children<false, false, true>), dim3(a.big_grid), dim3(b.block), lds_big, s, b);
+  }
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, s, a, n_graphs);
-  if (root)
-    hipLaunchKernelGGL((k_children<true, true>), dim3(n_graphs), dim3(a.block), lds, s, a);
-  else
-    hipLaunchKernelGGL((k_children<false, true>), dim3(n_graphs), dim3(a.block), lds, s, a);
+  if (root) {
+    hipLaunchKernelGGL((k_children<true, true, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
+  } else {
+    hipLaunchKernelGGL((k_children<false, true, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
+    if (big) hipLaunchKernelGGL((k_children<false, true, true>), dim3(a.big_grid), dim3(b.block), lds_big, s, b);
+  }
   return hipGetLastError();
 }
 

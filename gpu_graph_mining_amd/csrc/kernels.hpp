@@ -33,6 +33,8 @@ struct LevelArgs {
   const int64_t *gstart;
   const int64_t *gcount;
   const int32_t *pid;
+  const int32_t *rgraph;  // graph of each row
+  int64_t n_rows;
   const uint8_t *vt;
   int32_t S;  // tuple stride in bytes
   const PatMeta *meta;
@@ -65,6 +67,7 @@ struct LevelArgs {
   // cand[gcbase[g] + rcoff[r] ...] with rccnt[r] entries (root level:
   // cand[vadj[v] ...] for vertex v); gcbase[g] < 0: no lists
   int32_t use_lists;
+  int32_t lists_ready;  // k_expand listed this level's candidates already
   uint2 *cand;
   const int64_t *gcbase;
   const uint32_t *rcoff;
@@ -80,6 +83,7 @@ struct LevelArgs {
   int64_t *cgstart;
   int64_t *cgcount;
   int32_t *cpid;
+  int32_t *crgraph;
   uint8_t *cvt;
   int32_t CS;  // child tuple stride
   // next level's candidate lists: per-graph region base ngcbase (-1: none)
@@ -93,13 +97,25 @@ struct LevelArgs {
   // dynamic LDS: dedup table bits, per-thread slot words (tuple_words of
   // tuple, then the rightmost path), staged vertex-offset and adjacency words
   int32_t table_bits, slot_words, tuple_words, stage_v, stage_a;
+  int32_t rm_words, src_words;  // k_expand row records: rightmost path, item prefix
   int32_t block;  // threads per workgroup
+  // Graphs with more than big_rows rows at this level run in a second launch
+  // of big_grid workgroups of big_block threads over the list big[0, *nbig)
+  // that the previous level's k_scan built; k_scan writes the next level's
+  // list to nbig_list / *nnbig.
+  const int32_t *big;
+  const unsigned long long *nbig;
+  int32_t big_rows, big_block, big_grid;
+  int32_t *nbig_list;
+  unsigned long long *nnbig;
 };
 
 size_t level_lds_bytes(const LevelArgs &a, bool support);
 hipError_t launch_reduce(const LevelArgs &a, bool root, hipStream_t s);
 
 hipError_t launch_support(const LevelArgs &a, int n_graphs, bool root, hipStream_t s);
+size_t expand_lds_bytes(const LevelArgs &a);
+hipError_t launch_expand(const LevelArgs &a, hipStream_t s);
 hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t s);
 hipError_t launch_scatter_i32(int32_t *dst, const uint64_t *idx, const int32_t *val, int64_t n, hipStream_t s);
 hipError_t launch_fill_i32(int32_t *dst, int64_t n, int32_t v, hipStream_t s);

@@ -25,6 +25,8 @@
 #include <atomic>
 #include <chrono>
 #include <cstddef>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <condition_variable>
@@ -365,6 +367,12 @@ int table_bits_for(uint64_t keys) {
   return b;
 }
 
+// Graphs with more rows than this at a level run in a second launch with
+// kBigBlock-thread workgroups, so that one large graph does not hold the
+// level up while a 64-lane workgroup walks its rows.
+constexpr int kBigRows = 128;
+constexpr int kBigBlock = 256;
+
 // Candidate-list entries (8 bytes) a level may keep before the write kernel
 // falls back to enumeration.
 constexpr uint64_t kMaxListEntries = 1ull << 32;
@@ -394,7 +402,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     grow(w.gs[i], sizeof(int64_t) * std::max(G, 1), s);
     grow(w.gc[i], sizeof(int64_t) * std::max(G, 1), s);
     grow(w.gcb[i], sizeof(int64_t) * std::max(G, 1), s);
+    grow(w.big[i], sizeof(int32_t) * std::max(G, 1), s);
   }
+  grow(w.nbig, 2 * sizeof(unsigned long long), s);
   // the root level's lists sit at the adjacency offsets
   if (opt.candidate_lists) grow(w.cand[0], sizeof(uint2) * std::max<int64_t>(db.n_adj, 1), s);
 
@@ -424,6 +434,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   // LDS: stage the largest graph's adjacency when it fits 16 KB (larger
   // graphs are read from HBM)
   a.stage_v = db.max_graph_v + 1;
+  a.big_rows = kBigRows;
+  a.big_block = kBigBlock;
   a.stage_a = std::min<int32_t>(db.max_graph_adj, 4096);
   int dev = 0, lds_max = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
@@ -440,6 +452,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   uint32_t last_fill = 0;     // largest dedup table fill of the last level
   int max_nrm = 0;            // longest rightmost path among this level's patterns
   bool lists_now = opt.candidate_lists;  // this level's rows have candidate lists
+  // GM_LEVEL_TRACE=1: one line per level step on stderr (sizes, timings)
+  const char *trace_env = getenv("GM_LEVEL_TRACE");
+  const bool trace_levels = trace_env && trace_env[0] == '1';
 
   auto to_original = [&](const std::vector<DFS> &c) {
     std::vector<DFS> o(c);
@@ -487,9 +502,16 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     a.gcbase = w.gcb[cur].as<int64_t>();
     a.rcoff = w.rcoff[cur].as<uint32_t>();
     a.rccnt = w.rccnt.as<uint16_t>();
+    a.big = w.big[cur].as<int32_t>();
+    a.nbig = w.nbig.as<unsigned long long>() + cur;
+    // at most rows / (big_rows + 1) graphs can be big (none at the root)
+    a.big_grid = root ? 0 : (int32_t)std::min<int64_t>(G, n_rows / (kBigRows + 1));
     a.gstart = root ? nullptr : w.gs[cur].as<int64_t>();
     a.gcount = root ? nullptr : w.gc[cur].as<int64_t>();
     a.pid = root ? nullptr : w.pid[cur].as<int32_t>();
+    a.rgraph = root ? nullptr : w.rgraph[cur].as<int32_t>();
+    a.n_rows = root ? 0 : n_rows;
+    a.lists_ready = 0;
     a.vt = root ? nullptr : w.vt[cur].as<uint8_t>();
     a.S = S;
     a.meta = root ? nullptr : reinterpret_cast<const PatMeta *>(w.upl[cur].as<char>());
@@ -524,13 +546,30 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const int rm_words = (int)round4(max_nrm);
     a.tuple_words = S / 4;
     a.slot_words = std::max(2, a.tuple_words + rm_words) | 1;
-    while (level_lds_bytes(a, true) > (size_t)lds_max && a.table_bits > 6) --a.table_bits;
-    if (level_lds_bytes(a, true) > (size_t)lds_max)
-      throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
+    {
+      LevelArgs b = a;
+      b.block = std::max(a.block, a.big_block);
+      while (level_lds_bytes(b, true) > (size_t)lds_max && b.table_bits > 6) --b.table_bits;
+      if (level_lds_bytes(b, true) > (size_t)lds_max)
+        throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
+      a.table_bits = b.table_bits;
+    }
 
     // 32-bit dedup keys: launch over pattern ranges with range * K < 2^32
     const int64_t chunk = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
     hip_check(hipEventRecord(ev0, s), "hipEventRecord");
+    if (!root && a.use_lists) {
+      // list every row's candidates in one parallel pass; the support
+      // launches then only deduplicate per graph
+      LevelArgs e = a;
+      e.rm_words = rm_words;
+      e.src_words = max_nrm + 2;
+      e.slot_words = (a.tuple_words + e.rm_words + e.src_words + 6) | 1;
+      if (expand_lds_bytes(e) > (size_t)lds_max)
+        throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
+      hip_check(launch_expand(e, s), "k_expand");
+      a.lists_ready = 1;
+    }
     for (int64_t lo = 0; lo < P; lo += chunk) {
       a.p_lo = (int32_t)lo;
       a.p_hi = (int32_t)std::min<int64_t>(P, lo + chunk);
@@ -544,6 +583,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     char *hbuf = static_cast<char *>(w.down.ensure(head));
     hip_check(hipMemcpyAsync(hbuf, dctr, head, hipMemcpyDeviceToHost, s), "download counters");
     hip_check(hipStreamSynchronize(s), "k_support");
+    Timer t_host;  // host work between the level's download and its upload
     Counters cnt;
     std::memcpy(&cnt, hbuf, sizeof(cnt));
     unsigned long long ecand = 0, max_fill = 0, cand_bound = 0;
@@ -590,6 +630,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     dev_sec += tdev.sec();
 
     // decode frequent keys into child codes; keep the minimal ones
+    const double us_prep = t_host.sec() * 1e6;
     Timer tcan;
     const int64_t F = (int64_t)fr.size();
     std::vector<Pat> kids(F);
@@ -633,8 +674,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     });
     if (too_big) throw Error(GM_E_UNSUPPORTED, "pattern with more than 32 vertices is frequent");
     canon_sec += tcan.sec();
+    const double us_canon = tcan.sec() * 1e6;
+    Timer t_sel;
     st.n_candidates += root ? 0 : F;
 
+    double us_select = 0, us_host = 0;
     std::vector<Pat> next;
     std::vector<unsigned long long> sel_keys;
     std::vector<int32_t> sel_vals;
@@ -676,9 +720,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       std::memcpy(hu + vals_off, sel_vals.data(), sizeof(int32_t) * C);
       char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
       hip_check(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, s), "upload children");
+      us_select = t_sel.sec() * 1e6;
+      us_host = t_host.sec() * 1e6;
       hip_check(launch_scatter_i32(w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(du + keys_off),
                                    reinterpret_cast<const int32_t *>(du + vals_off), C, s), "scatter");
       grow(w.pid[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
+      grow(w.rgraph[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.vt[nxt], (size_t)CS * std::max<int64_t>(child_rows, 1), s);
       grow(w.rowcnt, sizeof(uint16_t) * std::max<int64_t>(list_rows, 1), s);
       grow(w.rowbsum, sizeof(uint32_t) * std::max<int64_t>(list_rows, 1), s);
@@ -697,15 +744,22 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.cgstart = w.gs[nxt].as<int64_t>();
       a.cgcount = w.gc[nxt].as<int64_t>();
       a.cpid = w.pid[nxt].as<int32_t>();
+      a.crgraph = w.rgraph[nxt].as<int32_t>();
       a.cvt = w.vt[nxt].as<uint8_t>();
       a.CS = CS;
       a.next_lists = lists_next ? 1 : 0;
       a.tuple_words = std::max(S, CS) / 4;
       a.slot_words = (a.tuple_words + rm_words) | 1;
-      if (level_lds_bytes(a, false) > (size_t)lds_max)
-        throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
+      {
+        LevelArgs b = a;
+        b.block = std::max(a.block, a.big_block);
+        if (level_lds_bytes(b, false) > (size_t)lds_max)
+          throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
+      }
       a.ncand_cap = lists_next ? (int64_t)(w.cand[nxt].cap / sizeof(uint2)) : 0;
       a.ngcbase = w.gcb[nxt].as<int64_t>();
+      a.nbig_list = w.big[nxt].as<int32_t>();
+      a.nnbig = w.nbig.as<unsigned long long>() + nxt;
       a.nrcoff = w.rcoff[nxt].as<uint32_t>();
       hip_check(launch_write(a, G, root, s), "k_write");
       pending_rows = child_rows;
@@ -713,7 +767,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       size_t live = db.bytes + w.sup.cap + w.acc_rep.cap + w.freq.cap +
                     w.rowcnt.cap + w.rowbsum.cap + w.rccnt.cap;
       for (int i = 0; i < 2; ++i)
-        live += w.pid[i].cap + w.vt[i].cap + w.upl[i].cap + w.cand[i].cap + w.rcoff[i].cap;
+        live += w.pid[i].cap + w.rgraph[i].cap + w.vt[i].cap + w.upl[i].cap + w.cand[i].cap + w.rcoff[i].cap;
       peak = std::max(peak, live);
       cur = nxt;
       max_nrm = next_nrm;
@@ -728,6 +782,14 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       st.n_patterns++;
       if (!report(next[i].job, to_original(next[i].code), next[i].sup, nproj[i])) stopped = true;
     }
+    if (trace_levels)
+      fprintf(stderr,
+              "[gm level %d] patterns %lld rows %lld K %lld rep %d block %d table_bits %d support_us %.1f "
+              "candidates %llu frequent %u children %lld child_rows %lld splits %llu max_fill %llu lists %d "
+              "host_us prep %.1f canon %.1f select %.1f total %.1f\n",
+              level, (long long)P, (long long)(root ? db.n_vertices : n_rows), (long long)K, rep, a.block,
+              a.table_bits, ms * 1e3, ecand, cnt.fcount, (long long)C, (long long)child_rows, cnt.splits, max_fill,
+              a.use_lists, us_prep, us_canon, us_select, us_host);
     st.n_levels++;
     if (stopped) break;
     pats.swap(next);

@@ -46,9 +46,10 @@ struct HostBuf {
 // database so that repeated calls allocate nothing once warm.
 struct Workspace {
   DevBuf minsup, sup, acc_rep, freq, rowcnt, rowbsum, gbsum, rccnt;
-  // double-buffered by level: rows (gs/gc/pid/vt), pattern metadata and
-  // selected keys (upl), candidate lists (cand/gcb/rcoff)
-  DevBuf gs[2], gc[2], pid[2], vt[2], upl[2], cand[2], gcb[2], rcoff[2];
+  // double-buffered by level: rows (gs/gc/pid/rgraph/vt), pattern metadata and
+  // selected keys (upl), candidate lists (cand/gcb/rcoff), big-graph lists
+  DevBuf gs[2], gc[2], pid[2], rgraph[2], vt[2], upl[2], cand[2], gcb[2], rcoff[2], big[2];
+  DevBuf nbig;  // big-graph counts of the two buffered levels
   HostBuf up, down;
   // the replica counters hold zeros outside a level step (k_reduce clears
   // what it reads); false after a call that ended early
