@@ -105,6 +105,22 @@ def _batch_section(n_jobs: int, out: dict) -> None:
                     "batched_patterns": int(st["n_patterns"])}
 
 
+def _serving_section(n_requests: int, rate: float, out: dict) -> None:
+    """Config 4: Poisson arrivals of 90 % pattern-support queries (3-12
+    edges, random-walk subgraphs) and 10 % mining jobs (1k-10k graph
+    subsets at 5-20 %) against the resident molecule database, served with
+    continuous batching; latency = completion - arrival."""
+    from gpu_graph_mining_amd import serving
+    from gpu_graph_mining_amd import workloads as wl
+    db = wl.config4()
+    db.upload()
+    serving.serve(db, rate=rate, n_requests=min(20, n_requests), seed=40)  # warm the handles and the library
+    res = serving.serve(db, rate=rate, n_requests=n_requests, seed=4)
+    res.pop("_requests")
+    out["serving"] = {"workload": "config4: 50,000 molecule-like graphs, 90% support queries / 10% mining",
+                      **{k: v for k, v in res.items()}}
+
+
 def _copy_calibration(mib: int) -> None:
     """One device-to-device hipMemcpy of mib MiB, which the HIP runtime runs
     as a blit kernel (__amd_rocclr_copyBuffer) that reads and writes exactly
@@ -136,7 +152,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--jobs", type=int, default=0, help="also run the first N config-3 jobs batched vs sequential")
+    ap.add_argument("--jobs", type=int, default=256,
+                    help="batching: the first N jobs of the 256-job mix, batched vs one at a time (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=400,
+                    help="serving: requests against the resident 50,000-graph molecule database (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="serving: Poisson arrival rate, requests/s")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the run, make one device-to-device copy of MIB MiB (a blit kernel that reads and "
                          "writes MIB MiB): the reference that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -237,6 +257,8 @@ def main() -> int:
     line["roofline"] = roof
     if args.jobs:
         _batch_section(args.jobs, line)
+    if args.serve_requests:
+        _serving_section(args.serve_requests, args.serve_rate, line)
     if args.calibrate_copy:
         _copy_calibration(args.calibrate_copy)
     print(json.dumps(line), flush=True)

@@ -45,7 +45,7 @@ EXPORTED_SYMBOLS = (
     "gm_version", "gm_last_error", "gm_cfg_init", "gm_device_count", "gm_db_create", "gm_db_load",
     "gm_db_write_txt", "gm_db_counts", "gm_db_export", "gm_db_upload", "gm_db_destroy", "gm_mine",
     "gm_mine_batch", "gm_dfs_code_is_min", "gm_gen_random", "gm_gen_planted", "gm_gen_molecule", "gm_db_subset",
-    "gm_batch_create", "gm_batch_mine", "gm_batch_destroy",
+    "gm_batch_create", "gm_batch_mine", "gm_batch_destroy", "gm_support_queries", "gm_support_query",
 )
 
 
@@ -114,6 +114,8 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "gm_batch_create": (C.c_int, [C.c_int32, P(dbp), P(dbp)]),
         "gm_batch_mine": (C.c_int, [dbp, i32p, P(gm_cfg), JOB_PATTERN_CB, C.c_void_p, P(gm_stats)]),
         "gm_batch_destroy": (None, [dbp]),
+        "gm_support_queries": (C.c_int, [dbp, C.c_int32, i32p, i32p, P(gm_cfg), P(C.c_uint32), P(gm_stats)]),
+        "gm_support_query": (C.c_int, [dbp, i32p, C.c_int32, P(C.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -473,6 +475,26 @@ def mine_batch(dbs: Sequence[GraphDB], minsups: Sequence[int], *, label_style: i
     st = gm_stats()
     _check(lib.gm_mine_batch(len(dbs), handles, _ptr(ms), C.byref(cfg), ccb, None, C.byref(st)))
     return res, st.as_dict()
+
+
+def support(db: GraphDB, codes: Sequence[Sequence[Sequence[int]]], *, with_stats: bool = False):
+    """Support (number of graphs containing the pattern) of each query code
+    -- any valid DFS code of a connected, fully labelled pattern -- from one
+    batched projection over the resident database (gm_support_queries).
+    Returns a list of ints (and the run statistics with with_stats)."""
+    lib = load_library()
+    n = len(codes)
+    lens = np.array([len(c) for c in codes], dtype=np.int32)
+    flat = np.array([v for c in codes for t in c for v in t], dtype=np.int32)
+    if flat.size != 5 * int(lens.sum()):
+        raise ValueError("every DFS tuple needs 5 integers")
+    out = np.zeros(max(n, 1), dtype=np.uint32)
+    cfg = _make_cfg()
+    st = gm_stats()
+    _check(lib.gm_support_queries(db._h, n, _ptr(flat) if flat.size else None, _ptr(lens) if n else None,
+                                  C.byref(cfg), out.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(st)))
+    res = [int(x) for x in out[:n]]
+    return (res, st.as_dict()) if with_stats else res
 
 
 class Batch:

@@ -74,12 +74,11 @@ inline bool same_tuple(const DFS &a, const DFS &b) {
   return true;
 }
 
-}  // namespace
-
-bool dfs_code_is_min(const DFS *code, int n) {
-  if (n <= 1) return true;
+// The pattern graph of a DFS code (DFSCode::toGraph,
+// src/globals/dfs_code.cpp:245-265): vertex labels from the tuples that
+// carry them (-1 carries none), edges in code order.
+void code_graph(const DFS *code, int n, PGraph &G) {
   if (n > kMaxE) throw Error(GM_E_UNSUPPORTED, "pattern larger than 64 edges");
-  PGraph G;
   G.nv = 0;
   for (int i = 0; i < n; ++i) G.nv = std::max(G.nv, std::max(code[i].from, code[i].to) + 1);
   if (G.nv > kMaxV) throw Error(GM_E_UNSUPPORTED, "pattern larger than 64 vertices");
@@ -99,6 +98,13 @@ bool dfs_code_is_min(const DFS *code, int n) {
     G.el[b][G.deg[b]] = d.elabel;
     G.eid[b][G.deg[b]++] = i;
   }
+}
+
+// Grow the minimum DFS code of G (n edges) greedily.  With want != nullptr
+// stop at the first tuple that differs from want (returns false) or when
+// the code is complete (true); with want == nullptr write the whole minimum
+// code to out (returns true).
+bool grow_min(const PGraph &G, int n, const DFS *want_code, DFS *out) {
 
   // Step 1: smallest root tuple over directed edges with fromlabel <= tolabel.
   int bfl = 1 << 30, bel = 1 << 30, btl = 1 << 30;
@@ -134,7 +140,6 @@ bool dfs_code_is_min(const DFS *code, int n) {
     mc.build_rm();
     const int minlabel = mc.t[0].fromlabel;
     const int maxtoc = mc.t[mc.rm[0]].to;
-    const DFS &want = code[mc.n];
     nxt.clear();
 
     // Backward: rightmost-path edges rm[i], i from the root end down to 1.
@@ -158,7 +163,7 @@ bool dfs_code_is_min(const DFS *code, int n) {
       }
       if (found) {
         DFS t{maxtoc, e1.from, -1, best, -1};
-        if (!same_tuple(want, t)) return false;
+        if (want_code && !same_tuple(want_code[mc.n], t)) return false;
         for (const Emb &e : cur) {
           int r = e.map[e2.to], target = e.map[e1.from];
           int e1_tolabel = G.label[e.map[e1.to]], e2_tolabel = G.label[r];
@@ -219,9 +224,12 @@ bool dfs_code_is_min(const DFS *code, int n) {
         }
       }
     }
-    if (newfrom < 0) return true;  // no extension: the whole pattern is covered
+    if (newfrom < 0) {
+      if (want_code) return true;  // no extension: the whole pattern is covered
+      throw Error(GM_E_ARG, "pattern is not connected");
+    }
     DFS t{newfrom, maxtoc + 1, -1, bel2, btl2};
-    if (!same_tuple(want, t)) return false;
+    if (want_code && !same_tuple(want_code[mc.n], t)) return false;
     for (const Emb &e : cur) {
       int u = e.map[newfrom];
       int x = newfrom == maxtoc ? -1 : -2;
@@ -251,7 +259,41 @@ bool dfs_code_is_min(const DFS *code, int n) {
     mc.t[mc.n++] = t;
     cur.swap(nxt);
   }
+  if (out) {
+    // full labels: vertex labels of the minimum code's own numbering
+    int vl[kMaxV];
+    vl[0] = mc.t[0].fromlabel;
+    vl[1] = mc.t[0].tolabel;
+    for (int i = 1; i < mc.n; ++i)
+      if (mc.t[i].from < mc.t[i].to) vl[mc.t[i].to] = mc.t[i].tolabel;
+    for (int i = 0; i < mc.n; ++i) {
+      out[i] = mc.t[i];
+      out[i].fromlabel = vl[mc.t[i].from];
+      out[i].tolabel = vl[mc.t[i].to];
+    }
+  }
   return true;
+}
+
+}  // namespace
+
+bool dfs_code_is_min(const DFS *code, int n) {
+  if (n <= 1) return true;
+  PGraph G;
+  code_graph(code, n, G);
+  return grow_min(G, n, code, nullptr);
+}
+
+void min_dfs_code(const DFS *code, int n, std::vector<DFS> &out) {
+  out.clear();
+  if (n <= 0) return;
+  for (int i = 0; i < n; ++i)
+    if (code[i].from < code[i].to && (code[i].tolabel == -1 || (i == 0 && code[i].fromlabel == -1)))
+      throw Error(GM_E_ARG, "query code must carry every vertex label");
+  PGraph G;
+  code_graph(code, n, G);
+  out.resize(n);
+  grow_min(G, n, nullptr, out.data());
 }
 
 }  // namespace gm

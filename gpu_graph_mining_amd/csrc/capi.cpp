@@ -6,6 +6,7 @@
 // launch failures, src/cuda_pp/cuda_tools.hpp:10-18, kernel_execution.hpp:63-68).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -252,6 +253,63 @@ int gm_mine(gm_db *db, int32_t abs_minsup, const gm_cfg *cfg, gm_pattern_cb cb, 
     };
     if (!gm::mine(*db->dev, o, rep, stats, db->stream)) throw gm::Error(GM_E_CALLBACK, "callback stopped mining");
   });
+}
+
+int gm_support_queries(gm_db *db, int32_t n_queries, const int32_t *dfs5, const int32_t *n_edges, const gm_cfg *cfg,
+                       uint32_t *supports, gm_stats *stats) {
+  return guard([&]() {
+    if (!db || n_queries < 0 || (n_queries > 0 && (!dfs5 || !n_edges || !supports)))
+      throw gm::Error(GM_E_ARG, "null argument");
+    // minimum codes first: malformed queries fail before any device work
+    std::vector<std::vector<gm::DFS>> mins(n_queries);
+    std::vector<gm::DFS> code;
+    const int32_t *row = dfs5;
+    for (int32_t q = 0; q < n_queries; ++q) {
+      supports[q] = 0;
+      const int32_t n = n_edges[q];
+      if (n < 1) throw gm::Error(GM_E_ARG, "query " + std::to_string(q) + " has no edges");
+      code.resize(n);
+      for (int32_t i = 0; i < n; ++i, row += 5) code[i] = gm::DFS{row[0], row[1], row[2], row[3], row[4]};
+      gm::min_dfs_code(code.data(), n, mins[q]);
+      int32_t nv = 0;
+      for (const gm::DFS &t : mins[q]) nv = std::max(nv, std::max(t.from, t.to) + 1);
+      if (nv > gm::kMaxPatV) throw gm::Error(GM_E_UNSUPPORTED, "query pattern with more than 32 vertices");
+    }
+    ensure_device(db);
+    const gm::DeviceDB &d = *db->dev;
+    auto dense = [](const std::vector<int32_t> &values, int32_t v) {
+      auto it = std::lower_bound(values.begin(), values.end(), v);
+      return (it == values.end() || *it != v) ? -1 : (int32_t)(it - values.begin());
+    };
+    gm::QueryTrie trie;
+    for (int32_t q = 0; q < n_queries; ++q) {
+      bool present = true;  // every label occurs in the database
+      for (gm::DFS &t : mins[q]) {
+        t.fromlabel = dense(d.vl_values, t.fromlabel);
+        t.tolabel = dense(d.vl_values, t.tolabel);
+        t.elabel = dense(d.el_values, t.elabel);
+        if (t.fromlabel < 0 || t.tolabel < 0 || t.elabel < 0) present = false;
+      }
+      if (present) trie.add(mins[q], q);
+    }
+    if (n_queries == 0 || trie.nodes.size() == 1) {
+      if (stats) std::memset(stats, 0, sizeof(*stats));
+      return;
+    }
+    if (d.n_jobs != 1) throw gm::Error(GM_E_ARG, "queries run against a single database");
+    gm::MineOptions o = options(cfg);
+    o.minsup = {1};
+    o.queries = &trie;
+    std::vector<uint32_t> res(n_queries, 0);
+    o.query_support = &res;
+    gm::ReportFn none = [](int32_t, const std::vector<gm::DFS> &, uint32_t, uint64_t) { return true; };
+    gm::mine(*db->dev, o, none, stats, db->stream);
+    std::memcpy(supports, res.data(), sizeof(uint32_t) * n_queries);
+  });
+}
+
+int gm_support_query(gm_db *db, const int32_t *dfs5, int32_t n_edges, uint32_t *support) {
+  return gm_support_queries(db, 1, dfs5, &n_edges, nullptr, support, nullptr);
 }
 
 int gm_batch_create(int32_t n_jobs, gm_db *const *dbs, gm_batch **out) {

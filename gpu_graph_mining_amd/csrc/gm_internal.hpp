@@ -69,4 +69,10 @@ struct DFS {
 // sense of the sequential gSpan's is_min (src/gspan/ismin.cpp:36-140).
 bool dfs_code_is_min(const DFS *code, int n);
 
+// The minimum DFS code of the pattern a code describes (any valid DFS code
+// of a connected pattern; vertex labels carried by its forward tuples),
+// fully labelled, in canonical.cpp.  Throws Error(GM_E_ARG) for codes that
+// do not describe a connected pattern.
+void min_dfs_code(const DFS *code, int n, std::vector<DFS> &out);
+
 }  // namespace gm

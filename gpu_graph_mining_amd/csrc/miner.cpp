@@ -87,6 +87,31 @@ int64_t round4(int64_t x) { return (x + 3) & ~int64_t(3); }
 
 }  // namespace
 
+int32_t QueryTrie::add(const std::vector<DFS> &code, int32_t query) {
+  int32_t node = 0;
+  for (const DFS &t : code) {
+    int32_t next = child(node, t);
+    if (next < 0) {
+      next = (int32_t)nodes.size();
+      nodes[node].kids.emplace_back(t, next);
+      nodes.emplace_back();
+    }
+    node = next;
+  }
+  nodes[node].ends.push_back(query);
+  return node;
+}
+
+int32_t QueryTrie::child(int32_t node, const DFS &t) const {
+  for (const auto &k : nodes[node].kids) {
+    const DFS &a = k.first;
+    if (a.from == t.from && a.to == t.to && a.fromlabel == t.fromlabel && a.elabel == t.elabel &&
+        a.tolabel == t.tolabel)
+      return k.second;
+  }
+  return -1;
+}
+
 void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk) {
   out.n_jobs = (int32_t)dbs.size();
   std::vector<int32_t> vls, els;
@@ -211,7 +236,9 @@ struct Pat {
   std::vector<uint8_t> vlab;  // dense label of each pattern vertex
   int32_t job = 0;
   uint32_t sup = 0;
+  int32_t node = 0;  // query mode: trie node of the code
 };
+
 
 PatMeta make_meta(const Pat &p) {
   PatMeta m;
@@ -250,12 +277,15 @@ class Pool {
     return p;
   }
   // Run fn(i) for i in [0, n) on up to `threads` threads (the caller helps).
+  // Callers on different threads (concurrent mining calls on different
+  // handles) take turns: the pool runs one function at a time.
   void run(int64_t n, int threads, const std::function<void(int64_t)> &fn) {
     if (n <= 0) return;
     if (threads <= 1 || n < 64) {
       for (int64_t i = 0; i < n; ++i) fn(i);
       return;
     }
+    std::lock_guard<std::mutex> one_at_a_time(run_m_);
     std::unique_lock<std::mutex> lk(m_);
     while ((int)workers_.size() < threads - 1) workers_.emplace_back([this]() { loop(); });
     fn_ = &fn;
@@ -309,7 +339,7 @@ class Pool {
       if (--active_ == 0) done_.notify_all();
     }
   }
-  std::mutex m_, err_m_;
+  std::mutex run_m_, m_, err_m_;
   std::condition_variable cv_, done_;
   std::vector<std::thread> workers_;
   const std::function<void(int64_t)> *fn_ = nullptr;
@@ -455,6 +485,14 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   // GM_LEVEL_TRACE=1: one line per level step on stderr (sizes, timings)
   const char *trace_env = getenv("GM_LEVEL_TRACE");
   const bool trace_levels = trace_env && trace_env[0] == '1';
+
+  // query mode: record the supports of the queries that end at c's node;
+  // c is kept (its rows written) only when longer queries continue from it
+  auto query_reached = [&](const Pat &c) {
+    const QueryTrie::Node &nd = opt.queries->nodes[c.node];
+    for (int32_t q : nd.ends) (*opt.query_support)[q] = c.sup;  // each query ends at one node
+    return !nd.kids.empty();
+  };
 
   auto to_original = [&](const std::vector<DFS> &c) {
     std::vector<DFS> o(c);
@@ -646,6 +684,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         c.job = (int32_t)j;
         c.code.push_back(DFS{0, 1, fl, el, tl});
         c.vlab = {(uint8_t)fl, (uint8_t)tl};
+        if (opt.queries) {
+          c.node = opt.queries->child(0, c.code[0]);
+          keep[i] = c.node >= 0 && query_reached(c);
+          return;
+        }
         keep[i] = 1;
         return;
       }
@@ -663,6 +706,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         const int nl = (int)(y % NVL), el = (int)((y / NVL) % NEL), f = (int)(y / NVL / NEL);
         c.code.push_back(DFS{f, nv, par.vlab[f], el, nl});
         c.vlab.push_back((uint8_t)nl);
+      }
+      if (opt.queries) {  // follow the query codes only
+        c.node = opt.queries->child(par.node, c.code.back());
+        keep[i] = c.node >= 0 && query_reached(c) && (int)c.vlab.size() <= kMaxPatV;
+        return;
       }
       if (opt.max_edges > 0 && (int)c.code.size() > opt.max_edges) return;
       if (!dfs_code_is_min(c.code.data(), (int)c.code.size())) return;
@@ -778,9 +826,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       st.e_root += child_rows;
     else
       st.e_child += child_rows;
-    for (int64_t i = 0; i < C && !stopped; ++i) {
-      st.n_patterns++;
-      if (!report(next[i].job, to_original(next[i].code), next[i].sup, nproj[i])) stopped = true;
+    if (!opt.queries) {
+      for (int64_t i = 0; i < C && !stopped; ++i) {
+        st.n_patterns++;
+        if (!report(next[i].job, to_original(next[i].code), next[i].sup, nproj[i])) stopped = true;
+      }
     }
     if (trace_levels)
       fprintf(stderr,

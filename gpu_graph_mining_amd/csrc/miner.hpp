@@ -90,6 +90,19 @@ void upload_db(const PackedDB &pk, DeviceDB &out, hipStream_t s);
 // |Proj| (0 unless collected).
 typedef std::function<bool(int32_t job, const std::vector<DFS> &code, uint32_t sup, uint64_t n_proj)> ReportFn;
 
+// Pattern-support queries as a trie of minimum DFS codes (dense labels):
+// node 0 is the empty code; a node's children extend its code by one tuple;
+// `ends` lists the queries whose code ends at the node.
+struct QueryTrie {
+  struct Node {
+    std::vector<std::pair<DFS, int32_t>> kids;
+    std::vector<int32_t> ends;
+  };
+  std::vector<Node> nodes{Node()};
+  int32_t add(const std::vector<DFS> &code, int32_t query);  // returns the final node
+  int32_t child(int32_t node, const DFS &t) const;           // -1 when absent
+};
+
 struct MineOptions {
   std::vector<int32_t> minsup;  // per job
   int32_t max_edges = 0;
@@ -97,6 +110,11 @@ struct MineOptions {
   int32_t block_threads = 0;   // 0 = by rows per graph
   bool collect_proj = false;
   bool candidate_lists = true;  // k_support stores candidates for k_write
+  // Query mode: instead of mining, follow the codes of `queries` (every
+  // prefix of a minimum code is one) and store each query's support in
+  // query_support; the report callback is not called.
+  const QueryTrie *queries = nullptr;
+  std::vector<uint32_t> *query_support = nullptr;
 };
 
 // Runs the level-synchronous miner; returns false when the report callback

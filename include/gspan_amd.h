@@ -177,6 +177,23 @@ int gm_batch_mine(gm_batch *batch, const int32_t *abs_minsup, const gm_cfg *cfg,
                   gm_stats *stats);
 void gm_batch_destroy(gm_batch *batch);
 
+/* ---- pattern-support queries ------------------------------------------ */
+/* Support (number of database graphs containing the pattern) of each of
+ * n_queries patterns, given as DFS codes: query q has n_edges[q] rows of
+ * (from, to, fromlabel, elabel, tolabel) in dfs5, the queries one after the
+ * other.  A code may be any valid DFS code of a connected pattern in which
+ * every vertex label is given (forward rows carry tolabel, the first row
+ * fromlabel); it need not be minimal.  All queries run in one batched
+ * projection over the resident database (level steps shared by all
+ * queries).  supports[q] = 0 when the pattern does not occur.  No reference
+ * counterpart in the mining path: the reference's support of a code is
+ * gSpan::support over its projection (src/gspan/gspan.cpp:88-102), which
+ * these queries reproduce for any code. */
+int gm_support_queries(gm_db *db, int32_t n_queries, const int32_t *dfs5, const int32_t *n_edges,
+                       const gm_cfg *cfg, uint32_t *supports, gm_stats *stats);
+/* One query (gm_support_queries with n_queries = 1, default options). */
+int gm_support_query(gm_db *db, const int32_t *dfs5, int32_t n_edges, uint32_t *support);
+
 /* ---- canonical form ---------------------------------------------------- */
 /* Minimum-DFS-code check of one code (n rows of from,to,fromlabel,elabel,
  * tolabel; either label style).  Returns 1 minimal, 0 not minimal, or a

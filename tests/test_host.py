@@ -226,3 +226,32 @@ def test_error_paths_before_device_work(tmp_path):
     assert _err(lambda: gm.GraphDB.load(bad, "xml")) == gm.GM_E_ARG
     assert _err(lambda: gm.GraphDB.from_arrays([0, 2], [0, 1], [0, 1], [0], [7], [0])) == gm.GM_E_ARG
     assert _err(lambda: gm.DFSCode([(0, 0, 1, 1, 1), (0, 1, 1, 1, 1)]).is_min()) == gm.GM_E_ARG
+
+
+def test_support_queries_reject_malformed_codes_before_device_work():
+    """Disconnected patterns, missing labels and empty queries are argument
+    errors raised on the host (no GPU needed)."""
+    db = gm.GraphDB.from_graphs([([0, 1, 2], [(0, 1, 0), (1, 2, 0)])])
+    for codes in ([[(0, 1, 0, 0, 1), (2, 3, 1, 0, 2)]],   # two components
+                  [[(0, 1, 0, 0, -1)]],                   # forward tuple without its to-label
+                  [[]]):
+        with pytest.raises(gm.GMError) as e:
+            gm.support(db, codes)
+        assert e.value.code == gm.GM_E_ARG
+
+
+def test_serving_request_mix_is_seeded_and_connected():
+    from gpu_graph_mining_amd import serving
+    db = gm.GraphDB.molecule(4, 300)
+    a = serving.make_requests(db, 40, seed=5)
+    b = serving.make_requests(db, 40, seed=5)
+    assert [(r.kind, r.code, r.graphs, r.minsup) for r in a] == [(r.kind, r.code, r.graphs, r.minsup) for r in b]
+    for r in a:
+        if r.kind == serving.QUERY:
+            assert 3 <= len(r.code) <= 12
+            seen = {r.code[0][0]}
+            for f, t, *_ in r.code:  # every tuple touches the part grown so far
+                assert f in seen or t in seen
+                seen |= {f, t}
+        else:
+            assert r.minsup >= 1 and len(r.graphs) == len(set(r.graphs))
