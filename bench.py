@@ -227,7 +227,7 @@ def main() -> int:
     }
 
     roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-            "traffic": _traffic("k_support", config["workload"]), "kernel": "k_support (extension + support)",
+            "traffic": _traffic("k_support", config["workload"]), "kernel": "k_expand + k_support (extension + support stage of a level step)",
             "launches_per_step": launches / args.steps,
             "avg_launch_ms": (ext_s / launches * 1e3) if launches else None}
     if not args.no_cpu:

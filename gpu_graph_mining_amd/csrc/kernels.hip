@@ -11,7 +11,7 @@
 //   k_children the child rows (pattern id + tuple) of the selected keys,
 //              read from the candidate lists (or, for a graph whose lists
 //              were not kept, from a fresh enumeration): a count pass, a
-//              one-workgroup scan (k_scan) and a write pass, which also lay
+//              tiled scan (k_scan_*) and a write pass, which also lay
 //              out the next level's candidate lists.
 // The first level runs the same two kernels over 1-edge keys.
 //
@@ -1046,60 +1046,112 @@ __global__ void __launch_bounds__(kMaxBlock) k_children(LevelArgs a) {
     children_enum<ROOT, false, WRITE>(a, g, r0, r1, l, sh);
 }
 
-// One workgroup of kScanBlock threads: cgstart = exclusive scan of cgcount,
-// ngcbase = exclusive scan of gbsum (or -1 where the region would pass the
-// store or the 32-bit offsets), totals into *cursor / *ncand_cursor.  A
-// graph whose rows would pass ccap gets no rows (the host compares the
-// total with the embedding count and fails the call).
-constexpr int kScanBlock = 1024;
+// Placement of the children, three launches over tiles of kScanTile graphs
+// (one graph per thread): k_scan_tiles sums each tile's child rows, list
+// entries (gbsum) and big graphs; k_scan_top (one workgroup) turns the tile
+// sums into exclusive tile offsets and writes the totals (*cursor = rows,
+// *ncand_cursor = list entries, *nnbig = big graphs of the next level,
+// cursor[2] = the largest graph's rows); k_scan_apply writes per graph
+// cgstart = exclusive scan of cgcount, ngcbase = exclusive scan of gbsum
+// (-1 where the region would pass the store or the 32-bit offsets) and the
+// next level's big-graph list in graph order.  A graph whose rows would pass
+// ccap gets no rows (the host compares the total with the embedding count
+// and fails the call).
+constexpr int kScanTile = kScanTileGraphs;
 
-__global__ void __launch_bounds__(kScanBlock) k_scan(LevelArgs a, int n) {
-  __shared__ long long s[kScanBlock / 64 + 1];
-  __shared__ unsigned long long su[kScanBlock / 64 + 1];
-  long long rows = 0;
-  unsigned long long ents = 0;
-  const int per = (n + kScanBlock - 1) / kScanBlock;  // graphs per thread, contiguous
-  const int g0 = min(n, (int)threadIdx.x * per), g1 = min(n, g0 + per);
-  long long my_rows = 0;
-  unsigned long long my_ents = 0;
-  for (int g = g0; g < g1; ++g) {
-    my_rows += a.cgcount[g];
-    my_ents += a.gbsum[g];
-  }
-  long long trows;
-  unsigned long long tents;
-  rows = block_exclusive_scan<long long>(my_rows, trows, s);
-  ents = block_exclusive_scan<unsigned long long>(my_ents, tents, su);
-  for (int g = g0; g < g1; ++g) {
-    a.cgstart[g] = rows;
-    const long long c = a.cgcount[g];
-    if (rows + c > a.ccap) a.cgcount[g] = 0;  // past the allocation: not written; the host sees the total
-    rows += c;
-    const unsigned long long b = a.gbsum[g];
-    long long nb = -1;
-    if (a.next_lists && b > 0 && b <= 0xFFFFFFFFull && ents + b <= (unsigned long long)a.ncand_cap) nb = (long long)ents;
-    a.ngcbase[g] = nb;
-    ents += b;
-  }
-  // the next level's big graphs, in graph order
-  int my_big = 0;
-  for (int g = g0; g < g1; ++g) my_big += a.cgcount[g] > a.big_rows ? 1 : 0;
-  int tbig;
-  int at = block_exclusive_scan<int>(my_big, tbig, reinterpret_cast<int *>(s));
-  for (int g = g0; g < g1; ++g)
-    if (a.cgcount[g] > a.big_rows) a.nbig_list[at++] = g;
-  if (threadIdx.x == 0) *a.nnbig = (unsigned long long)tbig;
-  long long my_max = 0;
-  for (int g = g0; g < g1; ++g) my_max = max(my_max, (long long)a.cgcount[g]);
-  for (int o = 32; o > 0; o >>= 1) my_max = max(my_max, (long long)__shfl_down(my_max, o, 64));
-  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = my_max;
+__device__ __forceinline__ void tile_values(const LevelArgs &a, int g, int n, long long &c, unsigned long long &b,
+                                            int &big) {
+  c = g < n ? a.cgcount[g] : 0;
+  b = g < n ? a.gbsum[g] : 0;
+  big = c > a.big_rows ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kScanTile) k_scan_tiles(LevelArgs a, int n) {
+  __shared__ long long s[kScanTile / 64];
+  __shared__ unsigned long long su[kScanTile / 64];
+  __shared__ int si[kScanTile / 64];
+  const int g = blockIdx.x * kScanTile + threadIdx.x;
+  long long c;
+  unsigned long long b;
+  int big;
+  tile_values(a, g, n, c, b, big);
+  long long m = c;
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (long long)__shfl_down(m, o, 64));
+  const long long rows = block_sum<long long>(c, s);
+  const unsigned long long ents = block_sum<unsigned long long>(b, su);
+  const int bigs = block_sum<int>(big, si);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int i = 1; i < kScanBlock / 64; ++i) my_max = max(my_max, s[i]);
-    *a.cursor = (unsigned long long)trows;
-    *a.ncand_cursor = tents;
-    a.cursor[2] = (unsigned long long)my_max;  // largest graph's child rows
+    for (int i = 1; i < kScanTile / 64; ++i) m = max(m, s[i]);
+    a.tile_rows[blockIdx.x] = rows;
+    a.tile_ents[blockIdx.x] = ents;
+    a.tile_big[blockIdx.x] = bigs;
+    a.tile_max[blockIdx.x] = m;
   }
+}
+
+__global__ void __launch_bounds__(kScanTile) k_scan_top(LevelArgs a, int ntiles) {
+  __shared__ long long s[kScanTile / 64 + 1];
+  __shared__ unsigned long long su[kScanTile / 64 + 1];
+  __shared__ int si[kScanTile / 64 + 1];
+  long long crow = 0, cmax = 0;
+  unsigned long long cent = 0;
+  int cbig = 0;
+  for (int base = 0; base < ntiles; base += kScanTile) {
+    const int t = base + threadIdx.x;
+    const bool live = t < ntiles;
+    long long tr, tm = live ? a.tile_max[t] : 0;
+    unsigned long long te;
+    int tb;
+    const long long r = block_exclusive_scan<long long>(live ? a.tile_rows[t] : 0, tr, s);
+    const unsigned long long e = block_exclusive_scan<unsigned long long>(live ? a.tile_ents[t] : 0, te, su);
+    const int bg = block_exclusive_scan<int>(live ? (int)a.tile_big[t] : 0, tb, si);
+    if (live) {
+      a.tile_rows[t] = crow + r;
+      a.tile_ents[t] = cent + e;
+      a.tile_big[t] = cbig + bg;
+    }
+    crow += tr;
+    cent += te;
+    cbig += tb;
+    for (int o = 32; o > 0; o >>= 1) tm = max(tm, (long long)__shfl_down(tm, o, 64));
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = tm;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int i = 0; i < kScanTile / 64; ++i) cmax = max(cmax, s[i]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.cursor[0] = (unsigned long long)crow;
+    *a.ncand_cursor = cent;
+    a.cursor[2] = (unsigned long long)cmax;  // largest graph's child rows
+    *a.nnbig = (unsigned long long)cbig;
+  }
+}
+
+__global__ void __launch_bounds__(kScanTile) k_scan_apply(LevelArgs a, int n) {
+  __shared__ long long s[kScanTile / 64 + 1];
+  __shared__ unsigned long long su[kScanTile / 64 + 1];
+  __shared__ int si[kScanTile / 64 + 1];
+  const int g = blockIdx.x * kScanTile + threadIdx.x;
+  long long c;
+  unsigned long long b;
+  int big;
+  tile_values(a, g, n, c, b, big);
+  long long tr;
+  unsigned long long te;
+  int tb;
+  const long long rows = a.tile_rows[blockIdx.x] + block_exclusive_scan<long long>(c, tr, s);
+  const unsigned long long ents = a.tile_ents[blockIdx.x] + block_exclusive_scan<unsigned long long>(b, te, su);
+  const int at = (int)a.tile_big[blockIdx.x] + block_exclusive_scan<int>(big, tb, si);
+  if (g >= n) return;
+  a.cgstart[g] = rows;
+  if (rows + c > a.ccap) a.cgcount[g] = 0;  // past the allocation: not written; the host sees the total
+  long long nb = -1;
+  if (a.next_lists && b > 0 && b <= 0xFFFFFFFFull && ents + b <= (unsigned long long)a.ncand_cap) nb = (long long)ents;
+  a.ngcbase[g] = nb;
+  if (big) a.nbig_list[at] = g;
 }
 
 // ------------------------------------------------------------ primitives
@@ -1169,7 +1221,10 @@ hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t
     hipLaunchKernelGGL((k_children<false, false, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
     if (big) hipLaunchKernelGGL((k_children<false, false, true>), dim3(a.big_grid), dim3(b.block), lds_big, s, b);
   }
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, s, a, n_graphs);
+  const unsigned tiles = (unsigned)((n_graphs + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanTile), 0, s, a, n_graphs);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanTile), 0, s, a, (int)tiles);
+  hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanTile), 0, s, a, n_graphs);
   if (root) {
     hipLaunchKernelGGL((k_children<true, true, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
   } else {

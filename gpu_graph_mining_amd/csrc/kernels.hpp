@@ -12,6 +12,7 @@ namespace gm {
 // the level's average rows per graph.  A row's children number at most
 // 65535 (uint16 counts): a row extends by at most 32 * 256 edges.
 constexpr int kMaxBlock = 256;
+constexpr int kScanTileGraphs = 1024;  // graphs per tile of the placement scan
 
 // Statistics counters that every workgroup bumps are spread over kSpread
 // cache lines: atomics on one address serialise across the device.
@@ -108,6 +109,11 @@ struct LevelArgs {
   int32_t big_rows, big_block, big_grid;
   int32_t *nbig_list;
   unsigned long long *nnbig;
+  // placement scan: per tile of 1024 graphs, sums then exclusive offsets
+  long long *tile_rows;
+  unsigned long long *tile_ents;
+  long long *tile_big;
+  long long *tile_max;
 };
 
 size_t level_lds_bytes(const LevelArgs &a, bool support);

@@ -435,6 +435,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     grow(w.big[i], sizeof(int32_t) * std::max(G, 1), s);
   }
   grow(w.nbig, 2 * sizeof(unsigned long long), s);
+  {
+    const int64_t tiles = (G + kScanTileGraphs - 1) / kScanTileGraphs + 1;
+    grow(w.scan_tiles, 4 * sizeof(long long) * tiles, s);
+  }
   // the root level's lists sit at the adjacency offsets
   if (opt.candidate_lists) grow(w.cand[0], sizeof(uint2) * std::max<int64_t>(db.n_adj, 1), s);
 
@@ -465,6 +469,14 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   // graphs are read from HBM)
   a.stage_v = db.max_graph_v + 1;
   a.big_rows = kBigRows;
+  {
+    const int64_t tiles = (G + kScanTileGraphs - 1) / kScanTileGraphs + 1;
+    long long *t = w.scan_tiles.as<long long>();
+    a.tile_rows = t;
+    a.tile_ents = reinterpret_cast<unsigned long long *>(t + tiles);
+    a.tile_big = t + 2 * tiles;
+    a.tile_max = t + 3 * tiles;
+  }
   a.big_block = kBigBlock;
   a.stage_a = std::min<int32_t>(db.max_graph_adj, 4096);
   int dev = 0, lds_max = 0;

@@ -50,6 +50,7 @@ struct Workspace {
   // selected keys (upl), candidate lists (cand/gcb/rcoff), big-graph lists
   DevBuf gs[2], gc[2], pid[2], rgraph[2], vt[2], upl[2], cand[2], gcb[2], rcoff[2], big[2];
   DevBuf nbig;  // big-graph counts of the two buffered levels
+  DevBuf scan_tiles;  // placement scan: per-tile sums and offsets
   HostBuf up, down;
   // the replica counters hold zeros outside a level step (k_reduce clears
   // what it reads); false after a call that ended early
