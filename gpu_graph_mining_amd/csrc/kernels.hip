@@ -771,6 +771,208 @@ __global__ void __launch_bounds__(kMaxBlock) k_support(LevelArgs a) {
     support_body<ROOT, false>(a, g, r0, r1, l, sh);
 }
 
+// ------------------------------------------------------------ ROOT WAVES
+// The root level deduplicates one graph per wave: a persistent grid of
+// waves loops over the graph ids, each wave with its own LDS table, so no
+// workgroup barrier is needed and the kernel arguments are loaded once per
+// wave rather than once per graph -- databases of many small graphs (the
+// batched job mix: hundreds of thousands) spend most of a per-graph
+// workgroup's life on that fixed cost.  Each lane enumerates one vertex
+// (adjacency read from HBM) and writes its list at the vertex's adjacency
+// offset.  Overflow splits the graph's pass by residue class of the vertex
+// label as in support_body; with lists, the split passes read them back.
+constexpr int kWaveBlock = 256;    // 4 waves per workgroup
+constexpr int kWaveTableBits = 9;  // per-wave dedup table: 512 keys
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T>
+__device__ __forceinline__ T wave_exclusive_scan(T x, T &total) {
+  const int lane = threadIdx.x & 63;
+  T v = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const T y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  total = __shfl(v, 63, 64);
+  return v - x;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// Per-wave LDS (32-bit words): table keys [cap], counts [cap], item prefix
+// [64], row records [64 x 2] (list offset, key base), split stack [2 x 32],
+// fill and overflow flags.
+constexpr int kWaveWords = (2 << kWaveTableBits) + 64 + 128 + 64 + 2;
+
+__global__ void __launch_bounds__(kWaveBlock) k_root_wave(LevelArgs a, int n_graphs) {
+  __shared__ uint32_t wl[(kWaveBlock / 64) * kWaveWords];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t *base = wl + wid * kWaveWords;
+  uint32_t *table = base;
+  uint32_t *tcnt = base + (1 << kWaveTableBits);
+  int *pre = reinterpret_cast<int *>(base + (2 << kWaveTableBits));
+  uint32_t *rec = base + (2 << kWaveTableBits) + 64;
+  uint32_t *stack_mod = rec + 128, *stack_res = rec + 160;
+  int *fill = reinterpret_cast<int *>(rec + 192);
+  int *over_flag = fill + 1;
+  const int NEL = a.NEL, NVL = a.NVL;
+  unsigned long long my_cand = 0, my_bound = 0, my_fill = 0;
+  const int total_waves = gridDim.x * (kWaveBlock / 64);
+  for (int g = blockIdx.x * (kWaveBlock / 64) + wid; g < n_graphs; g += total_waves) {
+    const int32_t vb = a.gvbase[g];
+    const int nvert = a.gvbase[g + 1] - vb;
+    if (nvert == 0) continue;
+    const int job = a.gjob[g];
+    if (job < a.p_lo || job >= a.p_hi) continue;
+    uint2 *lists = a.use_lists ? a.cand : nullptr;
+    GView G;
+    G.vadj = a.vadj + vb;
+    G.A = a.adj;
+    G.aoff = 0;
+    // the graph's adjacency entries bound its distinct root keys
+    const unsigned long long ents = (unsigned long long)(G.end(nvert - 1) - G.begin(0));
+    int tb = 6;
+    while (tb < kWaveTableBits && ((1ull << tb) * 3) / 4 < ents) ++tb;
+    const uint32_t cap = 1u << tb, fill_limit = cap - cap / 4;
+    unsigned long long *rep = a.acc_rep + (int64_t)(g & (a.rep - 1)) * a.nkeys;
+    const uint32_t kb = (uint32_t)((int64_t)(job - a.p_lo) * a.K);
+    bool replay = false;
+    int top = 1;
+    if (lane == 0) {
+      stack_mod[0] = 1;
+      stack_res[0] = 0;
+    }
+    wave_sync();
+    while (top > 0) {
+      const uint32_t mod = stack_mod[top - 1], res = stack_res[top - 1];
+      for (uint32_t i = lane; i < cap; i += 64) {
+        table[i] = kEmpty32;
+        tcnt[i] = 0;
+      }
+      if (lane == 0) {
+        *fill = 0;
+        *over_flag = 0;
+      }
+      wave_sync();
+      unsigned long long cand = 0, bound_sum = 0;
+      int over = 0;
+      auto insert = [&](uint32_t k) {
+        if (over || __hip_atomic_load(over_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return;
+        uint32_t h = hash_slot32(k, tb);
+        for (uint32_t probe = 0; probe < cap; ++probe) {
+          const uint32_t old = atomicCAS(&table[h], kEmpty32, k);
+          if (old == kEmpty32 || old == k) {
+            atomicAdd(&tcnt[h], 1u);
+            if (old == kEmpty32 && (uint32_t)atomicAdd(fill, 1) >= fill_limit) {
+              over = 1;
+              __hip_atomic_store(over_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            return;
+          }
+          h = (h + 1) & (cap - 1);
+        }
+        over = 1;
+        __hip_atomic_store(over_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      };
+      if (!replay) {  // one vertex per lane
+        for (int u = lane; u < nvert; u += 64) {
+          const int lu = a.vlab[vb + u];
+          if (mod > 1 && ((uint32_t)lu & (mod - 1)) != res) continue;
+          uint2 *out = lists ? lists + G.begin(u) : nullptr;
+          int nc = 0;
+          enumerate_root(G, u, lu, NEL, NVL, [&](int key, int w, int bound) {
+            insert(kb + (uint32_t)key);
+            ++cand;
+            bound_sum += (unsigned long long)bound;
+            if (out) out[nc++] = make_uint2(cand_code(true, key, w), (uint32_t)bound);
+          });
+          if (out) a.rccnt[vb + u] = (uint16_t)nc;
+        }
+      } else {  // read the lists back, one entry per lane
+        for (int cb = 0; cb < nvert; cb += 64) {
+          const int u = cb + lane;
+          int n = 0;
+          if (u < nvert) {
+            const int lu = a.vlab[vb + u];
+            if (!(mod > 1 && ((uint32_t)lu & (mod - 1)) != res)) {
+              n = a.rccnt[vb + u];
+              rec[2 * lane] = (uint32_t)G.begin(u);
+            }
+          }
+          int T;
+          pre[lane] = wave_exclusive_scan<int>(n, T);
+          wave_sync();
+          const int nrow = min(64, nvert - cb);
+          for (int j = lane; j < T; j += 64) {
+            const int k = find_row(pre, j, nrow);
+            const uint2 c = lists[rec[2 * k] + (j - pre[k])];
+            int key, w;
+            cand_decode(true, c.x, key, w);
+            insert(kb + (uint32_t)key);
+            ++cand;
+            bound_sum += c.y;
+          }
+          wave_sync();  // the next chunk reuses pre and rec
+        }
+      }
+      wave_sync();
+      const bool failed = __hip_atomic_load(over_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0 ||
+                          __any(over);
+      if (failed) {
+        const bool give_up = mod >= (1u << 30) || top + 1 > 32;
+        if (lane == 0) {
+          atomicAdd(a.split_passes, 1ull);
+          if (give_up) {
+            atomicOr(a.error_flag, 1);
+          } else {
+            stack_mod[top - 1] = mod * 2;
+            stack_res[top - 1] = res + mod;
+            stack_mod[top] = mod * 2;
+            stack_res[top] = res;
+          }
+        }
+        my_fill = max(my_fill, 2ull * cap);
+        if (give_up) break;
+        top += 1;
+        if (lists) replay = true;  // every vertex's list is complete
+        wave_sync();
+        continue;
+      }
+      my_cand += cand;
+      my_bound += bound_sum;
+      for (uint32_t i = lane; i < cap; i += 64) {
+        const uint32_t k = table[i];
+        if (k == kEmpty32) continue;
+        atomicAdd(&rep[(int64_t)a.p_lo * a.K + (int64_t)k], (1ull << kSupShift) | (unsigned long long)tcnt[i]);
+      }
+      my_fill = max(my_fill, (unsigned long long)*fill);
+      top -= 1;
+      wave_sync();
+    }
+  }
+  // statistics: one atomic per wave each, spread over kSpread lines
+  my_cand = wave_sum(my_cand);
+  my_bound = wave_sum(my_bound);
+  if (lane == 0) {
+    unsigned long long *line =
+        a.spread + ((blockIdx.x * (kWaveBlock / 64) + wid) & (kSpread - 1)) * kSpreadStride;
+    if (my_cand) atomicAdd(&line[kSpreadCand], my_cand);
+    if (my_bound) atomicAdd(&line[kSpreadBound], my_bound);
+    if (my_fill) atomicMax(&line[kSpreadFill], my_fill);
+  }
+}
+
 // Sum the replicas of every key (and clear them for the next level); keys
 // reaching their job's min support go to the frequent list as
 // (key, support, embeddings).
@@ -1177,7 +1379,9 @@ size_t level_lds_bytes(const LevelArgs &a, bool support) {
 hipError_t launch_support(const LevelArgs &a, int n_graphs, bool root, hipStream_t s) {
   if (n_graphs <= 0) return hipSuccess;
   if (root) {
-    hipLaunchKernelGGL((k_support<true, false>), dim3(n_graphs), dim3(a.block), level_lds_bytes(a, true), s, a);
+    // per-wave kernel: enough waves to fill the device, never more than graphs
+    const unsigned wave_grid = (unsigned)std::min<int64_t>((n_graphs + 3) / 4, a.wave_grid);
+    hipLaunchKernelGGL(k_root_wave, dim3(wave_grid), dim3(kWaveBlock), 0, s, a, n_graphs);
   } else {
     hipLaunchKernelGGL((k_support<false, false>), dim3(n_graphs), dim3(a.block), level_lds_bytes(a, true), s, a);
     if (a.big_grid > 0) {

@@ -482,6 +482,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   int dev = 0, lds_max = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   hip_check(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+  {
+    int cus = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    a.wave_grid = std::max(1, cus) * 8;  // 8 workgroups of 4 waves per CU
+  }
 
   std::vector<Pat> pats;  // patterns of the current level
   bool stopped = false;
