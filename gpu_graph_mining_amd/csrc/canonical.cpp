@@ -119,7 +119,12 @@ bool grow_min(const PGraph &G, int n, const DFS *want_code, DFS *out) {
         btl = tl;
       }
     }
-  std::vector<Emb> cur, nxt;
+  // embedding lists, reused by later calls on this thread (no allocation
+  // once warm; grow_min does not recurse)
+  thread_local std::vector<Emb> cur_store, nxt_store;
+  std::vector<Emb> &cur = cur_store, &nxt = nxt_store;
+  cur.clear();
+  nxt.clear();
   for (int u = 0; u < G.nv; ++u)
     for (int k = 0; k < G.deg[u]; ++k) {
       int v = G.nbr[u][k];

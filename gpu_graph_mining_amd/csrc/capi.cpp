@@ -81,9 +81,9 @@ gm_db *new_db(gm::HostDB &&h) {
   return db;
 }
 
-void flatten(const std::vector<gm::DFS> &code, int style, std::vector<int32_t> &out) {
-  out.resize(code.size() * 5);
-  for (size_t i = 0; i < code.size(); ++i) {
+void flatten(const gm::DFS *code, int32_t n, int style, std::vector<int32_t> &out) {
+  out.resize((size_t)n * 5);
+  for (int32_t i = 0; i < n; ++i) {
     const gm::DFS &d = code[i];
     out[5 * i + 0] = d.from;
     out[5 * i + 1] = d.to;
@@ -246,11 +246,12 @@ int gm_mine(gm_db *db, int32_t abs_minsup, const gm_cfg *cfg, gm_pattern_cb cb, 
     o.minsup = {abs_minsup};
     const int style = cfg ? cfg->label_style : GM_LABELS_FULL;
     std::vector<int32_t> flat;
-    gm::ReportFn rep = [&](int32_t, const std::vector<gm::DFS> &code, uint32_t sup, uint64_t nproj) {
-      if (!cb) return true;
-      flatten(code, style, flat);
-      return cb(user, flat.data(), (int32_t)code.size(), sup, nproj) == 0;
-    };
+    gm::ReportFn rep;
+    if (cb)
+      rep = [&](int32_t, const gm::DFS *code, int32_t n, uint32_t sup, uint64_t nproj) {
+        flatten(code, n, style, flat);
+        return cb(user, flat.data(), n, sup, nproj) == 0;
+      };
     if (!gm::mine(*db->dev, o, rep, stats, db->stream)) throw gm::Error(GM_E_CALLBACK, "callback stopped mining");
   });
 }
@@ -302,8 +303,7 @@ int gm_support_queries(gm_db *db, int32_t n_queries, const int32_t *dfs5, const 
     o.queries = &trie;
     std::vector<uint32_t> res(n_queries, 0);
     o.query_support = &res;
-    gm::ReportFn none = [](int32_t, const std::vector<gm::DFS> &, uint32_t, uint64_t) { return true; };
-    gm::mine(*db->dev, o, none, stats, db->stream);
+    gm::mine(*db->dev, o, gm::ReportFn(), stats, db->stream);
     std::memcpy(supports, res.data(), sizeof(uint32_t) * n_queries);
   });
 }
@@ -341,11 +341,12 @@ int gm_batch_mine(gm_batch *batch, const int32_t *abs_minsup, const gm_cfg *cfg,
     o.minsup.assign(abs_minsup, abs_minsup + n_jobs);
     const int style = cfg ? cfg->label_style : GM_LABELS_FULL;
     std::vector<int32_t> flat;
-    gm::ReportFn rep = [&](int32_t job, const std::vector<gm::DFS> &code, uint32_t sup, uint64_t nproj) {
-      if (!cb) return true;
-      flatten(code, style, flat);
-      return cb(user, job, flat.data(), (int32_t)code.size(), sup, nproj) == 0;
-    };
+    gm::ReportFn rep;
+    if (cb)
+      rep = [&](int32_t job, const gm::DFS *code, int32_t n, uint32_t sup, uint64_t nproj) {
+        flatten(code, n, style, flat);
+        return cb(user, job, flat.data(), n, sup, nproj) == 0;
+      };
     if (!gm::mine(batch->dev, o, rep, stats, batch->stream))
       throw gm::Error(GM_E_CALLBACK, "callback stopped mining");
   });

@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <condition_variable>
 #include <functional>
 #include <mutex>
@@ -230,37 +231,114 @@ void upload_db(const PackedDB &pk, DeviceDB &out, hipStream_t s) {
 
 namespace {
 
-// Host view of one pattern of the current level.
-struct Pat {
-  std::vector<DFS> code;      // dense labels, fully labelled
-  std::vector<uint8_t> vlab;  // dense label of each pattern vertex
-  int32_t job = 0;
-  uint32_t sup = 0;
-  int32_t node = 0;  // query mode: trie node of the code
+// Growable host array without value-initialisation, reused across levels
+// (its contents are not kept when it grows).
+template <class T>
+struct Arr {
+  std::unique_ptr<T[]> p;
+  int64_t n = 0, cap = 0;
+  void resize(int64_t k) {
+    if (k > cap) {
+      const int64_t want = std::max<int64_t>(k, cap + cap / 2);
+      p.reset(new T[want]);
+      cap = want;
+    }
+    n = k;
+  }
+  T *data() { return p.get(); }
+  const T *data() const { return p.get(); }
+  T &operator[](int64_t i) { return p[i]; }
+  const T &operator[](int64_t i) const { return p[i]; }
 };
 
+// The patterns of one level, flat.  Level-synchronous growth adds one
+// tuple per level, so every code of a level has the same length.
+struct Level {
+  int len = 0;         // tuples per code
+  int64_t n = 0;       // patterns
+  Arr<DFS> code;       // n * len tuples, dense labels, fully labelled
+  Arr<uint8_t> vlab;   // kMaxPatV dense vertex labels per pattern
+  Arr<uint8_t> nv;     // vertices per pattern
+  Arr<int32_t> job;
+  Arr<uint32_t> sup;
+  Arr<int32_t> node;   // query mode: trie node of the code
+  void resize(int64_t k, int l) {
+    n = k;
+    len = l;
+    code.resize(k * l);
+    vlab.resize(k * kMaxPatV);
+    nv.resize(k);
+    job.resize(k);
+    sup.resize(k);
+    node.resize(k);
+  }
+  const DFS *c(int64_t i) const { return code.data() + i * len; }
+  DFS *c(int64_t i) { return code.data() + i * len; }
+  const uint8_t *vl(int64_t i) const { return vlab.data() + i * kMaxPatV; }
+  uint8_t *vl(int64_t i) { return vlab.data() + i * kMaxPatV; }
+};
 
-PatMeta make_meta(const Pat &p) {
+PatMeta make_meta(const DFS *code, int len, const uint8_t *vlab, int nv, int32_t job) {
   PatMeta m;
   std::memset(&m, 0, sizeof(m));
-  m.job = p.job;
-  const int nv = (int)p.vlab.size();
+  m.job = job;
   m.nv = (uint8_t)nv;
-  m.minlabel = p.vlab[0];
-  m.maxlabel = p.vlab[nv - 1];
+  m.minlabel = vlab[0];
+  m.maxlabel = vlab[nv - 1];
   // rightmost path (src/globals/dfs_code.cpp:75-90)
   int old_from = -1, nrm = 0;
-  for (int i = (int)p.code.size() - 1; i >= 0; --i) {
-    const DFS &d = p.code[i];
+  for (int i = len - 1; i >= 0; --i) {
+    const DFS &d = code[i];
     if (d.from < d.to && (nrm == 0 || old_from == d.to)) {
-      m.rm[nrm++] = (uint32_t)d.from | ((uint32_t)d.elabel << 8) | ((uint32_t)p.vlab[d.to] << 16);
+      m.rm[nrm++] = (uint32_t)d.from | ((uint32_t)d.elabel << 8) | ((uint32_t)vlab[d.to] << 16);
       old_from = d.from;
     }
   }
   m.nrm = (uint8_t)nrm;
-  for (const DFS &d : p.code)
-    if (d.from == nv - 1 && d.to < d.from) m.bwd_mask |= 1u << d.to;
+  for (int i = 0; i < len; ++i)
+    if (code[i].from == nv - 1 && code[i].to < code[i].from) m.bwd_mask |= 1u << code[i].to;
   return m;
+}
+
+// One frequent key as k_reduce appends it: key, support, embeddings.
+struct Freq {
+  unsigned long long key, sup, proj;
+};
+
+// The frequent keys in ascending key order (the order of the level's
+// children, so that pattern ids do not depend on the device's append
+// order).  Keys are unique and below key_end; LSD radix sort over 11-bit
+// digits of the bits in use.
+void sort_freq(const Freq *in, int64_t n, uint64_t key_end, Arr<Freq> &out, Arr<Freq> &tmp) {
+  out.resize(n);
+  if (n < 4096) {
+    std::copy(in, in + n, out.data());
+    std::sort(out.data(), out.data() + n, [](const Freq &x, const Freq &y) { return x.key < y.key; });
+    return;
+  }
+  int bits = 1;
+  while (bits < 64 && (key_end >> bits)) ++bits;
+  const int passes = (bits + 10) / 11;
+  tmp.resize(n);
+  Freq *bufs[2] = {out.data(), tmp.data()};
+  int dst = (passes & 1) ? 0 : 1;  // the last pass writes `out`
+  const Freq *src = in;
+  std::vector<uint32_t> cnt(2048);
+  for (int ps = 0; ps < passes; ++ps) {
+    const int sh = 11 * ps;
+    std::fill(cnt.begin(), cnt.end(), 0u);
+    for (int64_t i = 0; i < n; ++i) cnt[(src[i].key >> sh) & 2047]++;
+    uint32_t run = 0;
+    for (uint32_t &c : cnt) {
+      const uint32_t x = c;
+      c = run;
+      run += x;
+    }
+    Freq *d = bufs[dst];
+    for (int64_t i = 0; i < n; ++i) d[cnt[(src[i].key >> sh) & 2047]++] = src[i];
+    src = d;
+    dst ^= 1;
+  }
 }
 
 struct Timer {
@@ -488,7 +566,13 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     a.wave_grid = std::max(1, cus) * 8;  // 8 workgroups of 4 waves per CU
   }
 
-  std::vector<Pat> pats;  // patterns of the current level
+  Level L, N;             // patterns of the current and of the next level
+  Arr<Freq> frall, fr, frtmp;
+  Arr<int64_t> kidx;      // per frequent key: child index, -1 if dropped
+  Arr<DFS> ktup;          // per frequent key: the tuple it adds
+  Arr<uint8_t> knv;       // per frequent key: the child's vertices
+  Arr<int32_t> knode;     // per frequent key (query mode): trie node
+  Arr<uint64_t> nproj;    // per child: embeddings reported
   bool stopped = false;
   int level = 0;          // 0 = root stage
   int S = 0;              // tuple stride of the current level
@@ -503,29 +587,20 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   const char *trace_env = getenv("GM_LEVEL_TRACE");
   const bool trace_levels = trace_env && trace_env[0] == '1';
 
-  // query mode: record the supports of the queries that end at c's node;
-  // c is kept (its rows written) only when longer queries continue from it
-  auto query_reached = [&](const Pat &c) {
-    const QueryTrie::Node &nd = opt.queries->nodes[c.node];
-    for (int32_t q : nd.ends) (*opt.query_support)[q] = c.sup;  // each query ends at one node
+  // query mode: record the supports of the queries that end at `node`; the
+  // child is kept (its rows written) only when longer queries continue
+  auto query_reached = [&](int32_t node, uint32_t sup) {
+    const QueryTrie::Node &nd = opt.queries->nodes[node];
+    for (int32_t q : nd.ends) (*opt.query_support)[q] = sup;  // each query ends at one node
     return !nd.kids.empty();
   };
-
-  auto to_original = [&](const std::vector<DFS> &c) {
-    std::vector<DFS> o(c);
-    for (DFS &d : o) {
-      d.fromlabel = db.vl_values[d.fromlabel];
-      d.tolabel = db.vl_values[d.tolabel];
-      d.elabel = db.el_values[d.elabel];
-    }
-    return o;
-  };
+  std::vector<DFS> orig;  // a reported code in the database's own labels
 
   for (;;) {
     const bool root = level == 0;
-    if (!root && pats.empty()) break;
+    if (!root && L.n == 0) break;
     Timer tdev;
-    const int64_t P = root ? db.n_jobs : (int64_t)pats.size();
+    const int64_t P = root ? db.n_jobs : L.n;
     const int64_t K = root ? (int64_t)NVL * NEL * NVL : (int64_t)S * NEL * (NVL + 1);
     const int64_t nkeys = P * K;
     // Packed support / embedding counters: `rep` replicas (summed and
@@ -665,77 +740,84 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       throw Error(GM_E_OOM, "frequent-key list overflow (" + std::to_string(cnt.fcount) + " keys)");
     last_fcount = cnt.fcount;
     last_fill = (uint32_t)max_fill;
-    struct Freq {
-      unsigned long long key, sup, proj;
-    };
-    std::vector<Freq> fr(cnt.fcount);
-    if (cnt.fcount) {
+    const int64_t F = cnt.fcount;
+    if (F) {
       const uint32_t have = std::min(cnt.fcount, guess);
-      std::memcpy(fr.data(), hbuf + sizeof(Counters), sizeof(Freq) * have);
-      if (cnt.fcount > have) {
+      const Freq *src = reinterpret_cast<const Freq *>(hbuf + sizeof(Counters));
+      if (cnt.fcount > have) {  // the rest of the list, after the part downloaded
+        frall.resize(F);
+        std::memcpy(frall.data(), src, sizeof(Freq) * have);
         const size_t rest = sizeof(Freq) * (cnt.fcount - have);
         char *h2 = static_cast<char *>(w.down.ensure(rest));
         hip_check(hipMemcpyAsync(h2, reinterpret_cast<const char *>(a.freq) + sizeof(Freq) * have, rest,
                                  hipMemcpyDeviceToHost, s), "download frequent keys");
         hip_check(hipStreamSynchronize(s), "download frequent keys");
-        std::memcpy(fr.data() + have, h2, rest);
+        std::memcpy(frall.data() + have, h2, rest);
+        src = frall.data();
       }
+      sort_freq(src, F, (uint64_t)nkeys, fr, frtmp);
     }
-    std::sort(fr.begin(), fr.end(), [](const Freq &x, const Freq &y) { return x.key < y.key; });
     dev_sec += tdev.sec();
 
-    // decode frequent keys into child codes; keep the minimal ones
+    // decode the frequent keys into child tuples; keep the minimal codes
     const double us_prep = t_host.sec() * 1e6;
     Timer tcan;
-    const int64_t F = (int64_t)fr.size();
-    std::vector<Pat> kids(F);
-    std::vector<char> keep(F, 0);
-    bool too_big = false;
+    const int new_len = root ? 1 : L.len + 1;
+    kidx.resize(F);
+    ktup.resize(F);
+    knv.resize(F);
+    knode.resize(F);
+    std::atomic<bool> too_big(false);
     Pool::get().run(F, threads, [&](int64_t i) {
       const uint64_t key = fr[i].key;
-      Pat &c = kids[i];
-      c.sup = (uint32_t)fr[i].sup;
-      if (root) {
-        const int64_t j = key / K, x = key % K;
-        const int fl = (int)(x / (NEL * NVL)), el = (int)((x / NVL) % NEL), tl = (int)(x % NVL);
-        c.job = (int32_t)j;
-        c.code.push_back(DFS{0, 1, fl, el, tl});
-        c.vlab = {(uint8_t)fl, (uint8_t)tl};
-        if (opt.queries) {
-          c.node = opt.queries->child(0, c.code[0]);
-          keep[i] = c.node >= 0 && query_reached(c);
-          return;
-        }
-        keep[i] = 1;
-        return;
-      }
       const int64_t p = key / K, x = key % K;
-      const Pat &par = pats[p];
-      c.job = par.job;
-      c.code = par.code;
-      c.vlab = par.vlab;
-      const int nv = (int)par.vlab.size(), maxtoc = nv - 1;
-      if (x < (int64_t)S * NEL) {
-        const int t = (int)(x / NEL), el = (int)(x % NEL);
-        c.code.push_back(DFS{maxtoc, t, par.vlab[maxtoc], el, par.vlab[t]});
+      const uint32_t sup = (uint32_t)fr[i].sup;
+      bool keep = false;
+      int32_t node = 0;
+      DFS t;
+      int cnv;
+      if (root) {
+        const int fl = (int)(x / (NEL * NVL)), el = (int)((x / NVL) % NEL), tl = (int)(x % NVL);
+        t = DFS{0, 1, fl, el, tl};
+        cnv = 2;
+        if (opt.queries) {
+          node = opt.queries->child(0, t);
+          keep = node >= 0 && query_reached(node, sup);
+        } else {
+          keep = true;
+        }
       } else {
-        const int64_t y = x - (int64_t)S * NEL;
-        const int nl = (int)(y % NVL), el = (int)((y / NVL) % NEL), f = (int)(y / NVL / NEL);
-        c.code.push_back(DFS{f, nv, par.vlab[f], el, nl});
-        c.vlab.push_back((uint8_t)nl);
+        const uint8_t *pv = L.vl(p);
+        const int nv = L.nv[p], maxtoc = nv - 1;
+        if (x < (int64_t)S * NEL) {
+          const int to = (int)(x / NEL), el = (int)(x % NEL);
+          t = DFS{maxtoc, to, pv[maxtoc], el, pv[to]};
+          cnv = nv;
+        } else {
+          const int64_t y = x - (int64_t)S * NEL;
+          const int nl = (int)(y % NVL), el = (int)((y / NVL) % NEL), f = (int)(y / NVL / NEL);
+          t = DFS{f, nv, pv[f], el, nl};
+          cnv = nv + 1;
+        }
+        if (opt.queries) {  // follow the query codes only
+          node = opt.queries->child(L.node[p], t);
+          keep = node >= 0 && query_reached(node, sup) && cnv <= kMaxPatV;
+        } else if (opt.max_edges <= 0 || new_len <= opt.max_edges) {
+          thread_local std::vector<DFS> code;
+          code.assign(L.c(p), L.c(p) + L.len);
+          code.push_back(t);
+          if (dfs_code_is_min(code.data(), new_len)) {
+            if (cnv > kMaxPatV)
+              too_big = true;
+            else
+              keep = true;
+          }
+        }
       }
-      if (opt.queries) {  // follow the query codes only
-        c.node = opt.queries->child(par.node, c.code.back());
-        keep[i] = c.node >= 0 && query_reached(c) && (int)c.vlab.size() <= kMaxPatV;
-        return;
-      }
-      if (opt.max_edges > 0 && (int)c.code.size() > opt.max_edges) return;
-      if (!dfs_code_is_min(c.code.data(), (int)c.code.size())) return;
-      if ((int)c.vlab.size() > kMaxPatV) {
-        too_big = true;
-        return;
-      }
-      keep[i] = 1;
+      kidx[i] = keep ? 0 : -1;
+      ktup[i] = t;
+      knv[i] = (uint8_t)std::min(cnv, 255);
+      knode[i] = node;
     });
     if (too_big) throw Error(GM_E_UNSUPPORTED, "pattern with more than 32 vertices is frequent");
     canon_sec += tcan.sec();
@@ -743,21 +825,18 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     Timer t_sel;
     st.n_candidates += root ? 0 : F;
 
-    double us_select = 0, us_host = 0;
-    std::vector<Pat> next;
-    std::vector<unsigned long long> sel_keys;
-    std::vector<int32_t> sel_vals;
-    std::vector<uint64_t> nproj;
-    int64_t child_rows = 0;
+    // children in key order
+    int64_t C = 0, child_rows = 0;
+    int max_nv = 0;
     for (int64_t i = 0; i < F; ++i) {
-      if (!keep[i]) continue;
-      sel_keys.push_back(fr[i].key);
-      sel_vals.push_back(-((int32_t)next.size() + 1));
-      nproj.push_back(fr[i].proj);
+      if (kidx[i] < 0) continue;
+      kidx[i] = C++;
       child_rows += (int64_t)fr[i].proj;
-      next.push_back(std::move(kids[i]));
+      max_nv = std::max(max_nv, (int)knv[i]);
     }
-    const int64_t C = (int64_t)next.size();
+    N.resize(C, new_len);
+    nproj.resize(C);
+    double us_select = 0, us_host = 0;
     int CS = 4;
     pending_rows = -1;
     bool lists_next = false;
@@ -766,8 +845,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       // reserves each graph's rows from a cursor.  One upload carries the
       // selected keys and the children's metadata (the next level's).
       Timer tdev2;
-      int max_nv = 0;
-      for (const Pat &c : next) max_nv = std::max(max_nv, (int)c.vlab.size());
       CS = (int)round4(max_nv);
       const int nxt = cur ^ 1;
       const size_t meta_bytes = sizeof(PatMeta) * C;
@@ -776,13 +853,39 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       const size_t up_bytes = vals_off + sizeof(int32_t) * C;
       char *hu = static_cast<char *>(w.up.ensure(up_bytes));
       PatMeta *hm = reinterpret_cast<PatMeta *>(hu);
+      unsigned long long *hk = reinterpret_cast<unsigned long long *>(hu + keys_off);
+      int32_t *hv = reinterpret_cast<int32_t *>(hu + vals_off);
+      Pool::get().run(F, threads, [&](int64_t i) {
+        const int64_t c = kidx[i];
+        if (c < 0) return;
+        const uint64_t key = fr[i].key;
+        const int64_t p = key / K;
+        DFS *cc = N.c(c);
+        uint8_t *cv = N.vl(c);
+        const DFS &t = ktup[i];
+        const int cnv = knv[i];
+        if (root) {
+          cc[0] = t;
+          cv[0] = (uint8_t)t.fromlabel;
+          cv[1] = (uint8_t)t.tolabel;
+          N.job[c] = (int32_t)p;
+        } else {
+          std::memcpy(cc, L.c(p), sizeof(DFS) * L.len);
+          cc[L.len] = t;
+          std::memcpy(cv, L.vl(p), L.nv[p]);
+          if (cnv > L.nv[p]) cv[cnv - 1] = (uint8_t)t.tolabel;
+          N.job[c] = L.job[p];
+        }
+        N.nv[c] = (uint8_t)cnv;
+        N.sup[c] = (uint32_t)fr[i].sup;
+        N.node[c] = knode[i];
+        nproj[c] = opt.collect_proj ? fr[i].proj : 0;
+        hm[c] = make_meta(cc, new_len, cv, cnv, N.job[c]);
+        hk[c] = key;
+        hv[c] = -(int32_t)(c + 1);
+      });
       int next_nrm = 0;
-      for (int64_t i = 0; i < C; ++i) {
-        hm[i] = make_meta(next[i]);
-        next_nrm = std::max(next_nrm, (int)hm[i].nrm);
-      }
-      std::memcpy(hu + keys_off, sel_keys.data(), sizeof(unsigned long long) * C);
-      std::memcpy(hu + vals_off, sel_vals.data(), sizeof(int32_t) * C);
+      for (int64_t c = 0; c < C; ++c) next_nrm = std::max(next_nrm, (int)hm[c].nrm);
       char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
       hip_check(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, s), "upload children");
       us_select = t_sel.sec() * 1e6;
@@ -838,15 +941,23 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       max_nrm = next_nrm;
     }
     lists_now = lists_next;
-    if (!opt.collect_proj) std::fill(nproj.begin(), nproj.end(), 0);
     if (root)
       st.e_root += child_rows;
     else
       st.e_child += child_rows;
     if (!opt.queries) {
-      for (int64_t i = 0; i < C && !stopped; ++i) {
-        st.n_patterns++;
-        if (!report(next[i].job, to_original(next[i].code), next[i].sup, nproj[i])) stopped = true;
+      if (!report) {
+        st.n_patterns += C;
+      } else {
+        orig.resize(new_len);
+        for (int64_t c = 0; c < C && !stopped; ++c) {
+          st.n_patterns++;
+          const DFS *cc = N.c(c);
+          for (int k = 0; k < new_len; ++k)
+            orig[k] = DFS{cc[k].from, cc[k].to, db.vl_values[cc[k].fromlabel], db.el_values[cc[k].elabel],
+                          db.vl_values[cc[k].tolabel]};
+          if (!report(N.job[c], orig.data(), new_len, N.sup[c], nproj[c])) stopped = true;
+        }
       }
     }
     if (trace_levels)
@@ -859,7 +970,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
               a.use_lists, us_prep, us_canon, us_select, us_host);
     st.n_levels++;
     if (stopped) break;
-    pats.swap(next);
+    std::swap(L, N);
     S = CS;
     n_rows = child_rows;
     ++level;

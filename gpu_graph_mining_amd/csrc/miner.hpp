@@ -89,7 +89,10 @@ void upload_db(const PackedDB &pk, DeviceDB &out, hipStream_t s);
 
 // Reported pattern: job, fully labelled code in original labels, support,
 // |Proj| (0 unless collected).
-typedef std::function<bool(int32_t job, const std::vector<DFS> &code, uint32_t sup, uint64_t n_proj)> ReportFn;
+// Called per frequent pattern, level by level, with its code in the
+// database's own labels; an empty function reports nothing (the patterns
+// are still counted).
+typedef std::function<bool(int32_t job, const DFS *code, int32_t n, uint32_t sup, uint64_t n_proj)> ReportFn;
 
 // Pattern-support queries as a trie of minimum DFS codes (dense labels):
 // node 0 is the empty code; a node's children extend its code by one tuple;
