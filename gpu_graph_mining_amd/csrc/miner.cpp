@@ -354,12 +354,14 @@ class Pool {
     static Pool p;
     return p;
   }
-  // Run fn(i) for i in [0, n) on up to `threads` threads (the caller helps).
-  // Callers on different threads (concurrent mining calls on different
-  // handles) take turns: the pool runs one function at a time.
-  void run(int64_t n, int threads, const std::function<void(int64_t)> &fn) {
+  // Run fn(i) for i in [0, n) on up to `threads` threads (the caller helps);
+  // below `serial_below` items the caller runs them alone (waking the
+  // workers costs tens of microseconds).  Callers on different threads
+  // (concurrent mining calls on different handles) take turns: the pool
+  // runs one function at a time.
+  void run(int64_t n, int threads, const std::function<void(int64_t)> &fn, int64_t serial_below = 64) {
     if (n <= 0) return;
-    if (threads <= 1 || n < 64) {
+    if (threads <= 1 || n < serial_below) {
       for (int64_t i = 0; i < n; ++i) fn(i);
       return;
     }
@@ -883,7 +885,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         hm[c] = make_meta(cc, new_len, cv, cnv, N.job[c]);
         hk[c] = key;
         hv[c] = -(int32_t)(c + 1);
-      });
+      }, 4096);
       int next_nrm = 0;
       for (int64_t c = 0; c < C; ++c) next_nrm = std::max(next_nrm, (int)hm[c].nrm);
       char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
