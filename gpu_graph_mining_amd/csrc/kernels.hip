@@ -442,10 +442,12 @@ __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
   s_pre[t] = pre;
   __syncthreads();
   int cnt = 0;  // candidates of row t listed so far
-  unsigned long long cand = 0, bound_sum = 0;
+  unsigned long long cand = 0, bound_sum = 0, pruned = 0;
+  const int ms = a.mask_stride;
   for (int jb = 0; jb < T; jb += kExpandRows) {
     const int j = jb + t;
     int k = 0, key = -1, w = 0, bound = 0;
+    bool drop = false;  // a valid extension the path rule removes
     if (j < T) {
       k = find_row(s_pre, j, nrow);
       const int i = j - s_pre[k];
@@ -463,6 +465,7 @@ __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
       G.A = a.adj;
       G.aoff = 0;
       const int own = (int)sp[nrm + 1];
+      const unsigned long long *mk = ms ? a.pmask + (int64_t)misc[0] * 2 * ms : nullptr;
       int src = 0;  // 0: the rightmost vertex; s >= 1: rightmost-path source s - 1
       while ((int)sp[src + 1] <= i) ++src;
       if (src == 0) {
@@ -474,6 +477,7 @@ __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
           if (nl >= minlabel) {
             key = fwd_base + (maxtoc * NEL + el) * NVL + nl;
             bound = own + G.deg(w);
+            drop = mk && !((mk[0] >> (el * NVL + nl)) & 1ull);
           }
         } else {
           const uint32_t bwd_mask = misc[2];
@@ -485,6 +489,7 @@ __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
             if (!((bwd_mask >> f) & 1u) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
               key = f * NEL + el;
               bound = own;
+              drop = mk && !((mk[ms + 1 + q] >> el) & 1ull);
             }
             break;
           }
@@ -501,7 +506,12 @@ __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
         if (!((vrow[w >> 6] >> (w & 63)) & 1ull) && nl >= minlabel && (e1 < el || (e1 == el && l1 <= nl))) {
           key = fwd_base + (f * NEL + el) * NVL + nl;
           bound = own - (int)sp[src] + G.deg(w);  // sources src - 1 .. root, plus w
+          drop = mk && !((mk[src] >> (el * NVL + nl)) & 1ull);
         }
+      }
+      if (drop) {  // counted as a candidate, never listed
+        key = -1;
+        ++pruned;
       }
     }
     const bool emit = key >= 0;
@@ -526,12 +536,14 @@ __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
   if (has_list) a.rccnt[r] = (uint16_t)cnt;
   // statistics: one atomic per workgroup each, spread over kSpread lines
   __shared__ unsigned long long s_red[2];
-  cand = block_sum<unsigned long long>(cand, s_red);
+  cand = block_sum<unsigned long long>(cand + pruned, s_red);
   bound_sum = block_sum<unsigned long long>(bound_sum, s_red);
+  pruned = block_sum<unsigned long long>(pruned, s_red);
   if (t == 0) {
     unsigned long long *line = a.spread + (blockIdx.x & (kSpread - 1)) * kSpreadStride;
     if (cand) atomicAdd(&line[kSpreadCand], cand);
     if (bound_sum) atomicAdd(&line[kSpreadBound], bound_sum);
+    if (pruned) atomicAdd(&line[kSpreadPruned], pruned);
   }
 }
 
@@ -771,18 +783,50 @@ __global__ void __launch_bounds__(kMaxBlock) k_support(LevelArgs a) {
     support_body<ROOT, false>(a, g, r0, r1, l, sh);
 }
 
+// Merge one graph's count for key k into a workgroup's accumulation table
+// (keys, graphs, embeddings; 2^bits entries each).  New keys stop being
+// admitted at 3/4 fill; returns false when k is not in the table then (the
+// caller commits it to HBM itself).
+__device__ __forceinline__ bool acc_add(uint32_t *acc, int bits, int *fill, uint32_t k, uint32_t emb) {
+  const uint32_t cap = 1u << bits, limit = cap - cap / 4;
+  uint32_t h = hash_slot32(k, bits);
+  for (uint32_t probe = 0; probe < cap; ++probe) {
+    uint32_t cur = __hip_atomic_load(&acc[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == kEmpty32) {
+      if (__hip_atomic_load(fill, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= (int)limit) return false;
+      cur = atomicCAS(&acc[h], kEmpty32, k);
+      if (cur == kEmpty32) {
+        atomicAdd(fill, 1);
+        cur = k;
+      }
+    }
+    if (cur == k) {
+      atomicAdd(&acc[cap + h], 1u);
+      atomicAdd(&acc[2 * cap + h], emb);
+      return true;
+    }
+    h = (h + 1) & (cap - 1);
+  }
+  return false;
+}
+
 // ------------------------------------------------------------ ROOT WAVES
 // The root level deduplicates one graph per wave: a persistent grid of
-// waves loops over the graph ids, each wave with its own LDS table, so no
-// workgroup barrier is needed and the kernel arguments are loaded once per
-// wave rather than once per graph -- databases of many small graphs (the
-// batched job mix: hundreds of thousands) spend most of a per-graph
-// workgroup's life on that fixed cost.  Each lane enumerates one vertex
+// workgroups, each over a contiguous range of graph ids that its waves take
+// in turn, each wave with its own LDS table, so no workgroup barrier is
+// needed and the kernel arguments are loaded once per wave rather than once
+// per graph -- databases of many small graphs (the batched job mix:
+// hundreds of thousands) spend most of a per-graph workgroup's life on that
+// fixed cost.  A graph's distinct keys are merged into the workgroup's LDS
+// accumulation table (graphs of one job share their root keys), committed
+// to the packed counters once at the end: one HBM atomic per key and
+// workgroup instead of one per key and graph.  Each lane enumerates one vertex
 // (adjacency read from HBM) and writes its list at the vertex's adjacency
 // offset.  Overflow splits the graph's pass by residue class of the vertex
 // label as in support_body; with lists, the split passes read them back.
 constexpr int kWaveBlock = 256;    // 4 waves per workgroup
 constexpr int kWaveTableBits = 9;  // per-wave dedup table: 512 keys
+constexpr int kRootAccBits = 9;    // workgroup accumulation table: 512 keys
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -817,6 +861,15 @@ constexpr int kWaveWords = (2 << kWaveTableBits) + 64 + 128 + 64 + 2;
 
 __global__ void __launch_bounds__(kWaveBlock) k_root_wave(LevelArgs a, int n_graphs) {
   __shared__ uint32_t wl[(kWaveBlock / 64) * kWaveWords];
+  __shared__ uint32_t racc[3 << kRootAccBits];
+  __shared__ int racc_fill;
+  for (int i = threadIdx.x; i < (1 << kRootAccBits); i += kWaveBlock) {
+    racc[i] = kEmpty32;
+    racc[(1 << kRootAccBits) + i] = 0;
+    racc[(2 << kRootAccBits) + i] = 0;
+  }
+  if (threadIdx.x == 0) racc_fill = 0;
+  __syncthreads();
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   uint32_t *base = wl + wid * kWaveWords;
   uint32_t *table = base;
@@ -828,8 +881,9 @@ __global__ void __launch_bounds__(kWaveBlock) k_root_wave(LevelArgs a, int n_gra
   int *over_flag = fill + 1;
   const int NEL = a.NEL, NVL = a.NVL;
   unsigned long long my_cand = 0, my_bound = 0, my_fill = 0;
-  const int total_waves = gridDim.x * (kWaveBlock / 64);
-  for (int g = blockIdx.x * (kWaveBlock / 64) + wid; g < n_graphs; g += total_waves) {
+  const int chunk = (n_graphs + gridDim.x - 1) / gridDim.x;
+  const int g_end = min(n_graphs, (int)(blockIdx.x + 1) * chunk);
+  for (int g = blockIdx.x * chunk + wid; g < g_end; g += kWaveBlock / 64) {
     const int32_t vb = a.gvbase[g];
     const int nvert = a.gvbase[g + 1] - vb;
     if (nvert == 0) continue;
@@ -954,11 +1008,24 @@ __global__ void __launch_bounds__(kWaveBlock) k_root_wave(LevelArgs a, int n_gra
       for (uint32_t i = lane; i < cap; i += 64) {
         const uint32_t k = table[i];
         if (k == kEmpty32) continue;
+        if (acc_add(racc, kRootAccBits, &racc_fill, k, tcnt[i])) continue;
         atomicAdd(&rep[(int64_t)a.p_lo * a.K + (int64_t)k], (1ull << kSupShift) | (unsigned long long)tcnt[i]);
       }
       my_fill = max(my_fill, (unsigned long long)*fill);
       top -= 1;
       wave_sync();
+    }
+  }
+  // the workgroup's accumulated keys
+  __syncthreads();
+  {
+    unsigned long long *rep = a.acc_rep + (int64_t)(blockIdx.x & (a.rep - 1)) * a.nkeys;
+    for (int i = threadIdx.x; i < (1 << kRootAccBits); i += kWaveBlock) {
+      const uint32_t k = racc[i];
+      if (k == kEmpty32) continue;
+      atomicAdd(&rep[(int64_t)a.p_lo * a.K + (int64_t)k],
+                ((unsigned long long)racc[(1 << kRootAccBits) + i] << kSupShift) |
+                    (unsigned long long)racc[(2 << kRootAccBits) + i]);
     }
   }
   // statistics: one atomic per wave each, spread over kSpread lines

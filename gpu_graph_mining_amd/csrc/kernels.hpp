@@ -21,6 +21,7 @@ constexpr int kSpreadStride = 16;  // 64-bit words per line
 constexpr int kSpreadCand = 0;     // extension candidates enumerated
 constexpr int kSpreadFill = 1;     // largest dedup table fill (atomicMax)
 constexpr int kSpreadBound = 2;    // sum of the candidates' child bounds
+constexpr int kSpreadPruned = 3;   // candidates dropped by the 2-edge path rule
 
 // Everything one level-step kernel reads or writes.  Plain device pointers.
 struct LevelArgs {
@@ -99,6 +100,14 @@ struct LevelArgs {
   // tuple, then the rightmost path), staged vertex-offset and adjacency words
   int32_t table_bits, slot_words, tuple_words, stage_v, stage_a;
   int32_t rm_words, src_words;  // k_expand row records: rightmost path, item prefix
+  // Path pruning masks (k_expand), 2 * mask_stride words per pattern:
+  // fwd[0] new edges at the rightmost vertex, fwd[1 + i] new edges at
+  // rightmost-path vertex rm[i].from (bit elabel * NVL + label of the new
+  // neighbour); bwd[1 + i] backward edges to rm[i].from (bit elabel).  A
+  // clear bit marks an extension one of whose 2-edge sub-paths is
+  // infrequent.  mask_stride 0: no pruning.
+  const unsigned long long *pmask;
+  int32_t mask_stride;
   int32_t block;  // threads per workgroup
   // Graphs with more than big_rows rows at this level run in a second launch
   // of big_grid workgroups of big_block threads over the list big[0, *nbig)

@@ -584,6 +584,21 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   uint32_t last_fcount = 0;
   uint32_t last_fill = 0;     // largest dedup table fill of the last level
   int max_nrm = 0;            // longest rightmost path among this level's patterns
+  // Path pruning: t2[((job * NVL) + centre label) * AR + arm] is the mask of
+  // the arms (elabel * NVL + neighbour label) that form a frequent 2-edge
+  // path with `arm` around a vertex of that label -- the job's frequent
+  // 2-edge patterns, known once the level of 2-edge children is selected.
+  // Every connected sub-pattern of a frequent pattern is frequent, so an
+  // extension whose new edge and an existing edge at either end form an
+  // infrequent path is dropped before it is listed (same results, fewer
+  // candidates).  Off in query mode (its 2-edge level follows the queries
+  // only) and when NEL * NVL > 64 (the masks are 64-bit).
+  const int AR = NEL * NVL;
+  const bool prune = opt.path_pruning && !opt.queries && AR <= 64;
+  std::vector<uint64_t> t2;
+  bool t2_ready = false;
+  size_t mask_off = 0;  // this level's masks in its pattern upload (stride 0: none)
+  int mask_stride = 0;
   bool lists_now = opt.candidate_lists;  // this level's rows have candidate lists
   // GM_LEVEL_TRACE=1: one line per level step on stderr (sizes, timings)
   const char *trace_env = getenv("GM_LEVEL_TRACE");
@@ -647,6 +662,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     a.vt = root ? nullptr : w.vt[cur].as<uint8_t>();
     a.S = S;
     a.meta = root ? nullptr : reinterpret_cast<const PatMeta *>(w.upl[cur].as<char>());
+    a.mask_stride = root ? 0 : mask_stride;
+    a.pmask = a.mask_stride ? reinterpret_cast<const unsigned long long *>(w.upl[cur].as<char>() + mask_off) : nullptr;
     a.K = K;
     a.sup = w.sup.as<int32_t>();
     a.nkeys = nkeys;
@@ -718,9 +735,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     Timer t_host;  // host work between the level's download and its upload
     Counters cnt;
     std::memcpy(&cnt, hbuf, sizeof(cnt));
-    unsigned long long ecand = 0, max_fill = 0, cand_bound = 0;
+    unsigned long long ecand = 0, max_fill = 0, cand_bound = 0, pruned = 0;
     for (int i = 0; i < kSpread; ++i) {
       ecand += cnt.spread[i * kSpreadStride + kSpreadCand];
+      pruned += cnt.spread[i * kSpreadStride + kSpreadPruned];
       max_fill = std::max(max_fill, cnt.spread[i * kSpreadStride + kSpreadFill]);
       cand_bound += cnt.spread[i * kSpreadStride + kSpreadBound];
     }
@@ -732,6 +750,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       st.e_cand += ecand;
     }
     st.split_passes += cnt.splits;
+    st.pruned += pruned;
     if (pending_rows >= 0) {
       check_write(cnt, pending_rows);
       st.max_graph_rows = std::max<uint64_t>(st.max_graph_rows, cnt.max_rows);
@@ -852,7 +871,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       const size_t meta_bytes = sizeof(PatMeta) * C;
       const size_t keys_off = (meta_bytes + 7) & ~size_t(7);
       const size_t vals_off = keys_off + sizeof(unsigned long long) * C;
-      const size_t up_bytes = vals_off + sizeof(int32_t) * C;
+      // path masks of the children with >= 2 edges: a forward and a
+      // backward word per rightmost-path vertex (at most max_nv of them)
+      const bool masks = prune && new_len >= 2;
+      const int mstride = masks ? max_nv : 0;
+      const size_t masks_off = (vals_off + sizeof(int32_t) * C + 7) & ~size_t(7);
+      const size_t up_bytes = masks_off + sizeof(uint64_t) * 2 * mstride * C;
       char *hu = static_cast<char *>(w.up.ensure(up_bytes));
       PatMeta *hm = reinterpret_cast<PatMeta *>(hu);
       unsigned long long *hk = reinterpret_cast<unsigned long long *>(hu + keys_off);
@@ -888,8 +912,24 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       }, 4096);
       int next_nrm = 0;
       for (int64_t c = 0; c < C; ++c) next_nrm = std::max(next_nrm, (int)hm[c].nrm);
+      if (masks && new_len == 2) {
+        // the job's frequent 2-edge patterns are these children: (0,1,a,e1,b)
+        // then (1,2,b,e2,c) (centre b) or (0,2,a,e2,c) (centre a)
+        t2.assign((size_t)db.n_jobs * NVL * AR, 0ull);
+        for (int64_t c = 0; c < C; ++c) {
+          const DFS *cc = N.c(c);
+          const bool at1 = cc[1].from == 1;
+          const int centre = at1 ? cc[0].tolabel : cc[0].fromlabel;
+          const int arm1 = cc[0].elabel * NVL + (at1 ? cc[0].fromlabel : cc[0].tolabel);
+          const int arm2 = cc[1].elabel * NVL + cc[1].tolabel;
+          uint64_t *row = &t2[((size_t)N.job[c] * NVL + centre) * AR];
+          row[arm1] |= 1ull << arm2;
+          row[arm2] |= 1ull << arm1;
+        }
+        t2_ready = true;
+      }
       char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
-      hip_check(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, s), "upload children");
+      hip_check(hipMemcpyAsync(du, hu, masks_off, hipMemcpyHostToDevice, s), "upload children");
       us_select = t_sel.sec() * 1e6;
       us_host = t_host.sec() * 1e6;
       hip_check(launch_scatter_i32(w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(du + keys_off),
@@ -932,6 +972,41 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.nnbig = w.nbig.as<unsigned long long>() + nxt;
       a.nrcoff = w.rcoff[nxt].as<uint32_t>();
       hip_check(launch_write(a, G, root, s), "k_write");
+      if (masks && t2_ready) {
+        // only the next level's k_expand reads the masks: computed while the
+        // device writes the children, uploaded behind them
+        uint64_t *hmask = reinterpret_cast<uint64_t *>(hu + masks_off);
+        Pool::get().run(C, threads, [&](int64_t c) {
+          uint64_t *fw = hmask + (size_t)c * 2 * mstride, *bw = fw + mstride;
+          const DFS *cc = N.c(c);
+          const uint8_t *cv = N.vl(c);
+          const PatMeta &m = hm[c];
+          const uint64_t *tj = &t2[(size_t)N.job[c] * NVL * AR];
+          // per vertex: the arms that form a frequent path with every
+          // pattern edge at it (one pass over the edges)
+          uint64_t vm[kMaxPatV];
+          for (int v = 0; v < m.nv; ++v) vm[v] = ~0ull;
+          for (int k = 0; k < new_len; ++k) {
+            const int u = cc[k].from, v = cc[k].to, e = cc[k].elabel * NVL;
+            vm[u] &= tj[(size_t)cv[u] * AR + e + cv[v]];
+            vm[v] &= tj[(size_t)cv[v] * AR + e + cv[u]];
+          }
+          const int rv = m.nv - 1;
+          fw[0] = vm[rv];
+          bw[0] = 0;
+          for (int i = 0; i < m.nrm; ++i) {
+            const int t = m.rm[i] & 0xFF;
+            fw[1 + i] = vm[t];
+            uint64_t b = 0;  // backward rv -> t: a new arm at both ends
+            if (i > 0)
+              for (int el = 0; el < NEL; ++el)
+                b |= ((vm[rv] >> (el * NVL + cv[t])) & (vm[t] >> (el * NVL + cv[rv])) & 1ull) << el;
+            bw[1 + i] = b;
+          }
+        }, 4096);
+        hip_check(hipMemcpyAsync(du + masks_off, hu + masks_off, up_bytes - masks_off, hipMemcpyHostToDevice, s),
+                  "upload path masks");
+      }
       pending_rows = child_rows;
       dev_sec += tdev2.sec();
       size_t live = db.bytes + w.sup.cap + w.acc_rep.cap + w.freq.cap +
@@ -941,6 +1016,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       peak = std::max(peak, live);
       cur = nxt;
       max_nrm = next_nrm;
+      mask_off = masks_off;
+      mask_stride = masks && t2_ready ? mstride : 0;
     }
     lists_now = lists_next;
     if (root)
@@ -965,10 +1042,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     if (trace_levels)
       fprintf(stderr,
               "[gm level %d] patterns %lld rows %lld K %lld rep %d block %d table_bits %d support_us %.1f "
-              "candidates %llu frequent %u children %lld child_rows %lld splits %llu max_fill %llu lists %d "
+              "candidates %llu pruned %llu frequent %u children %lld child_rows %lld splits %llu max_fill %llu lists %d "
               "host_us prep %.1f canon %.1f select %.1f total %.1f\n",
               level, (long long)P, (long long)(root ? db.n_vertices : n_rows), (long long)K, rep, a.block,
-              a.table_bits, ms * 1e3, ecand, cnt.fcount, (long long)C, (long long)child_rows, cnt.splits, max_fill,
+              a.table_bits, ms * 1e3, ecand, pruned, cnt.fcount, (long long)C, (long long)child_rows, cnt.splits, max_fill,
               a.use_lists, us_prep, us_canon, us_select, us_host);
     st.n_levels++;
     if (stopped) break;

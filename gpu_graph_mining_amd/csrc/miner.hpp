@@ -114,6 +114,7 @@ struct MineOptions {
   int32_t block_threads = 0;   // 0 = by rows per graph
   bool collect_proj = false;
   bool candidate_lists = true;  // k_support stores candidates for k_write
+  bool path_pruning = true;     // k_expand drops candidates with an infrequent 2-edge sub-path
   // Query mode: instead of mining, follow the codes of `queries` (every
   // prefix of a minimum code is one) and store each query's support in
   // query_support; the report callback is not called.

@@ -59,7 +59,9 @@ typedef struct gm_cfg {
   int32_t collect_proj;     /* 1 = also report |Proj(P)| per pattern (no extra pass) */
   int32_t candidate_lists;  /* 0 = the support pass keeps each embedding's extension candidates for
                                the write pass (default); 1 = the write pass enumerates again */
-  int32_t reserved[6];
+  int32_t path_pruning;     /* 0 = drop extension candidates that have an infrequent 2-edge sub-path
+                               (exact: such a child is infrequent; default); 1 = keep every candidate */
+  int32_t reserved[5];
 } gm_cfg;
 
 /* Per-run statistics (filled when a non-NULL pointer is passed). */
@@ -79,7 +81,8 @@ typedef struct gm_stats {
   uint64_t split_passes;   /* per-graph dedup passes redone in smaller parts */
   uint64_t max_graph_rows; /* most embeddings one graph held at one level   */
   uint64_t max_table_fill; /* most distinct extensions one graph produced in one pass */
-  uint64_t reserved[3];
+  uint64_t pruned;         /* candidates dropped by the 2-edge path rule (counted in e_cand) */
+  uint64_t reserved[2];
 } gm_stats;
 
 /* Pattern callback: n_edges rows of (from, to, fromlabel, elabel, tolabel),

@@ -165,6 +165,23 @@ def test_gpu_write_paths_agree(golden, name):
                                                                runs[0][1]["e_root"])
 
 
+@pytest.mark.parametrize("name", ["config1_1k", "chem_34", "config4_2k", "labeled_2"])
+def test_gpu_path_pruning_same_results(golden, name):
+    """Dropping the candidates whose new edge forms an infrequent 2-edge path
+    with an existing edge changes no pattern, support or |Proj|, and the
+    dropped candidates still count in E_cand (the oracle's push count)."""
+    case = golden[name]
+    db = case_db(case)
+    on, st_on = gm.mine(db, case["minsup"], collect_proj=True)
+    off, st_off = gm.mine(db, case["minsup"], collect_proj=True, path_pruning=False)
+    assert _by_code(on) == _by_code(off)
+    check_listing(listing(on), case)
+    assert (st_on["e_child"], st_on["e_cand"], st_on["e_root"]) == (st_off["e_child"], st_off["e_cand"],
+                                                                    st_off["e_root"])
+    assert st_off["pruned"] == 0
+    assert 0 <= st_on["pruned"] <= st_on["e_cand"]
+
+
 def test_gpu_bad_block_threads_rejected(golden):
     db = case_db(golden["chem_17"])
     with pytest.raises(gm.GMError) as e:
