@@ -1360,6 +1360,20 @@ __global__ void __launch_bounds__(kScanTile) k_scan_tiles(LevelArgs a, int n) {
   }
 }
 
+// Zero the counters the next level step's support stage accumulates (frequent
+// key count, error flag, split passes, spread statistics).  The level's own
+// values were downloaded before its write pass began.
+__device__ __forceinline__ void clear_level_counters(const LevelArgs &a) {
+  if (threadIdx.x == 0) {
+    *a.freq_count = 0;
+    *a.error_flag = 0;
+    *a.split_passes = 0;
+  }
+  for (int i = threadIdx.x; i < kSpread * kSpreadStride; i += blockDim.x) a.spread[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_clear_counters(LevelArgs a) { clear_level_counters(a); }
+
 __global__ void __launch_bounds__(kScanTile) k_scan_top(LevelArgs a, int ntiles) {
   __shared__ long long s[kScanTile / 64 + 1];
   __shared__ unsigned long long su[kScanTile / 64 + 1];
@@ -1397,6 +1411,72 @@ __global__ void __launch_bounds__(kScanTile) k_scan_top(LevelArgs a, int ntiles)
     a.cursor[2] = (unsigned long long)cmax;  // largest graph's child rows
     *a.nnbig = (unsigned long long)cbig;
   }
+  clear_level_counters(a);
+}
+
+// The three scan kernels in one workgroup, for up to kScanTile *
+// kScanSinglePer graphs: thread t places the consecutive graphs
+// [t * per, (t + 1) * per).
+constexpr int kScanSinglePer = 16;
+
+__global__ void __launch_bounds__(kScanTile) k_scan_single(LevelArgs a, int n) {
+  __shared__ long long s[kScanTile / 64 + 1];
+  __shared__ unsigned long long su[kScanTile / 64 + 1];
+  __shared__ int si[kScanTile / 64 + 1];
+  const int per = (n + kScanTile - 1) / kScanTile;  // <= kScanSinglePer (the launcher checks)
+  const int g0 = min(n, (int)threadIdx.x * per), g1 = min(n, g0 + per);
+  // all of the thread's loads first (one memory latency, not per graph)
+  long long cv[kScanSinglePer];
+  unsigned long long bv[kScanSinglePer];
+#pragma unroll
+  for (int k = 0; k < kScanSinglePer; ++k) {
+    const int g = g0 + k;
+    const bool in = g < g1;
+    cv[k] = in ? a.cgcount[g] : 0;
+    bv[k] = in ? a.gbsum[g] : 0;
+  }
+  long long csum = 0, cmax = 0;
+  unsigned long long bsum = 0;
+  int bigs = 0;
+#pragma unroll
+  for (int k = 0; k < kScanSinglePer; ++k) {
+    csum += cv[k];
+    bsum += bv[k];
+    bigs += cv[k] > a.big_rows ? 1 : 0;
+    cmax = max(cmax, cv[k]);
+  }
+  long long tr;
+  unsigned long long te;
+  int tb;
+  long long rows = block_exclusive_scan<long long>(csum, tr, s);
+  unsigned long long ents = block_exclusive_scan<unsigned long long>(bsum, te, su);
+  int at = block_exclusive_scan<int>(bigs, tb, si);
+#pragma unroll
+  for (int k = 0; k < kScanSinglePer; ++k) {
+    const int g = g0 + k;
+    if (g >= g1) break;
+    const long long c = cv[k];
+    const unsigned long long b = bv[k];
+    a.cgstart[g] = rows;
+    if (rows + c > a.ccap) a.cgcount[g] = 0;  // past the allocation: not written; the host sees the total
+    long long nb = -1;
+    if (a.next_lists && b > 0 && b <= 0xFFFFFFFFull && ents + b <= (unsigned long long)a.ncand_cap) nb = (long long)ents;
+    a.ngcbase[g] = nb;
+    if (c > a.big_rows) a.nbig_list[at++] = g;
+    rows += c;
+    ents += b;
+  }
+  for (int o = 32; o > 0; o >>= 1) cmax = max(cmax, (long long)__shfl_down(cmax, o, 64));
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = cmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kScanTile / 64; ++i) cmax = max(cmax, s[i]);
+    a.cursor[0] = (unsigned long long)tr;
+    *a.ncand_cursor = te;
+    a.cursor[2] = (unsigned long long)cmax;  // largest graph's child rows
+    *a.nnbig = (unsigned long long)tb;
+  }
+  clear_level_counters(a);
 }
 
 __global__ void __launch_bounds__(kScanTile) k_scan_apply(LevelArgs a, int n) {
@@ -1424,15 +1504,25 @@ __global__ void __launch_bounds__(kScanTile) k_scan_apply(LevelArgs a, int n) {
 }
 
 // ------------------------------------------------------------ primitives
-__global__ void k_scatter_i32(int32_t *dst, const uint64_t *idx, const int32_t *val, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[idx[i]] = val[i];
+// Copies between device memory and mapped pinned host memory on the compute
+// queue (a copy engine transfer between two kernels costs a cross-engine
+// wait; the level step's transfers are small).
+__global__ void k_copy8(unsigned long long *dst, const unsigned long long *src, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
-__global__ void k_fill_i32(int32_t *dst, int64_t n, int32_t v) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = v;
+// The children's upload (copied to device memory) and sup[key] = child id
+// for the selected keys (read from the upload's source).
+__global__ void k_upload_scatter(unsigned long long *dst, const unsigned long long *src, int64_t n, int32_t *sup,
+                                 const uint64_t *keys, const int32_t *vals, int64_t C) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = t; i < n; i += stride) dst[i] = src[i];
+  for (int64_t i = t; i < C; i += stride) sup[keys[i]] = vals[i];
 }
+
+
 
 }  // namespace
 
@@ -1479,6 +1569,33 @@ hipError_t launch_reduce(const LevelArgs &a, bool root, hipStream_t s) {
   return hipGetLastError();
 }
 
+static unsigned copy_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 2048)); }
+
+hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  if (bytes % 8) return hipErrorInvalidValue;
+  const int64_t n = (int64_t)(bytes / 8);
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy8, dim3(copy_grid(n)), dim3(256), 0, s, static_cast<unsigned long long *>(dst),
+                     static_cast<const unsigned long long *>(src), n);
+  return hipGetLastError();
+}
+
+hipError_t launch_upload_scatter(void *dst, const void *src, size_t bytes, int32_t *sup, const uint64_t *keys,
+                                 const int32_t *vals, int64_t C, hipStream_t s) {
+  if (bytes % 8) return hipErrorInvalidValue;
+  const int64_t n = (int64_t)(bytes / 8);
+  if (n == 0 && C == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_upload_scatter, dim3(copy_grid(std::max(n, C))), dim3(256), 0, s,
+                     static_cast<unsigned long long *>(dst), static_cast<const unsigned long long *>(src), n, sup,
+                     keys, vals, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_clear_counters(const LevelArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t s) {
   if (n_graphs <= 0) return hipSuccess;
   const size_t lds = level_lds_bytes(a, false);
@@ -1492,10 +1609,14 @@ hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t
     hipLaunchKernelGGL((k_children<false, false, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
     if (big) hipLaunchKernelGGL((k_children<false, false, true>), dim3(a.big_grid), dim3(b.block), lds_big, s, b);
   }
-  const unsigned tiles = (unsigned)((n_graphs + kScanTile - 1) / kScanTile);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanTile), 0, s, a, n_graphs);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanTile), 0, s, a, (int)tiles);
-  hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanTile), 0, s, a, n_graphs);
+  if (n_graphs <= kScanTile * kScanSinglePer) {
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanTile), 0, s, a, n_graphs);
+  } else {
+    const unsigned tiles = (unsigned)((n_graphs + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanTile), 0, s, a, n_graphs);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanTile), 0, s, a, (int)tiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3(tiles), dim3(kScanTile), 0, s, a, n_graphs);
+  }
   if (root) {
     hipLaunchKernelGGL((k_children<true, true, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
   } else {
@@ -1505,17 +1626,6 @@ hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t
   return hipGetLastError();
 }
 
-hipError_t launch_scatter_i32(int32_t *dst, const uint64_t *idx, const int32_t *val, int64_t n, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scatter_i32, dim3(grid1(n, 256)), dim3(256), 0, s, dst, idx, val, n);
-  return hipGetLastError();
-}
 
-hipError_t launch_fill_i32(int32_t *dst, int64_t n, int32_t v, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  const unsigned g = (unsigned)std::min<int64_t>(grid1(n, 256), 8192);
-  hipLaunchKernelGGL(k_fill_i32, dim3(g), dim3(256), 0, s, dst, n, v);
-  return hipGetLastError();
-}
 
 }  // namespace gm

@@ -132,8 +132,16 @@ hipError_t launch_reduce(const LevelArgs &a, bool root, hipStream_t s);
 hipError_t launch_support(const LevelArgs &a, int n_graphs, bool root, hipStream_t s);
 size_t expand_lds_bytes(const LevelArgs &a);
 hipError_t launch_expand(const LevelArgs &a, hipStream_t s);
+// Copy `bytes` (a multiple of 8) between device memory and mapped pinned
+// host memory with a kernel on stream s.
+hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s);
+// Copy the children's upload (bytes, a multiple of 8) and set
+// sup[keys[i]] = vals[i] for i < C, in one kernel.
+hipError_t launch_upload_scatter(void *dst, const void *src, size_t bytes, int32_t *sup, const uint64_t *keys,
+                                 const int32_t *vals, int64_t C, hipStream_t s);
+// Zero the per-level counters (the write pass's scan also does it for the
+// next level).
+hipError_t launch_clear_counters(const LevelArgs &a, hipStream_t s);
 hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t s);
-hipError_t launch_scatter_i32(int32_t *dst, const uint64_t *idx, const int32_t *val, int64_t n, hipStream_t s);
-hipError_t launch_fill_i32(int32_t *dst, int64_t n, int32_t v, hipStream_t s);
 
 }  // namespace gm

@@ -70,9 +70,11 @@ void *HostBuf::ensure(size_t bytes) {
   const size_t want = std::max<size_t>({bytes, cap + cap / 2, 4096});
   if (p) (void)hipHostFree(p);
   p = nullptr;
+  dev = nullptr;
   cap = 0;
-  hip_check(hipHostMalloc(&p, want, hipHostMallocDefault), "hipHostMalloc");
+  hip_check(hipHostMalloc(&p, want, hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
   cap = want;
+  hip_check(hipHostGetDevicePointer(&dev, p, 0), "hipHostGetDevicePointer");
   return p;
 }
 
@@ -584,6 +586,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   uint32_t last_fcount = 0;
   uint32_t last_fill = 0;     // largest dedup table fill of the last level
   int max_nrm = 0;            // longest rightmost path among this level's patterns
+  // the counter block is zero but for the child-row words (the write pass's
+  // scan of the last level cleared it)
+  bool counters_clear = false;
+  const Counters *dctr_last = nullptr;
   // Path pruning: t2[((job * NVL) + centre label) * AR + arm] is the mask of
   // the arms (elabel * NVL + neighbour label) that form a frequent 2-edge
   // path with `arm` around a vertex of that label -- the job's frequent
@@ -637,11 +643,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       check_write(c, pending_rows);
       pending_rows = -1;
     }
-    grow(w.freq, freq_bytes, s);
+    if (grow(w.freq, freq_bytes, s) != (void *)dctr_last) counters_clear = false;  // a new block
     Counters *dctr = w.freq.as<Counters>();
-    // clear all but the child-row words (k_scan of the last level wrote them)
-    hip_check(hipMemsetAsync(dctr, 0, offsetof(Counters, wcursor), s), "memset counters");
-    hip_check(hipMemsetAsync(dctr->spread, 0, sizeof(dctr->spread), s), "memset counters");
+    dctr_last = dctr;
     const int64_t list_rows = root ? db.n_vertices : n_rows;
     a.use_lists = lists_now ? 1 : 0;
     if (a.use_lists) grow(w.rccnt, sizeof(uint16_t) * std::max<int64_t>(list_rows, 1), s);
@@ -678,6 +682,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     a.split_passes = &dctr->splits;
     a.cursor = &dctr->wcursor;
     a.ncand_cursor = &dctr->ncursor;
+    if (!counters_clear) hip_check(launch_clear_counters(a, s), "clear counters");
+    counters_clear = false;  // this level's kernels count into it
     // workgroup size: enough lanes for the average graph's rows (or vertices)
     if (opt.block_threads > 0) {
       a.block = opt.block_threads;
@@ -730,7 +736,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const uint32_t guess = std::min<uint32_t>(fcap, std::max<uint32_t>(4096, 2 * last_fcount));
     const size_t head = sizeof(Counters) + 3 * sizeof(unsigned long long) * guess;
     char *hbuf = static_cast<char *>(w.down.ensure(head));
-    hip_check(hipMemcpyAsync(hbuf, dctr, head, hipMemcpyDeviceToHost, s), "download counters");
+    hip_check(launch_copy8(w.down.dev, dctr, head, s), "download counters");
     hip_check(hipStreamSynchronize(s), "k_support");
     Timer t_host;  // host work between the level's download and its upload
     Counters cnt;
@@ -770,8 +776,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         std::memcpy(frall.data(), src, sizeof(Freq) * have);
         const size_t rest = sizeof(Freq) * (cnt.fcount - have);
         char *h2 = static_cast<char *>(w.down.ensure(rest));
-        hip_check(hipMemcpyAsync(h2, reinterpret_cast<const char *>(a.freq) + sizeof(Freq) * have, rest,
-                                 hipMemcpyDeviceToHost, s), "download frequent keys");
+        hip_check(launch_copy8(w.down.dev, reinterpret_cast<const char *>(a.freq) + sizeof(Freq) * have, rest, s),
+                  "download frequent keys");
         hip_check(hipStreamSynchronize(s), "download frequent keys");
         std::memcpy(frall.data() + have, h2, rest);
         src = frall.data();
@@ -929,11 +935,15 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         t2_ready = true;
       }
       char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
-      hip_check(hipMemcpyAsync(du, hu, masks_off, hipMemcpyHostToDevice, s), "upload children");
+      // one kernel copies the children's metadata in and points the
+      // selected keys at their child ids
+      const char *su = static_cast<const char *>(w.up.dev);
+      hip_check(launch_upload_scatter(du, su, masks_off, w.sup.as<int32_t>(),
+                                      reinterpret_cast<const uint64_t *>(su + keys_off),
+                                      reinterpret_cast<const int32_t *>(su + vals_off), C, s),
+                "upload children");
       us_select = t_sel.sec() * 1e6;
       us_host = t_host.sec() * 1e6;
-      hip_check(launch_scatter_i32(w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(du + keys_off),
-                                   reinterpret_cast<const int32_t *>(du + vals_off), C, s), "scatter");
       grow(w.pid[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.rgraph[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.vt[nxt], (size_t)CS * std::max<int64_t>(child_rows, 1), s);
@@ -972,6 +982,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.nnbig = w.nbig.as<unsigned long long>() + nxt;
       a.nrcoff = w.rcoff[nxt].as<uint32_t>();
       hip_check(launch_write(a, G, root, s), "k_write");
+      counters_clear = true;  // its scan cleared the counters for the next level
       if (masks && t2_ready) {
         // only the next level's k_expand reads the masks: computed while the
         // device writes the children, uploaded behind them
@@ -1004,7 +1015,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
             bw[1 + i] = b;
           }
         }, 4096);
-        hip_check(hipMemcpyAsync(du + masks_off, hu + masks_off, up_bytes - masks_off, hipMemcpyHostToDevice, s),
+        hip_check(launch_copy8(du + masks_off, static_cast<const char *>(w.up.dev) + masks_off, up_bytes - masks_off, s),
                   "upload path masks");
       }
       pending_rows = child_rows;

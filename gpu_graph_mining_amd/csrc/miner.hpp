@@ -29,9 +29,11 @@ struct DevBuf {
   T *as() const { return static_cast<T *>(p); }
 };
 
-// Growable pinned host allocation (staging for stream-ordered copies).
+// Growable pinned host allocation, mapped into the device's address space:
+// the level step's copy kernels read and write it on the compute queue.
 struct HostBuf {
   void *p = nullptr;
+  void *dev = nullptr;  // the same memory as the device addresses it (mapped)
   size_t cap = 0;
   HostBuf() = default;
   HostBuf(const HostBuf &) = delete;
