@@ -29,6 +29,7 @@ namespace gm {
 
 constexpr int kMaxPatV = 32;  // vertices per pattern (tuple stride <= 32 bytes)
 constexpr int kMaxGraphV = 256;
+constexpr uint32_t kPatDead = 1;  // PatMeta::flags
 
 // Per-pattern facts the extension rules need (src/globals/graph_types.cpp:
 // 305-413 evaluated on the pattern side).  Rightmost-path edge i (i = 0 is
@@ -42,7 +43,7 @@ struct alignas(16) PatMeta {
   uint8_t minlabel;  // dense label of vertex 0
   uint8_t maxlabel;  // dense label of the rightmost vertex
   uint32_t bwd_mask; // bit t: the pattern already has edge (rightmost, t)
-  uint32_t pad;
+  uint32_t flags;    // kPatDead: the code is not minimal (its rows are not extended)
   uint32_t rm[kMaxPatV];
 };
 

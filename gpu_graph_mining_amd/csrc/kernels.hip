@@ -397,11 +397,11 @@ __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
   if (t < nrow) {
     const int g = a.rgraph[r];
     const long long gb = a.gcbase[g];
-    if (gb >= 0) {
-      has_list = true;
-      const int32_t p = a.pid[r];
-      const PatMeta *pm = a.meta + p;
-      const uint4 head = *reinterpret_cast<const uint4 *>(pm);
+    const int32_t p = gb >= 0 ? a.pid[r] : 0;
+    const PatMeta *pm = a.meta + p;
+    const uint4 head = gb >= 0 ? *reinterpret_cast<const uint4 *>(pm) : make_uint4(0, 0, 0, 0);
+    has_list = gb >= 0;
+    if (gb >= 0 && !(head.w & kPatDead)) {  // a dead pattern's row lists nothing
       const int nv = (int)(head.y & 0xFF), nrm = (int)((head.y >> 8) & 0xFF);
       const int32_t vb = a.gvbase[g];
       GView G;
@@ -652,10 +652,14 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
           if (p < a.p_lo || p >= a.p_hi) continue;
           if (mod > 1 && ((uint32_t)p & (mod - 1)) != res) continue;
           const PatMeta *pm = a.meta + p;
+          uint2 *out = lists ? lists + a.rcoff[r] : nullptr;
+          if (pm->flags & kPatDead) {  // not extended
+            if (out) a.rccnt[r] = 0;
+            continue;
+          }
           load_slot(slot, a.vt, r, a.S);
           stage_rm(pm, rmv);
           const uint32_t base = (uint32_t)((int64_t)(p - a.p_lo) * a.K);
-          uint2 *out = lists ? lists + a.rcoff[r] : nullptr;
           enumerate_row(G, pm, slot, rmv, a.S, a.NEL, a.NVL, [&](int key, int w, int bound) {
             insert(base + (uint32_t)key);
             ++cand;
@@ -1228,6 +1232,7 @@ __device__ __forceinline__ void children_enum(const LevelArgs &a, int g, int64_t
       enumerate_root(G, (int)r, a.vlab[vb + (int)r], a.NEL, a.NVL, f);
     } else {
       const PatMeta *pm = a.meta + a.pid[r];
+      if (pm->flags & kPatDead) return;  // never extended: no candidate is selected
       stage_rm(pm, rmv);
       enumerate_row(G, pm, slot, rmv, a.S, a.NEL, a.NVL, f);
     }

@@ -264,6 +264,7 @@ struct Level {
   Arr<int32_t> job;
   Arr<uint32_t> sup;
   Arr<int32_t> node;   // query mode: trie node of the code
+  Arr<uint8_t> alive;  // the code is minimal (a dead pattern's rows are not extended)
   void resize(int64_t k, int l) {
     n = k;
     len = l;
@@ -273,6 +274,7 @@ struct Level {
     job.resize(k);
     sup.resize(k);
     node.resize(k);
+    alive.resize(k);
   }
   const DFS *c(int64_t i) const { return code.data() + i * len; }
   DFS *c(int64_t i) { return code.data() + i * len; }
@@ -576,6 +578,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   Arr<DFS> ktup;          // per frequent key: the tuple it adds
   Arr<uint8_t> knv;       // per frequent key: the child's vertices
   Arr<int32_t> knode;     // per frequent key (query mode): trie node
+  Arr<uint8_t> kalive;    // per frequent key: the child's code is minimal
   Arr<uint64_t> nproj;    // per child: embeddings reported
   bool stopped = false;
   int level = 0;          // 0 = root stage
@@ -626,11 +629,15 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const int64_t P = root ? db.n_jobs : L.n;
     const int64_t K = root ? (int64_t)NVL * NEL * NVL : (int64_t)S * NEL * (NVL + 1);
     const int64_t nkeys = P * K;
-    // Packed support / embedding counters: `rep` replicas (summed and
-    // cleared by k_reduce) spread the atomics of small key spaces, where
-    // every graph hits the same keys; the replicas stay within 512 KB.
+    // Packed support / embedding counters: `rep` replicas (graph g counts
+    // into replica g mod rep; k_reduce sums and clears them) spread the
+    // atomics on the keys every graph hits -- one address takes ~12 ns per
+    // atomic.  The replicas take at most 16 bytes of k_reduce traffic per
+    // row of the level (8-256 MB).
+    const int64_t rows_now = root ? db.n_vertices : n_rows;
+    const int64_t rep_budget = std::min<int64_t>(256ll << 20, std::max<int64_t>(8ll << 20, 16 * rows_now));
     int rep = 1;
-    while (rep < 64 && (int64_t)rep * 2 * nkeys * 8 <= (int64_t)(512 << 10)) rep *= 2;
+    while (rep < 64 && (int64_t)rep * 2 * nkeys * 8 <= rep_budget) rep *= 2;
     grow(w.sup, sizeof(int32_t) * nkeys, s);
     grow_zero(w.acc_rep, sizeof(unsigned long long) * nkeys * rep, s);
     const uint32_t fcap = (uint32_t)std::min<int64_t>(nkeys, std::max<int64_t>(1 << 20, P * 64));
@@ -786,7 +793,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     }
     dev_sec += tdev.sec();
 
-    // decode the frequent keys into child tuples; keep the minimal codes
+    // decode the frequent keys into child tuples
     const double us_prep = t_host.sec() * 1e6;
     Timer tcan;
     const int new_len = root ? 1 : L.len + 1;
@@ -794,26 +801,17 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     ktup.resize(F);
     knv.resize(F);
     knode.resize(F);
-    std::atomic<bool> too_big(false);
+    kalive.resize(F);
     Pool::get().run(F, threads, [&](int64_t i) {
-      const uint64_t key = fr[i].key;
-      const int64_t p = key / K, x = key % K;
-      const uint32_t sup = (uint32_t)fr[i].sup;
-      bool keep = false;
-      int32_t node = 0;
+      const int64_t x = (int64_t)(fr[i].key % K);
       DFS t;
       int cnv;
       if (root) {
         const int fl = (int)(x / (NEL * NVL)), el = (int)((x / NVL) % NEL), tl = (int)(x % NVL);
         t = DFS{0, 1, fl, el, tl};
         cnv = 2;
-        if (opt.queries) {
-          node = opt.queries->child(0, t);
-          keep = node >= 0 && query_reached(node, sup);
-        } else {
-          keep = true;
-        }
       } else {
+        const int64_t p = (int64_t)(fr[i].key / K);
         const uint8_t *pv = L.vl(p);
         const int nv = L.nv[p], maxtoc = nv - 1;
         if (x < (int64_t)S * NEL) {
@@ -826,29 +824,57 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
           t = DFS{f, nv, pv[f], el, nl};
           cnv = nv + 1;
         }
-        if (opt.queries) {  // follow the query codes only
-          node = opt.queries->child(L.node[p], t);
-          keep = node >= 0 && query_reached(node, sup) && cnv <= kMaxPatV;
-        } else if (opt.max_edges <= 0 || new_len <= opt.max_edges) {
-          thread_local std::vector<DFS> code;
-          code.assign(L.c(p), L.c(p) + L.len);
-          code.push_back(t);
-          if (dfs_code_is_min(code.data(), new_len)) {
-            if (cnv > kMaxPatV)
-              too_big = true;
-            else
-              keep = true;
-          }
-        }
       }
-      kidx[i] = keep ? 0 : -1;
       ktup[i] = t;
       knv[i] = (uint8_t)std::min(cnv, 255);
-      knode[i] = node;
-    });
-    if (too_big) throw Error(GM_E_UNSUPPORTED, "pattern with more than 32 vertices is frequent");
+      knode[i] = 0;
+    }, 4096);
+    int max_nv_all = 0;
+    for (int64_t i = 0; i < F; ++i) max_nv_all = std::max(max_nv_all, (int)knv[i]);
+    // the canonical check of child i (its parent's code plus its tuple)
+    auto is_min_child = [&](int64_t i) {
+      const int64_t p = (int64_t)(fr[i].key / K);
+      thread_local std::vector<DFS> code;
+      code.assign(L.c(p), L.c(p) + L.len);
+      code.push_back(ktup[i]);
+      return dfs_code_is_min(code.data(), new_len);
+    };
+    // Speculation: at later levels every frequent key becomes a child at
+    // once (ids in key order), and the device writes the children while the
+    // host checks which codes are minimal; the others are marked dead in
+    // their metadata and the next level skips their rows.  Not at the root
+    // (every 1-edge code is minimal), in query mode (the trie selects), past
+    // max_edges, or when a child would have more than kMaxPatV vertices.
+    const bool past_max = opt.max_edges > 0 && new_len > opt.max_edges;
+    const bool spec = !root && !opt.queries && !past_max && max_nv_all <= kMaxPatV;
+    if (!spec) {
+      std::atomic<bool> too_big(false);
+      Pool::get().run(F, threads, [&](int64_t i) {
+        const uint32_t sup = (uint32_t)fr[i].sup;
+        const int cnv = knv[i];
+        bool keep = false;
+        if (opt.queries) {  // follow the query codes only
+          const int32_t parent = root ? 0 : L.node[fr[i].key / K];
+          const int32_t node = opt.queries->child(parent, ktup[i]);
+          knode[i] = node;
+          keep = node >= 0 && query_reached(node, sup) && cnv <= kMaxPatV;
+        } else if (root) {
+          keep = true;
+        } else if (!past_max && is_min_child(i)) {
+          if (cnv > kMaxPatV)
+            too_big = true;
+          else
+            keep = true;
+        }
+        kidx[i] = keep ? 0 : -1;
+        kalive[i] = keep ? 1 : 0;
+      });
+      if (too_big) throw Error(GM_E_UNSUPPORTED, "pattern with more than 32 vertices is frequent");
+    } else {
+      for (int64_t i = 0; i < F; ++i) kidx[i] = 0;
+    }
     canon_sec += tcan.sec();
-    const double us_canon = tcan.sec() * 1e6;
+    double us_canon = tcan.sec() * 1e6;
     Timer t_sel;
     st.n_candidates += root ? 0 : F;
 
@@ -863,14 +889,17 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     }
     N.resize(C, new_len);
     nproj.resize(C);
-    double us_select = 0, us_host = 0;
+    double us_select = 0, us_host = 0, us_overlap = 0;
+    int64_t alive = 0, alive_rows = 0;  // minimal children and their embeddings
     int CS = 4;
     pending_rows = -1;
     bool lists_next = false;
     if (C > 0) {
       // The embedding counts size the child level exactly; the write kernel
-      // reserves each graph's rows from a cursor.  One upload carries the
-      // selected keys and the children's metadata (the next level's).
+      // reserves each graph's rows from a cursor.  The selected keys point
+      // at their child ids first (read from the mapped staging buffer) and
+      // the write pass starts; the children's metadata (the next level's)
+      // follows in one upload.
       Timer tdev2;
       CS = (int)round4(max_nv);
       const int nxt = cur ^ 1;
@@ -887,63 +916,17 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       PatMeta *hm = reinterpret_cast<PatMeta *>(hu);
       unsigned long long *hk = reinterpret_cast<unsigned long long *>(hu + keys_off);
       int32_t *hv = reinterpret_cast<int32_t *>(hu + vals_off);
-      Pool::get().run(F, threads, [&](int64_t i) {
+      for (int64_t i = 0; i < F; ++i) {
         const int64_t c = kidx[i];
-        if (c < 0) return;
-        const uint64_t key = fr[i].key;
-        const int64_t p = key / K;
-        DFS *cc = N.c(c);
-        uint8_t *cv = N.vl(c);
-        const DFS &t = ktup[i];
-        const int cnv = knv[i];
-        if (root) {
-          cc[0] = t;
-          cv[0] = (uint8_t)t.fromlabel;
-          cv[1] = (uint8_t)t.tolabel;
-          N.job[c] = (int32_t)p;
-        } else {
-          std::memcpy(cc, L.c(p), sizeof(DFS) * L.len);
-          cc[L.len] = t;
-          std::memcpy(cv, L.vl(p), L.nv[p]);
-          if (cnv > L.nv[p]) cv[cnv - 1] = (uint8_t)t.tolabel;
-          N.job[c] = L.job[p];
-        }
-        N.nv[c] = (uint8_t)cnv;
-        N.sup[c] = (uint32_t)fr[i].sup;
-        N.node[c] = knode[i];
-        nproj[c] = opt.collect_proj ? fr[i].proj : 0;
-        hm[c] = make_meta(cc, new_len, cv, cnv, N.job[c]);
-        hk[c] = key;
+        if (c < 0) continue;
+        hk[c] = fr[i].key;
         hv[c] = -(int32_t)(c + 1);
-      }, 4096);
-      int next_nrm = 0;
-      for (int64_t c = 0; c < C; ++c) next_nrm = std::max(next_nrm, (int)hm[c].nrm);
-      if (masks && new_len == 2) {
-        // the job's frequent 2-edge patterns are these children: (0,1,a,e1,b)
-        // then (1,2,b,e2,c) (centre b) or (0,2,a,e2,c) (centre a)
-        t2.assign((size_t)db.n_jobs * NVL * AR, 0ull);
-        for (int64_t c = 0; c < C; ++c) {
-          const DFS *cc = N.c(c);
-          const bool at1 = cc[1].from == 1;
-          const int centre = at1 ? cc[0].tolabel : cc[0].fromlabel;
-          const int arm1 = cc[0].elabel * NVL + (at1 ? cc[0].fromlabel : cc[0].tolabel);
-          const int arm2 = cc[1].elabel * NVL + cc[1].tolabel;
-          uint64_t *row = &t2[((size_t)N.job[c] * NVL + centre) * AR];
-          row[arm1] |= 1ull << arm2;
-          row[arm2] |= 1ull << arm1;
-        }
-        t2_ready = true;
       }
       char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
-      // one kernel copies the children's metadata in and points the
-      // selected keys at their child ids
       const char *su = static_cast<const char *>(w.up.dev);
-      hip_check(launch_upload_scatter(du, su, masks_off, w.sup.as<int32_t>(),
-                                      reinterpret_cast<const uint64_t *>(su + keys_off),
+      hip_check(launch_upload_scatter(du, su, 0, w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(su + keys_off),
                                       reinterpret_cast<const int32_t *>(su + vals_off), C, s),
-                "upload children");
-      us_select = t_sel.sec() * 1e6;
-      us_host = t_host.sec() * 1e6;
+                "scatter child ids");
       grow(w.pid[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.rgraph[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.vt[nxt], (size_t)CS * std::max<int64_t>(child_rows, 1), s);
@@ -983,11 +966,75 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.nrcoff = w.rcoff[nxt].as<uint32_t>();
       hip_check(launch_write(a, G, root, s), "k_write");
       counters_clear = true;  // its scan cleared the counters for the next level
+      us_select = t_sel.sec() * 1e6;
+      us_host = t_host.sec() * 1e6;
+
+      // while the device writes the children: the canonical check of the
+      // speculative children, their codes and metadata, the path masks
+      Timer t_over;
+      if (spec) {
+        Timer tc2;
+        Pool::get().run(F, threads, [&](int64_t i) { kalive[i] = is_min_child(i) ? 1 : 0; });
+        canon_sec += tc2.sec();
+        us_canon += tc2.sec() * 1e6;
+      }
+      Pool::get().run(F, threads, [&](int64_t i) {
+        const int64_t c = kidx[i];
+        if (c < 0) return;
+        const uint64_t key = fr[i].key;
+        const int64_t p = key / K;
+        DFS *cc = N.c(c);
+        uint8_t *cv = N.vl(c);
+        const DFS &t = ktup[i];
+        const int cnv = knv[i];
+        if (root) {
+          cc[0] = t;
+          cv[0] = (uint8_t)t.fromlabel;
+          cv[1] = (uint8_t)t.tolabel;
+          N.job[c] = (int32_t)p;
+        } else {
+          std::memcpy(cc, L.c(p), sizeof(DFS) * L.len);
+          cc[L.len] = t;
+          std::memcpy(cv, L.vl(p), L.nv[p]);
+          if (cnv > L.nv[p]) cv[cnv - 1] = (uint8_t)t.tolabel;
+          N.job[c] = L.job[p];
+        }
+        N.nv[c] = (uint8_t)cnv;
+        N.sup[c] = (uint32_t)fr[i].sup;
+        N.node[c] = knode[i];
+        N.alive[c] = kalive[i];
+        nproj[c] = fr[i].proj;
+        hm[c] = make_meta(cc, new_len, cv, cnv, N.job[c]);
+        if (!kalive[i]) hm[c].flags |= kPatDead;
+      }, 4096);
+      int next_nrm = 0;  // over every child: LDS slots hold any row's rightmost path
+      for (int64_t c = 0; c < C; ++c) {
+        next_nrm = std::max(next_nrm, (int)hm[c].nrm);
+        if (!N.alive[c]) continue;
+        ++alive;
+        alive_rows += (int64_t)nproj[c];
+      }
+      if (masks && new_len == 2) {
+        // the job's frequent 2-edge patterns are the minimal children: (0,1,a,e1,b)
+        // then (1,2,b,e2,c) (centre b) or (0,2,a,e2,c) (centre a)
+        t2.assign((size_t)db.n_jobs * NVL * AR, 0ull);
+        for (int64_t c = 0; c < C; ++c) {
+          if (!N.alive[c]) continue;
+          const DFS *cc = N.c(c);
+          const bool at1 = cc[1].from == 1;
+          const int centre = at1 ? cc[0].tolabel : cc[0].fromlabel;
+          const int arm1 = cc[0].elabel * NVL + (at1 ? cc[0].fromlabel : cc[0].tolabel);
+          const int arm2 = cc[1].elabel * NVL + cc[1].tolabel;
+          uint64_t *row = &t2[((size_t)N.job[c] * NVL + centre) * AR];
+          row[arm1] |= 1ull << arm2;
+          row[arm2] |= 1ull << arm1;
+        }
+        t2_ready = true;
+      }
       if (masks && t2_ready) {
-        // only the next level's k_expand reads the masks: computed while the
-        // device writes the children, uploaded behind them
         uint64_t *hmask = reinterpret_cast<uint64_t *>(hu + masks_off);
         Pool::get().run(C, threads, [&](int64_t c) {
+          if (!N.alive[c]) return;  // dead patterns' rows are never extended
           uint64_t *fw = hmask + (size_t)c * 2 * mstride, *bw = fw + mstride;
           const DFS *cc = N.c(c);
           const uint8_t *cv = N.vl(c);
@@ -1015,9 +1062,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
             bw[1 + i] = b;
           }
         }, 4096);
-        hip_check(launch_copy8(du + masks_off, static_cast<const char *>(w.up.dev) + masks_off, up_bytes - masks_off, s),
-                  "upload path masks");
       }
+      hip_check(launch_copy8(du, su, up_bytes, s), "upload children");
+      us_overlap = t_over.sec() * 1e6;
       pending_rows = child_rows;
       dev_sec += tdev2.sec();
       size_t live = db.bytes + w.sup.cap + w.acc_rep.cap + w.freq.cap +
@@ -1032,21 +1079,22 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     }
     lists_now = lists_next;
     if (root)
-      st.e_root += child_rows;
+      st.e_root += alive_rows;
     else
-      st.e_child += child_rows;
+      st.e_child += alive_rows;
     if (!opt.queries) {
       if (!report) {
-        st.n_patterns += C;
+        st.n_patterns += alive;
       } else {
         orig.resize(new_len);
         for (int64_t c = 0; c < C && !stopped; ++c) {
+          if (!N.alive[c]) continue;
           st.n_patterns++;
           const DFS *cc = N.c(c);
           for (int k = 0; k < new_len; ++k)
             orig[k] = DFS{cc[k].from, cc[k].to, db.vl_values[cc[k].fromlabel], db.el_values[cc[k].elabel],
                           db.vl_values[cc[k].tolabel]};
-          if (!report(N.job[c], orig.data(), new_len, N.sup[c], nproj[c])) stopped = true;
+          if (!report(N.job[c], orig.data(), new_len, N.sup[c], opt.collect_proj ? nproj[c] : 0)) stopped = true;
         }
       }
     }
@@ -1054,10 +1102,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       fprintf(stderr,
               "[gm level %d] patterns %lld rows %lld K %lld rep %d block %d table_bits %d support_us %.1f "
               "candidates %llu pruned %llu frequent %u children %lld child_rows %lld splits %llu max_fill %llu lists %d "
-              "host_us prep %.1f canon %.1f select %.1f total %.1f\n",
+              "host_us prep %.1f canon %.1f select %.1f total %.1f overlap %.1f\n",
               level, (long long)P, (long long)(root ? db.n_vertices : n_rows), (long long)K, rep, a.block,
               a.table_bits, ms * 1e3, ecand, pruned, cnt.fcount, (long long)C, (long long)child_rows, cnt.splits, max_fill,
-              a.use_lists, us_prep, us_canon, us_select, us_host);
+              a.use_lists, us_prep, us_canon, us_select, us_host, us_overlap);
     st.n_levels++;
     if (stopped) break;
     std::swap(L, N);
