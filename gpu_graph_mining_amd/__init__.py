@@ -53,7 +53,7 @@ class gm_cfg(C.Structure):
     _fields_ = [
         ("label_style", C.c_int32), ("max_edges", C.c_int32), ("canon_threads", C.c_int32),
         ("block_threads", C.c_int32), ("mem_budget_bytes", C.c_int64), ("collect_proj", C.c_int32),
-        ("candidate_lists", C.c_int32), ("path_pruning", C.c_int32), ("reserved", C.c_int32 * 5),
+        ("candidate_lists", C.c_int32), ("path_pruning", C.c_int32), ("speculate", C.c_int32), ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -417,7 +417,7 @@ class graph_collecting_output(graph_output):
 
 # ------------------------------------------------------------------- mining
 def _make_cfg(label_style=GM_LABELS_FULL, max_edges=0, canon_threads=0, collect_proj=False, candidate_lists=True,
-              block_threads=0, path_pruning=True) -> gm_cfg:
+              block_threads=0, path_pruning=True, speculate=0) -> gm_cfg:
     cfg = gm_cfg()
     load_library().gm_cfg_init(C.byref(cfg))
     cfg.label_style = label_style
@@ -427,13 +427,15 @@ def _make_cfg(label_style=GM_LABELS_FULL, max_edges=0, canon_threads=0, collect_
     cfg.candidate_lists = 0 if candidate_lists else 1
     cfg.block_threads = block_threads
     cfg.path_pruning = 0 if path_pruning else 1
+    cfg.speculate = speculate
     return cfg
 
 
 def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edges: int = 0,
          collect_proj: bool = False, canon_threads: int = 0,
          on_pattern: Optional[Callable[[DFSCode, int, int], Optional[bool]]] = None, count_only: bool = False,
-         candidate_lists: bool = True, block_threads: int = 0, path_pruning: bool = True):
+         candidate_lists: bool = True, block_threads: int = 0, path_pruning: bool = True,
+         speculate: int = 0):
     """Mine db on the GPU.  Returns (patterns, stats) where patterns is a
     list of (DFSCode, support, |Proj|) in emission order (empty when
     on_pattern is given; it then receives each pattern, and returning False
@@ -442,7 +444,9 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
     enumerate extensions again instead of reading the support pass's lists;
     block_threads fixes the workgroup size (64, 128 or 256; 0 = automatic);
     path_pruning=False keeps the candidates whose new edge forms an
-    infrequent 2-edge path (the results are the same either way)."""
+    infrequent 2-edge path; speculate (0 adaptive, 1 never, 2 always)
+    controls whether a level's children are written before their canonical
+    check (the results are the same either way)."""
     lib = load_library()
     out: list = []
 
@@ -455,7 +459,7 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
 
     ccb = PATTERN_CB() if count_only else PATTERN_CB(cb)  # PATTERN_CB() is a NULL pointer
     cfg = _make_cfg(label_style, max_edges, canon_threads, collect_proj, candidate_lists, block_threads,
-                    path_pruning)
+                    path_pruning, speculate)
     st = gm_stats()
     _check(lib.gm_mine(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st)))
     return out, st.as_dict()
@@ -514,7 +518,7 @@ class Batch:
         self.n_jobs = len(dbs)
 
     def mine(self, minsups: Sequence[int], *, label_style: int = GM_LABELS_FULL, collect_proj: bool = False,
-             canon_threads: int = 0, count_only: bool = False, path_pruning: bool = True):
+             canon_threads: int = 0, count_only: bool = False, path_pruning: bool = True, speculate: int = 0):
         lib = load_library()
         res: List[list] = [[] for _ in range(self.n_jobs)]
 
@@ -525,7 +529,7 @@ class Batch:
 
         ccb = JOB_PATTERN_CB() if count_only else JOB_PATTERN_CB(cb)
         ms = _i32(minsups)
-        cfg = _make_cfg(label_style, 0, canon_threads, collect_proj, path_pruning=path_pruning)
+        cfg = _make_cfg(label_style, 0, canon_threads, collect_proj, path_pruning=path_pruning, speculate=speculate)
         st = gm_stats()
         _check(lib.gm_batch_mine(self._h, _ptr(ms), C.byref(cfg), ccb, None, C.byref(st)))
         return res, st.as_dict()

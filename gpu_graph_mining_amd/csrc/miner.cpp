@@ -608,6 +608,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   bool t2_ready = false;
   size_t mask_off = 0;  // this level's masks in its pattern upload (stride 0: none)
   int mask_stride = 0;
+  double dead_share = 0;  // the last level's rows of non-minimal children / its child rows
   bool lists_now = opt.candidate_lists;  // this level's rows have candidate lists
   // GM_LEVEL_TRACE=1: one line per level step on stderr (sizes, timings)
   const char *trace_env = getenv("GM_LEVEL_TRACE");
@@ -846,7 +847,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // (every 1-edge code is minimal), in query mode (the trie selects), past
     // max_edges, or when a child would have more than kMaxPatV vertices.
     const bool past_max = opt.max_edges > 0 && new_len > opt.max_edges;
-    const bool spec = !root && !opt.queries && !past_max && max_nv_all <= kMaxPatV;
+    // adaptive: speculate while the last level's non-minimal children held
+    // at most 30 % of its rows (each costs a written, unused row; molecule-
+    // like databases, rich in symmetric patterns, hold about half)
+    const bool want_spec = opt.speculate == 2 || (opt.speculate == 0 && dead_share <= 0.3);
+    const bool spec = want_spec && !root && !opt.queries && !past_max && max_nv_all <= kMaxPatV;
     if (!spec) {
       std::atomic<bool> too_big(false);
       Pool::get().run(F, threads, [&](int64_t i) {
@@ -1008,12 +1013,23 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         if (!kalive[i]) hm[c].flags |= kPatDead;
       }, 4096);
       int next_nrm = 0;  // over every child: LDS slots hold any row's rightmost path
+      int64_t dead_rows = 0;
       for (int64_t c = 0; c < C; ++c) {
         next_nrm = std::max(next_nrm, (int)hm[c].nrm);
-        if (!N.alive[c]) continue;
+        if (!N.alive[c]) {
+          dead_rows += (int64_t)nproj[c];
+          continue;
+        }
         ++alive;
         alive_rows += (int64_t)nproj[c];
       }
+      // non-minimal children's share of the rows: measured when the check
+      // ran before the write too (the children the write would have made)
+      int64_t nonmin_rows = dead_rows;
+      if (!spec && !root && !opt.queries)
+        for (int64_t i = 0; i < F; ++i)
+          if (kidx[i] < 0) nonmin_rows += (int64_t)fr[i].proj;
+      dead_share = (double)nonmin_rows / (double)std::max<int64_t>(1, alive_rows + nonmin_rows);
       if (masks && new_len == 2) {
         // the job's frequent 2-edge patterns are the minimal children: (0,1,a,e1,b)
         // then (1,2,b,e2,c) (centre b) or (0,2,a,e2,c) (centre a)
@@ -1101,10 +1117,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     if (trace_levels)
       fprintf(stderr,
               "[gm level %d] patterns %lld rows %lld K %lld rep %d block %d table_bits %d support_us %.1f "
-              "candidates %llu pruned %llu frequent %u children %lld child_rows %lld splits %llu max_fill %llu lists %d "
+              "candidates %llu pruned %llu frequent %u children %lld child_rows %lld minimal %lld spec %d splits %llu "
+              "max_fill %llu lists %d "
               "host_us prep %.1f canon %.1f select %.1f total %.1f overlap %.1f\n",
               level, (long long)P, (long long)(root ? db.n_vertices : n_rows), (long long)K, rep, a.block,
-              a.table_bits, ms * 1e3, ecand, pruned, cnt.fcount, (long long)C, (long long)child_rows, cnt.splits, max_fill,
+              a.table_bits, ms * 1e3, ecand, pruned, cnt.fcount, (long long)C, (long long)child_rows, (long long)alive,
+              spec ? 1 : 0, cnt.splits, max_fill,
               a.use_lists, us_prep, us_canon, us_select, us_host, us_overlap);
     st.n_levels++;
     if (stopped) break;

@@ -117,6 +117,7 @@ struct MineOptions {
   bool collect_proj = false;
   bool candidate_lists = true;  // k_support stores candidates for k_write
   bool path_pruning = true;     // k_expand drops candidates with an infrequent 2-edge sub-path
+  int32_t speculate = 0;        // 0 adaptive, 1 never, 2 always (gm_cfg::speculate)
   // Query mode: instead of mining, follow the codes of `queries` (every
   // prefix of a minimum code is one) and store each query's support in
   // query_support; the report callback is not called.

@@ -61,7 +61,10 @@ typedef struct gm_cfg {
                                the write pass (default); 1 = the write pass enumerates again */
   int32_t path_pruning;     /* 0 = drop extension candidates that have an infrequent 2-edge sub-path
                                (exact: such a child is infrequent; default); 1 = keep every candidate */
-  int32_t reserved[5];
+  int32_t speculate;        /* children written before their canonical check (the check overlaps the
+                               write; non-minimal ones are dropped at the next level): 0 = when the
+                               last level's non-minimal share was small (default), 1 = never, 2 = always */
+  int32_t reserved[4];
 } gm_cfg;
 
 /* Per-run statistics (filled when a non-NULL pointer is passed). */

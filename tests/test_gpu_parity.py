@@ -182,6 +182,22 @@ def test_gpu_path_pruning_same_results(golden, name):
     assert 0 <= st_on["pruned"] <= st_on["e_cand"]
 
 
+@pytest.mark.parametrize("name", ["config1_1k", "chem_34", "config4_2k"])
+def test_gpu_speculation_modes_agree(golden, name):
+    """Children written before their canonical check (non-minimal ones
+    dropped at the next level), written after it, or chosen per level: the
+    same patterns, supports, |Proj|, E_cand and E_child."""
+    case = golden[name]
+    db = case_db(case)
+    runs = [gm.mine(db, case["minsup"], collect_proj=True, speculate=m) for m in (0, 1, 2)]
+    want = _by_code(runs[0][0])
+    check_listing(listing(runs[0][0]), case)
+    for pats, st in runs[1:]:
+        assert _by_code(pats) == want
+        assert (st["e_child"], st["e_cand"], st["e_root"], st["n_patterns"]) == (
+            runs[0][1]["e_child"], runs[0][1]["e_cand"], runs[0][1]["e_root"], runs[0][1]["n_patterns"])
+
+
 def test_gpu_bad_block_threads_rejected(golden):
     db = case_db(golden["chem_17"])
     with pytest.raises(gm.GMError) as e:
