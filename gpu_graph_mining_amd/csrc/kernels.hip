@@ -547,6 +547,8 @@ __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
   }
 }
 
+constexpr int kReplayUnroll = 4;  // list entries per lane per step of the list loops
+
 template <bool ROOT, bool LDS>
 __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t r0, int64_t r1, const Lds &l,
                                              SupportShared &sh) {
@@ -697,16 +699,32 @@ __device__ __forceinline__ void support_body(const LevelArgs &a, int g, int64_t 
         sh.pre[threadIdx.x] = block_exclusive_scan<int>(n, T, sh.ired);
         __syncthreads();
         const int nrow = (int)min<int64_t>(B, r1 - base);
-        for (int j = threadIdx.x; j < T; j += B) {
-          const int k = find_row(sh.pre, j, nrow);
-          const uint32_t *rec = l.slots + k * a.slot_words;
-          const uint2 c = lists[rec[0] + (j - sh.pre[k])];
-          int key, w;
-          cand_decode(ROOT, c.x, key, w);
-          insert(rec[1] + (uint32_t)key);
-          if (!listed) {
-            ++cand;
-            bound_sum += c.y;
+        // kReplayUnroll entries per lane per step, their list loads issued
+        // together (each entry's row is found by binary search first)
+        for (int j0 = threadIdx.x; j0 < T; j0 += kReplayUnroll * B) {
+          uint2 c[kReplayUnroll];
+          uint32_t kb[kReplayUnroll];
+#pragma unroll
+          for (int u = 0; u < kReplayUnroll; ++u) {
+            const int j = j0 + u * B;
+            if (j < T) {
+              const int k = find_row(sh.pre, j, nrow);
+              const uint32_t *rec = l.slots + k * a.slot_words;
+              c[u] = lists[rec[0] + (j - sh.pre[k])];
+              kb[u] = rec[1];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kReplayUnroll; ++u) {
+            if (j0 + u * B < T) {
+              int key, w;
+              cand_decode(ROOT, c[u].x, key, w);
+              insert(kb[u] + (uint32_t)key);
+              if (!listed) {
+                ++cand;
+                bound_sum += c[u].y;
+              }
+            }
           }
         }
         __syncthreads();  // the next chunk reuses the slots
@@ -1098,6 +1116,8 @@ struct WriteShared {
   int ired[5];
   int pre[kMaxBlock];        // list mode: prefix of the chunk's list lengths
   long long off[kMaxBlock];  // list mode: list offset of each chunk row
+  int pid[kMaxBlock];        // list mode: pattern of each chunk row
+  int nvp[kMaxBlock];        // list mode (write): its vertex count
 };
 
 // Write one child row at `pos` (parent tuple in slot; root: vertex u).
@@ -1134,10 +1154,13 @@ __device__ __forceinline__ void children_lists(const LevelArgs &a, int g, int64_
     const int64_t r = base + threadIdx.x;
     const bool live = r < r1;
     const int n = live ? (int)a.rccnt[ROOT ? vb + r : r] : 0;
+    const int p = !ROOT && live ? a.pid[r] : 0;
     int T;
     const int pre = block_exclusive_scan<int>(n, T, sh.ired);
     sh.pre[threadIdx.x] = pre;
     sh.off[threadIdx.x] = live ? (ROOT ? (long long)a.vadj[vb + r] : (long long)a.rcoff[r]) : 0;
+    sh.pid[threadIdx.x] = p;
+    if (WRITE && !ROOT) sh.nvp[threadIdx.x] = live && n ? (int)a.meta[p].nv : 0;
     __syncthreads();
     return T;
   };
@@ -1155,15 +1178,35 @@ __device__ __forceinline__ void children_lists(const LevelArgs &a, int g, int64_
     for (int64_t base = r0; base < r1; base += B) {
       const int T = chunk(base);
       const int nrow = (int)min<int64_t>(B, r1 - base);
-      for (int j = threadIdx.x; j < T; j += B) {
-        const int k = find(j, nrow);
-        const uint2 c = lists[sh.off[k] + (j - sh.pre[k])];
-        int key, w;
-        cand_decode(ROOT, c.x, key, w);
-        const uint64_t kbase = ROOT ? (uint64_t)job * a.K : (uint64_t)a.pid[base + k] * (uint64_t)a.K;
-        if (a.sup[kbase + key] < 0) {
-          ++cnt;
-          bsum += c.y;
+      for (int j0 = threadIdx.x; j0 < T; j0 += kReplayUnroll * B) {
+        uint2 c[kReplayUnroll];
+        int kk[kReplayUnroll];
+#pragma unroll
+        for (int u = 0; u < kReplayUnroll; ++u) {
+          const int j = j0 + u * B;
+          if (j < T) {
+            const int k = find(j, nrow);
+            kk[u] = k;
+            c[u] = lists[sh.off[k] + (j - sh.pre[k])];
+          }
+        }
+        int32_t v[kReplayUnroll];
+#pragma unroll
+        for (int u = 0; u < kReplayUnroll; ++u) {
+          v[u] = 0;
+          if (j0 + u * B < T) {
+            int key, w;
+            cand_decode(ROOT, c[u].x, key, w);
+            const uint64_t kbase = ROOT ? (uint64_t)job * a.K : (uint64_t)sh.pid[kk[u]] * (uint64_t)a.K;
+            v[u] = a.sup[kbase + key];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kReplayUnroll; ++u) {
+          if (v[u] < 0) {
+            ++cnt;
+            bsum += c[u].y;
+          }
         }
       }
       __syncthreads();
@@ -1192,7 +1235,7 @@ __device__ __forceinline__ void children_lists(const LevelArgs &a, int g, int64_
         const uint2 c = lists[sh.off[k] + (j - sh.pre[k])];
         cand_decode(ROOT, c.x, key, w);
         bound = c.y;
-        const uint64_t kbase = ROOT ? (uint64_t)job * a.K : (uint64_t)a.pid[base + k] * (uint64_t)a.K;
+        const uint64_t kbase = ROOT ? (uint64_t)job * a.K : (uint64_t)sh.pid[k] * (uint64_t)a.K;
         v = a.sup[kbase + key];
       }
       const long long x = v < 0 ? ((1ll << kPackShift) | (long long)bound) : 0;
@@ -1202,7 +1245,7 @@ __device__ __forceinline__ void children_lists(const LevelArgs &a, int g, int64_
         const int64_t r = base + k;
         int nvp = 0;
         if (!ROOT) {
-          nvp = a.meta[a.pid[r]].nv;
+          nvp = sh.nvp[k];
           load_slot(slot, a.vt, r, a.S);
         }
         put_child<ROOT>(a, g, out + (pre >> kPackShift), -v - 1, (int)r, w, nvp, slot, nb >= 0,

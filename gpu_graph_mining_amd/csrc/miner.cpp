@@ -977,13 +977,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       // while the device writes the children: the canonical check of the
       // speculative children, their codes and metadata, the path masks
       Timer t_over;
-      if (spec) {
-        Timer tc2;
-        Pool::get().run(F, threads, [&](int64_t i) { kalive[i] = is_min_child(i) ? 1 : 0; });
-        canon_sec += tc2.sec();
-        us_canon += tc2.sec() * 1e6;
-      }
-      Pool::get().run(F, threads, [&](int64_t i) {
+      uint64_t *hmask = reinterpret_cast<uint64_t *>(hu + masks_off);
+      auto build_child = [&](int64_t i) {
         const int64_t c = kidx[i];
         if (c < 0) return;
         const uint64_t key = fr[i].key;
@@ -1011,7 +1006,72 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         nproj[c] = fr[i].proj;
         hm[c] = make_meta(cc, new_len, cv, cnv, N.job[c]);
         if (!kalive[i]) hm[c].flags |= kPatDead;
-      }, 4096);
+      };
+      auto mask_child = [&](int64_t c) {
+        if (!N.alive[c]) return;  // dead patterns' rows are never extended
+        uint64_t *fw = hmask + (size_t)c * 2 * mstride, *bw = fw + mstride;
+        const DFS *cc = N.c(c);
+        const uint8_t *cv = N.vl(c);
+        const PatMeta &m = hm[c];
+        const uint64_t *tj = &t2[(size_t)N.job[c] * NVL * AR];
+        // per vertex: the arms that form a frequent path with every
+        // pattern edge at it (one pass over the edges)
+        uint64_t vm[kMaxPatV];
+        for (int v = 0; v < m.nv; ++v) vm[v] = ~0ull;
+        for (int k = 0; k < new_len; ++k) {
+          const int u = cc[k].from, v = cc[k].to, e = cc[k].elabel * NVL;
+          vm[u] &= tj[(size_t)cv[u] * AR + e + cv[v]];
+          vm[v] &= tj[(size_t)cv[v] * AR + e + cv[u]];
+        }
+        const int rv = m.nv - 1;
+        fw[0] = vm[rv];
+        bw[0] = 0;
+        for (int i = 0; i < m.nrm; ++i) {
+          const int t = m.rm[i] & 0xFF;
+          fw[1 + i] = vm[t];
+          uint64_t b = 0;  // backward rv -> t: a new arm at both ends
+          if (i > 0)
+            for (int el = 0; el < NEL; ++el)
+              b |= ((vm[rv] >> (el * NVL + cv[t])) & (vm[t] >> (el * NVL + cv[rv])) & 1ull) << el;
+          bw[1 + i] = b;
+        }
+      };
+      Timer tc2;
+      if (!masks || (t2_ready && new_len > 2)) {
+        // one pass per child: check, metadata, masks (the path table is known)
+        Pool::get().run(F, threads, [&](int64_t i) {
+          if (kidx[i] < 0) return;
+          if (spec) kalive[i] = is_min_child(i) ? 1 : 0;
+          build_child(i);
+          if (masks) mask_child(kidx[i]);
+        });
+      } else {
+        // the 2-edge level: its minimal children are the path table
+        if (spec) Pool::get().run(F, threads, [&](int64_t i) { kalive[i] = is_min_child(i) ? 1 : 0; });
+        Pool::get().run(F, threads, build_child, 4096);
+        if (new_len == 2) {
+          // the job's frequent 2-edge patterns: (0,1,a,e1,b) then
+          // (1,2,b,e2,c) (centre b) or (0,2,a,e2,c) (centre a)
+          t2.assign((size_t)db.n_jobs * NVL * AR, 0ull);
+          for (int64_t c = 0; c < C; ++c) {
+            if (!N.alive[c]) continue;
+            const DFS *cc = N.c(c);
+            const bool at1 = cc[1].from == 1;
+            const int centre = at1 ? cc[0].tolabel : cc[0].fromlabel;
+            const int arm1 = cc[0].elabel * NVL + (at1 ? cc[0].fromlabel : cc[0].tolabel);
+            const int arm2 = cc[1].elabel * NVL + cc[1].tolabel;
+            uint64_t *row = &t2[((size_t)N.job[c] * NVL + centre) * AR];
+            row[arm1] |= 1ull << arm2;
+            row[arm2] |= 1ull << arm1;
+          }
+          t2_ready = true;
+        }
+        if (t2_ready) Pool::get().run(C, threads, mask_child, 4096);
+      }
+      if (spec) {  // the check with the metadata it overlaps
+        canon_sec += tc2.sec();
+        us_canon += tc2.sec() * 1e6;
+      }
       int next_nrm = 0;  // over every child: LDS slots hold any row's rightmost path
       int64_t dead_rows = 0;
       for (int64_t c = 0; c < C; ++c) {
@@ -1030,55 +1090,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         for (int64_t i = 0; i < F; ++i)
           if (kidx[i] < 0) nonmin_rows += (int64_t)fr[i].proj;
       dead_share = (double)nonmin_rows / (double)std::max<int64_t>(1, alive_rows + nonmin_rows);
-      if (masks && new_len == 2) {
-        // the job's frequent 2-edge patterns are the minimal children: (0,1,a,e1,b)
-        // then (1,2,b,e2,c) (centre b) or (0,2,a,e2,c) (centre a)
-        t2.assign((size_t)db.n_jobs * NVL * AR, 0ull);
-        for (int64_t c = 0; c < C; ++c) {
-          if (!N.alive[c]) continue;
-          const DFS *cc = N.c(c);
-          const bool at1 = cc[1].from == 1;
-          const int centre = at1 ? cc[0].tolabel : cc[0].fromlabel;
-          const int arm1 = cc[0].elabel * NVL + (at1 ? cc[0].fromlabel : cc[0].tolabel);
-          const int arm2 = cc[1].elabel * NVL + cc[1].tolabel;
-          uint64_t *row = &t2[((size_t)N.job[c] * NVL + centre) * AR];
-          row[arm1] |= 1ull << arm2;
-          row[arm2] |= 1ull << arm1;
-        }
-        t2_ready = true;
-      }
-      if (masks && t2_ready) {
-        uint64_t *hmask = reinterpret_cast<uint64_t *>(hu + masks_off);
-        Pool::get().run(C, threads, [&](int64_t c) {
-          if (!N.alive[c]) return;  // dead patterns' rows are never extended
-          uint64_t *fw = hmask + (size_t)c * 2 * mstride, *bw = fw + mstride;
-          const DFS *cc = N.c(c);
-          const uint8_t *cv = N.vl(c);
-          const PatMeta &m = hm[c];
-          const uint64_t *tj = &t2[(size_t)N.job[c] * NVL * AR];
-          // per vertex: the arms that form a frequent path with every
-          // pattern edge at it (one pass over the edges)
-          uint64_t vm[kMaxPatV];
-          for (int v = 0; v < m.nv; ++v) vm[v] = ~0ull;
-          for (int k = 0; k < new_len; ++k) {
-            const int u = cc[k].from, v = cc[k].to, e = cc[k].elabel * NVL;
-            vm[u] &= tj[(size_t)cv[u] * AR + e + cv[v]];
-            vm[v] &= tj[(size_t)cv[v] * AR + e + cv[u]];
-          }
-          const int rv = m.nv - 1;
-          fw[0] = vm[rv];
-          bw[0] = 0;
-          for (int i = 0; i < m.nrm; ++i) {
-            const int t = m.rm[i] & 0xFF;
-            fw[1 + i] = vm[t];
-            uint64_t b = 0;  // backward rv -> t: a new arm at both ends
-            if (i > 0)
-              for (int el = 0; el < NEL; ++el)
-                b |= ((vm[rv] >> (el * NVL + cv[t])) & (vm[t] >> (el * NVL + cv[rv])) & 1ull) << el;
-            bw[1 + i] = b;
-          }
-        }, 4096);
-      }
       hip_check(launch_copy8(du, su, up_bytes, s), "upload children");
       us_overlap = t_over.sec() * 1e6;
       pending_rows = child_rows;
