@@ -258,6 +258,7 @@ struct Arr {
 struct Level {
   int len = 0;         // tuples per code
   int64_t n = 0;       // patterns
+  int max_nv = 0;      // most vertices of a pattern
   Arr<DFS> code;       // n * len tuples, dense labels, fully labelled
   Arr<uint8_t> vlab;   // kMaxPatV dense vertex labels per pattern
   Arr<uint8_t> nv;     // vertices per pattern
@@ -311,35 +312,35 @@ struct Freq {
 
 // The frequent keys in ascending key order (the order of the level's
 // children, so that pattern ids do not depend on the device's append
-// order).  Keys are unique and below key_end; LSD radix sort over 11-bit
-// digits of the bits in use.
+// order).  Keys are unique and below key_end; LSD radix sort over the bits
+// in use, 8-bit digits for short lists, 11-bit digits for long ones.
 void sort_freq(const Freq *in, int64_t n, uint64_t key_end, Arr<Freq> &out, Arr<Freq> &tmp) {
   out.resize(n);
-  if (n < 4096) {
-    std::copy(in, in + n, out.data());
-    std::sort(out.data(), out.data() + n, [](const Freq &x, const Freq &y) { return x.key < y.key; });
+  if (n < 2) {
+    if (n) out[0] = in[0];
     return;
   }
   int bits = 1;
   while (bits < 64 && (key_end >> bits)) ++bits;
-  const int passes = (bits + 10) / 11;
+  const int db = n < 4096 ? 8 : 11, nb = 1 << db;
+  const int passes = (bits + db - 1) / db;
   tmp.resize(n);
   Freq *bufs[2] = {out.data(), tmp.data()};
   int dst = (passes & 1) ? 0 : 1;  // the last pass writes `out`
   const Freq *src = in;
-  std::vector<uint32_t> cnt(2048);
+  std::vector<uint32_t> cnt(nb);  // on the heap: an 8 KB frame here slowed the level's host passes 2x
   for (int ps = 0; ps < passes; ++ps) {
-    const int sh = 11 * ps;
+    const int sh = db * ps;
     std::fill(cnt.begin(), cnt.end(), 0u);
-    for (int64_t i = 0; i < n; ++i) cnt[(src[i].key >> sh) & 2047]++;
+    for (int64_t i = 0; i < n; ++i) cnt[(src[i].key >> sh) & (nb - 1)]++;
     uint32_t run = 0;
-    for (uint32_t &c : cnt) {
-      const uint32_t x = c;
-      c = run;
+    for (int d = 0; d < nb; ++d) {
+      const uint32_t x = cnt[d];
+      cnt[d] = run;
       run += x;
     }
     Freq *d = bufs[dst];
-    for (int64_t i = 0; i < n; ++i) d[cnt[(src[i].key >> sh) & 2047]++] = src[i];
+    for (int64_t i = 0; i < n; ++i) d[cnt[(src[i].key >> sh) & (nb - 1)]++] = src[i];
     src = d;
     dst ^= 1;
   }
@@ -745,7 +746,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const size_t head = sizeof(Counters) + 3 * sizeof(unsigned long long) * guess;
     char *hbuf = static_cast<char *>(w.down.ensure(head));
     hip_check(launch_copy8(w.down.dev, dctr, head, s), "download counters");
+    Timer t_sync;
     hip_check(hipStreamSynchronize(s), "k_support");
+    const double us_sync = t_sync.sec() * 1e6;
     Timer t_host;  // host work between the level's download and its upload
     Counters cnt;
     std::memcpy(&cnt, hbuf, sizeof(cnt));
@@ -803,7 +806,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     knv.resize(F);
     knode.resize(F);
     kalive.resize(F);
-    Pool::get().run(F, threads, [&](int64_t i) {
+    auto decode = [&](int64_t i) {
       const int64_t x = (int64_t)(fr[i].key % K);
       DFS t;
       int cnv;
@@ -829,9 +832,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       ktup[i] = t;
       knv[i] = (uint8_t)std::min(cnv, 255);
       knode[i] = 0;
-    }, 4096);
-    int max_nv_all = 0;
-    for (int64_t i = 0; i < F; ++i) max_nv_all = std::max(max_nv_all, (int)knv[i]);
+    };
+    // a child has at most one vertex more than its parent
+    const int max_nv_bound = root ? 2 : L.max_nv + 1;
     // the canonical check of child i (its parent's code plus its tuple)
     auto is_min_child = [&](int64_t i) {
       const int64_t p = (int64_t)(fr[i].key / K);
@@ -851,8 +854,18 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // at most 30 % of its rows (each costs a written, unused row; molecule-
     // like databases, rich in symmetric patterns, hold about half)
     const bool want_spec = opt.speculate == 2 || (opt.speculate == 0 && dead_share <= 0.3);
-    const bool spec = want_spec && !root && !opt.queries && !past_max && max_nv_all <= kMaxPatV;
+    // short lists are decoded while the device writes (a serial pass on the
+    // critical path otherwise); long ones in parallel before the write
+    const bool early_decode = F >= 4096 || max_nv_bound > kMaxPatV;
+    if (early_decode) Pool::get().run(F, threads, decode, 4096);
+    int max_nv_exact = 0;
+    if (early_decode)
+      for (int64_t i = 0; i < F; ++i) max_nv_exact = std::max(max_nv_exact, (int)knv[i]);
+    const bool spec = want_spec && !root && !opt.queries && !past_max &&
+                      (early_decode ? max_nv_exact : max_nv_bound) <= kMaxPatV;
+    const bool late_decode = spec && !early_decode;
     if (!spec) {
+      if (!early_decode) Pool::get().run(F, threads, decode, 4096);
       std::atomic<bool> too_big(false);
       Pool::get().run(F, threads, [&](int64_t i) {
         const uint32_t sup = (uint32_t)fr[i].sup;
@@ -876,7 +889,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       });
       if (too_big) throw Error(GM_E_UNSUPPORTED, "pattern with more than 32 vertices is frequent");
     } else {
-      for (int64_t i = 0; i < F; ++i) kidx[i] = 0;
+      for (int64_t i = 0; i < F; ++i) kidx[i] = 0;  // decoded while the device writes
     }
     canon_sec += tcan.sec();
     double us_canon = tcan.sec() * 1e6;
@@ -890,11 +903,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       if (kidx[i] < 0) continue;
       kidx[i] = C++;
       child_rows += (int64_t)fr[i].proj;
-      max_nv = std::max(max_nv, (int)knv[i]);
+      if (!late_decode) max_nv = std::max(max_nv, (int)knv[i]);
     }
+    if (late_decode) max_nv = max_nv_bound;
     N.resize(C, new_len);
     nproj.resize(C);
-    double us_select = 0, us_host = 0, us_overlap = 0;
+    double us_select = 0, us_host = 0, us_overlap = 0, us_crit = 0;
     int64_t alive = 0, alive_rows = 0;  // minimal children and their embeddings
     int CS = 4;
     pending_rows = -1;
@@ -932,6 +946,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       hip_check(launch_upload_scatter(du, su, 0, w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(su + keys_off),
                                       reinterpret_cast<const int32_t *>(su + vals_off), C, s),
                 "scatter child ids");
+      us_crit = t_host.sec() * 1e6;
       grow(w.pid[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.rgraph[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.vt[nxt], (size_t)CS * std::max<int64_t>(child_rows, 1), s);
@@ -1041,13 +1056,20 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         // one pass per child: check, metadata, masks (the path table is known)
         Pool::get().run(F, threads, [&](int64_t i) {
           if (kidx[i] < 0) return;
-          if (spec) kalive[i] = is_min_child(i) ? 1 : 0;
+          if (spec) {
+            if (late_decode) decode(i);
+            kalive[i] = is_min_child(i) ? 1 : 0;
+          }
           build_child(i);
           if (masks) mask_child(kidx[i]);
         });
       } else {
         // the 2-edge level: its minimal children are the path table
-        if (spec) Pool::get().run(F, threads, [&](int64_t i) { kalive[i] = is_min_child(i) ? 1 : 0; });
+        if (spec)
+          Pool::get().run(F, threads, [&](int64_t i) {
+            if (late_decode) decode(i);
+            kalive[i] = is_min_child(i) ? 1 : 0;
+          });
         Pool::get().run(F, threads, build_child, 4096);
         if (new_len == 2) {
           // the job's frequent 2-edge patterns: (0,1,a,e1,b) then
@@ -1074,8 +1096,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       }
       int next_nrm = 0;  // over every child: LDS slots hold any row's rightmost path
       int64_t dead_rows = 0;
+      N.max_nv = 0;
       for (int64_t c = 0; c < C; ++c) {
         next_nrm = std::max(next_nrm, (int)hm[c].nrm);
+        N.max_nv = std::max(N.max_nv, (int)N.nv[c]);
         if (!N.alive[c]) {
           dead_rows += (int64_t)nproj[c];
           continue;
@@ -1130,11 +1154,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
               "[gm level %d] patterns %lld rows %lld K %lld rep %d block %d table_bits %d support_us %.1f "
               "candidates %llu pruned %llu frequent %u children %lld child_rows %lld minimal %lld spec %d splits %llu "
               "max_fill %llu lists %d "
-              "host_us prep %.1f canon %.1f select %.1f total %.1f overlap %.1f\n",
+              "host_us prep %.1f canon %.1f select %.1f total %.1f overlap %.1f sync %.1f crit %.1f\n",
               level, (long long)P, (long long)(root ? db.n_vertices : n_rows), (long long)K, rep, a.block,
               a.table_bits, ms * 1e3, ecand, pruned, cnt.fcount, (long long)C, (long long)child_rows, (long long)alive,
               spec ? 1 : 0, cnt.splits, max_fill,
-              a.use_lists, us_prep, us_canon, us_select, us_host, us_overlap);
+              a.use_lists, us_prep, us_canon, us_select, us_host, us_overlap, us_sync, us_crit);
     st.n_levels++;
     if (stopped) break;
     std::swap(L, N);
