@@ -73,6 +73,24 @@ struct GView {
   __device__ __forceinline__ uint32_t entry(int32_t a) const { return A[a - aoff]; }
 };
 
+// Degree of an adjacency entry's neighbour (packed in its top byte).
+__device__ __forceinline__ int entry_deg(uint32_t e) { return (int)(e >> 24); }
+
+// Whether graph vertex w is one of the first nv bytes of a tuple (words
+// rec[0..)): a zero byte of rec ^ (w in every byte) marks a match; bytes at
+// or past nv (padding) are masked out, and a borrow only moves towards
+// higher (masked) bytes.
+__device__ __forceinline__ bool tuple_has(const uint32_t *rec, int nv, int w) {
+  const uint32_t pat = (uint32_t)w * 0x01010101u;
+  for (int i = 0; i * 4 < nv; ++i) {
+    const uint32_t x = rec[i] ^ pat;
+    const int valid = min(4, nv - 4 * i);
+    const uint32_t live = valid == 4 ? 0x80808080u : (0x80808080u & ((1u << (8 * valid)) - 1u));
+    if ((x - 0x01010101u) & ~x & live) return true;
+  }
+  return false;
+}
+
 // 256-bit set of graph-local vertex ids held in four 64-bit values.
 struct VSet {
   uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;
@@ -155,7 +173,7 @@ __device__ __forceinline__ void enumerate_row(const GView &G, const PatMeta *__r
     const uint32_t e = G.entry(a);
     const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
     if (!vis.has(w)) {
-      if (nl >= minlabel) emit(fwd_base + (maxtoc * NEL + el) * NVL + nl, w, own + G.deg(w));
+      if (nl >= minlabel) emit(fwd_base + (maxtoc * NEL + el) * NVL + nl, w, own + entry_deg(e));
     } else {
       for (int i = 1; i < nrm; ++i) {
         const uint32_t rme = rmv[i];
@@ -179,7 +197,7 @@ __device__ __forceinline__ void enumerate_row(const GView &G, const PatMeta *__r
       const uint32_t e = G.entry(a);
       const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
       if (vis.has(w) || nl < minlabel) continue;
-      if (e1 < el || (e1 == el && l1 <= nl)) emit(fwd_base + (f * NEL + el) * NVL + nl, w, keep + G.deg(w));
+      if (e1 < el || (e1 == el && l1 <= nl)) emit(fwd_base + (f * NEL + el) * NVL + nl, w, keep + entry_deg(e));
     }
     below += G.deg(u);
   }
@@ -192,7 +210,7 @@ __device__ __forceinline__ void enumerate_root(const GView &G, int u, int lu, in
   for (int32_t a = G.begin(u), ae = G.end(u); a < ae; ++a) {
     const uint32_t e = G.entry(a);
     const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
-    if (lu <= nl) emit((lu * NEL + el) * NVL + nl, w, du + G.deg(w));
+    if (lu <= nl) emit((lu * NEL + el) * NVL + nl, w, du + entry_deg(e));
   }
 }
 
@@ -307,6 +325,34 @@ __device__ __forceinline__ T block_sum(T x, T *s) {
   return t;
 }
 
+// Wave-level helpers (one wave synchronises with itself only).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class T>
+__device__ __forceinline__ T wave_exclusive_scan(T x, T &total) {
+  const int lane = threadIdx.x & 63;
+  T v = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const T y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  total = __shfl(v, 63, 64);
+  return v - x;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+
 __device__ __forceinline__ uint32_t hash_slot32(uint32_t k, int bits) { return (k * 0x9E3779B1u) >> (32 - bits); }
 
 // Rows of graph g at the current level (root level: its vertices).
@@ -376,8 +422,11 @@ __device__ __forceinline__ int find_row(const int *pre, int j, int n) {
 // Row record (32-bit words): [0, TW) tuple bytes; [TW, TW + RMW) rightmost
 // path; [TW + RMW, + SPW) item prefix per source: 0, deg(rv), + deg(rm 0),
 // ...; then pattern id, PatMeta word 1 (nv, nrm, minlabel, maxlabel),
-// bwd_mask, the graph's first global vertex, list index (2 words).
+// bwd_mask, list index (2 words); then the first adjacency entry of each
+// source (SPW words).  Whether a neighbour is already in the embedding is
+// read from the tuple bytes (tuple_has).
 constexpr int kExpandRows = 64;
+constexpr int kExpandUnroll = 4;  // item windows whose loads are issued together
 
 __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
   __shared__ int s_pre[kExpandRows];
@@ -385,7 +434,6 @@ __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
   const int t = threadIdx.x;
   const int TW = a.tuple_words, RMW = a.rm_words, SPW = a.src_words;
   uint32_t *recs = dyn;
-  unsigned long long *vis = reinterpret_cast<unsigned long long *>(dyn + kExpandRows * a.slot_words);
   const int64_t r0 = (int64_t)blockIdx.x * kExpandRows;
   const int64_t r = r0 + t;
   const int nrow = (int)min<int64_t>(kExpandRows, a.n_rows - r0);
@@ -411,134 +459,147 @@ __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
       load_slot(mine, a.vt, r, a.S);
       uint32_t *rmv = mine + TW;
       stage_rm(pm, rmv);
-      VSet vs;
-      for (int i = 0; i < nv; ++i) vs.add(slot_get(mine, i));
-      vis[t * 4 + 0] = vs.m0;
-      vis[t * 4 + 1] = vs.m1;
-      vis[t * 4 + 2] = vs.m2;
-      vis[t * 4 + 3] = vs.m3;
       uint32_t *sp = rmv + RMW;
-      int acc = G.deg(slot_get(mine, nv - 1));
+      uint32_t *misc = sp + SPW;
+      uint32_t *sb = misc + 5;  // first adjacency entry of each source
+      const int rvv = slot_get(mine, nv - 1);
+      sb[0] = (uint32_t)G.begin(rvv);
+      int acc = G.end(rvv) - (int)sb[0];
       sp[0] = 0;
       sp[1] = (uint32_t)acc;
       for (int i = 0; i < nrm; ++i) {
-        acc += G.deg(slot_get(mine, rmv[i] & 0xFF));
+        const int u = slot_get(mine, rmv[i] & 0xFF);
+        sb[i + 1] = (uint32_t)G.begin(u);
+        acc += G.end(u) - (int)sb[i + 1];
         sp[i + 2] = (uint32_t)acc;
       }
-      uint32_t *misc = sp + SPW;
       const unsigned long long li = (unsigned long long)gb + a.rcoff[r];
       misc[0] = (uint32_t)p;
       misc[1] = head.y;
       misc[2] = head.z;
-      misc[3] = (uint32_t)vb;
-      misc[4] = (uint32_t)li;
-      misc[5] = (uint32_t)(li >> 32);
+      misc[3] = (uint32_t)li;
+      misc[4] = (uint32_t)(li >> 32);
       n = acc;
     }
   }
   int T;
-  const int pre = block_exclusive_scan<int>(n, T, s_pre);  // one wave: cheap barriers
-  __syncthreads();
+  const int pre = wave_exclusive_scan<int>(n, T);
   s_pre[t] = pre;
-  __syncthreads();
+  wave_sync();
   int cnt = 0;  // candidates of row t listed so far
   unsigned long long cand = 0, bound_sum = 0, pruned = 0;
   const int ms = a.mask_stride;
-  for (int jb = 0; jb < T; jb += kExpandRows) {
-    const int j = jb + t;
-    int k = 0, key = -1, w = 0, bound = 0;
-    bool drop = false;  // a valid extension the path rule removes
-    if (j < T) {
-      k = find_row(s_pre, j, nrow);
-      const int i = j - s_pre[k];
-      const uint32_t *rec = recs + k * a.slot_words;
-      const uint32_t *rmv = rec + TW;
-      const uint32_t *sp = rmv + RMW;
-      const uint32_t *misc = sp + SPW;
-      const uint32_t hy = misc[1];
-      const int nv = (int)(hy & 0xFF), nrm = (int)((hy >> 8) & 0xFF);
-      const int minlabel = (int)((hy >> 16) & 0xFF), maxlabel = (int)(hy >> 24);
-      const int maxtoc = nv - 1;
-      const unsigned long long *vrow = vis + k * 4;
-      GView G;
-      G.vadj = a.vadj + misc[3];
-      G.A = a.adj;
-      G.aoff = 0;
-      const int own = (int)sp[nrm + 1];
-      const unsigned long long *mk = ms ? a.pmask + (int64_t)misc[0] * 2 * ms : nullptr;
-      int src = 0;  // 0: the rightmost vertex; s >= 1: rightmost-path source s - 1
-      while ((int)sp[src + 1] <= i) ++src;
-      if (src == 0) {
-        const int rv = slot_get(rec, maxtoc);
-        const uint32_t e = G.entry(G.begin(rv) + i);
+  // kExpandUnroll windows of 64 items per step: every window's adjacency
+  // entry is loaded first, then the windows are evaluated and compacted in
+  // order (a row's list positions follow its items)
+  for (int jb0 = 0; jb0 < T; jb0 += kExpandUnroll * kExpandRows) {
+    int kk[kExpandUnroll], srcs[kExpandUnroll], ii[kExpandUnroll];
+    uint32_t ee[kExpandUnroll];
+#pragma unroll
+    for (int u = 0; u < kExpandUnroll; ++u) {
+      const int j = jb0 + u * kExpandRows + t;
+      kk[u] = 0;
+      srcs[u] = 0;
+      ii[u] = 0;
+      ee[u] = 0;
+      if (j < T) {
+        const int k = find_row(s_pre, j, nrow);
+        const int i = j - s_pre[k];
+        const uint32_t *sp = recs + k * a.slot_words + TW + RMW;
+        const uint32_t *sb = sp + SPW + 5;
+        int src = 0;  // 0: the rightmost vertex; s >= 1: rightmost-path source s - 1
+        while ((int)sp[src + 1] <= i) ++src;
+        kk[u] = k;
+        srcs[u] = src;
+        ii[u] = i;
+        ee[u] = a.adj[sb[src] + (i - (int)sp[src])];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kExpandUnroll; ++u) {
+      const int jb = jb0 + u * kExpandRows;
+      const int j = jb + t;
+      const int k = kk[u];
+      int key = -1, w = 0, bound = 0;
+      bool drop = false;  // a valid extension the path rule removes
+      if (j < T) {
+        const int i = ii[u], src = srcs[u];
+        const uint32_t e = ee[u];
+        const uint32_t *rec = recs + k * a.slot_words;
+        const uint32_t *rmv = rec + TW;
+        const uint32_t *sp = rmv + RMW;
+        const uint32_t *misc = sp + SPW;
+        const uint32_t hy = misc[1];
+        const int nv = (int)(hy & 0xFF), nrm = (int)((hy >> 8) & 0xFF);
+        const int minlabel = (int)((hy >> 16) & 0xFF), maxlabel = (int)(hy >> 24);
+        const int maxtoc = nv - 1;
+        const int own = (int)sp[nrm + 1];
+        const unsigned long long *mk = ms ? a.pmask + (int64_t)misc[0] * 2 * ms : nullptr;
         w = e & 0xFF;
         const int el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
-        if (!((vrow[w >> 6] >> (w & 63)) & 1ull)) {
-          if (nl >= minlabel) {
-            key = fwd_base + (maxtoc * NEL + el) * NVL + nl;
-            bound = own + G.deg(w);
-            drop = mk && !((mk[0] >> (el * NVL + nl)) & 1ull);
+        if (src == 0) {
+          if (!tuple_has(rec, nv, w)) {
+            if (nl >= minlabel) {
+              key = fwd_base + (maxtoc * NEL + el) * NVL + nl;
+              bound = own + entry_deg(e);
+              drop = mk && !((mk[0] >> (el * NVL + nl)) & 1ull);
+            }
+          } else {
+            const uint32_t bwd_mask = misc[2];
+            for (int q = 1; q < nrm; ++q) {
+              const uint32_t rme = rmv[q];
+              const int f = rme & 0xFF;
+              if (slot_get(rec, f) != w) continue;
+              const int e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
+              if (!((bwd_mask >> f) & 1u) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
+                key = f * NEL + el;
+                bound = own;
+                drop = mk && !((mk[ms + 1 + q] >> el) & 1ull);
+              }
+              break;
+            }
+            w = -1;
           }
         } else {
-          const uint32_t bwd_mask = misc[2];
-          for (int q = 1; q < nrm; ++q) {
-            const uint32_t rme = rmv[q];
-            const int f = rme & 0xFF;
-            if (slot_get(rec, f) != w) continue;
-            const int e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
-            if (!((bwd_mask >> f) & 1u) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
-              key = f * NEL + el;
-              bound = own;
-              drop = mk && !((mk[ms + 1 + q] >> el) & 1ull);
-            }
-            break;
+          const uint32_t rme = rmv[src - 1];
+          const int f = rme & 0xFF;
+          const int e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
+          if (nl >= minlabel && (e1 < el || (e1 == el && l1 <= nl)) && !tuple_has(rec, nv, w)) {
+            key = fwd_base + (f * NEL + el) * NVL + nl;
+            bound = own - (int)sp[src] + entry_deg(e);  // sources src - 1 .. root, plus w
+            drop = mk && !((mk[src] >> (el * NVL + nl)) & 1ull);
           }
-          w = -1;
         }
-      } else {
-        const uint32_t rme = rmv[src - 1];
-        const int f = rme & 0xFF;
-        const int u = slot_get(rec, f);
-        const int e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
-        const uint32_t e = G.entry(G.begin(u) + (i - (int)sp[src]));
-        w = e & 0xFF;
-        const int el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
-        if (!((vrow[w >> 6] >> (w & 63)) & 1ull) && nl >= minlabel && (e1 < el || (e1 == el && l1 <= nl))) {
-          key = fwd_base + (f * NEL + el) * NVL + nl;
-          bound = own - (int)sp[src] + G.deg(w);  // sources src - 1 .. root, plus w
-          drop = mk && !((mk[src] >> (el * NVL + nl)) & 1ull);
+        if (drop) {  // counted as a candidate, never listed
+          key = -1;
+          ++pruned;
         }
       }
-      if (drop) {  // counted as a candidate, never listed
-        key = -1;
-        ++pruned;
+      const bool emit = key >= 0;
+      const unsigned long long mask = __ballot(emit);
+      const int first = j < T ? max(s_pre[k], jb) - jb : 0;  // lane of row k's first item in the window
+      const unsigned long long below = mask & ((1ull << t) - 1) & ~((1ull << first) - 1);
+      const int listed = __shfl(cnt, k, kExpandRows);
+      if (emit) {
+        const uint32_t *misc = recs + k * a.slot_words + TW + RMW + SPW;
+        const unsigned long long li = (unsigned long long)misc[3] | ((unsigned long long)misc[4] << 32);
+        a.cand[li + listed + __popcll(below)] = make_uint2(cand_code(false, key, w), (uint32_t)bound);
+        ++cand;
+        bound_sum += (unsigned long long)bound;
       }
-    }
-    const bool emit = key >= 0;
-    const unsigned long long mask = __ballot(emit);
-    const int first = j < T ? max(s_pre[k], jb) - jb : 0;  // lane of row k's first item in the window
-    const unsigned long long below = mask & ((1ull << t) - 1) & ~((1ull << first) - 1);
-    const int listed = __shfl(cnt, k, kExpandRows);
-    if (emit) {
-      const uint32_t *misc = recs + k * a.slot_words + TW + RMW + SPW;
-      const unsigned long long li = (unsigned long long)misc[4] | ((unsigned long long)misc[5] << 32);
-      a.cand[li + listed + __popcll(below)] = make_uint2(cand_code(false, key, w), (uint32_t)bound);
-      ++cand;
-      bound_sum += (unsigned long long)bound;
-    }
-    // row t's items in this window: lanes [lo, hi)
-    const int lo = max(pre, jb) - jb, hi = min(pre + n, jb + kExpandRows) - jb;
-    if (hi > lo) {
-      const unsigned long long range = (hi - lo == 64 ? ~0ull : (((1ull << (hi - lo)) - 1) << lo));
-      cnt += __popcll(mask & range);
+      // row t's items in this window: lanes [lo, hi)
+      const int lo = max(pre, jb) - jb, hi = min(pre + n, jb + kExpandRows) - jb;
+      if (hi > lo) {
+        const unsigned long long range = (hi - lo == 64 ? ~0ull : (((1ull << (hi - lo)) - 1) << lo));
+        cnt += __popcll(mask & range);
+      }
     }
   }
   if (has_list) a.rccnt[r] = (uint16_t)cnt;
-  // statistics: one atomic per workgroup each, spread over kSpread lines
-  __shared__ unsigned long long s_red[2];
-  cand = block_sum<unsigned long long>(cand + pruned, s_red);
-  bound_sum = block_sum<unsigned long long>(bound_sum, s_red);
-  pruned = block_sum<unsigned long long>(pruned, s_red);
+  // statistics: one atomic per wave each, spread over kSpread lines
+  cand = wave_sum(cand + pruned);
+  bound_sum = wave_sum(bound_sum);
+  pruned = wave_sum(pruned);
   if (t == 0) {
     unsigned long long *line = a.spread + (blockIdx.x & (kSpread - 1)) * kSpreadStride;
     if (cand) atomicAdd(&line[kSpreadCand], cand);
@@ -849,32 +910,6 @@ __device__ __forceinline__ bool acc_add(uint32_t *acc, int bits, int *fill, uint
 constexpr int kWaveBlock = 256;    // 4 waves per workgroup
 constexpr int kWaveTableBits = 9;  // per-wave dedup table: 512 keys
 constexpr int kRootAccBits = 9;    // workgroup accumulation table: 512 keys
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <class T>
-__device__ __forceinline__ T wave_exclusive_scan(T x, T &total) {
-  const int lane = threadIdx.x & 63;
-  T v = x;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const T y = __shfl_up(v, o, 64);
-    if (lane >= o) v += y;
-  }
-  total = __shfl(v, 63, 64);
-  return v - x;
-}
-
-template <class T>
-__device__ __forceinline__ T wave_sum(T x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
 
 // Per-wave LDS (32-bit words): table keys [cap], counts [cap], item prefix
 // [64], row records [64 x 2] (list offset, key base), split stack [2 x 32],
@@ -1598,7 +1633,7 @@ hipError_t launch_support(const LevelArgs &a, int n_graphs, bool root, hipStream
   return hipGetLastError();
 }
 
-size_t expand_lds_bytes(const LevelArgs &a) { return 4ull * kExpandRows * a.slot_words + 32ull * kExpandRows; }
+size_t expand_lds_bytes(const LevelArgs &a) { return 4ull * kExpandRows * a.slot_words; }
 
 hipError_t launch_expand(const LevelArgs &a, hipStream_t s) {
   if (a.n_rows <= 0) return hipSuccess;

@@ -202,11 +202,13 @@ void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk
         acc += deg[v];
       }
       adj.resize(acc);
+      // entry: neighbour | elabel << 8 | neighbour label << 16 | neighbour
+      // degree << 24 (a simple graph of <= 256 vertices has degree <= 255)
       for (int32_t e = e0; e < e1; ++e) {
         int32_t a = db->efrom[e], b = db->eto[e];
         uint32_t el = (uint32_t)emap(db->elabel[e]);
-        adj[abase + fill[a]++] = (uint32_t)b | (el << 8) | ((uint32_t)dl[b] << 16);
-        adj[abase + fill[b]++] = (uint32_t)a | (el << 8) | ((uint32_t)dl[a] << 16);
+        adj[abase + fill[a]++] = (uint32_t)b | (el << 8) | ((uint32_t)dl[b] << 16) | ((uint32_t)deg[b] << 24);
+        adj[abase + fill[b]++] = (uint32_t)a | (el << 8) | ((uint32_t)dl[a] << 16) | ((uint32_t)deg[a] << 24);
       }
     }
     ++job;
@@ -728,7 +730,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       LevelArgs e = a;
       e.rm_words = rm_words;
       e.src_words = max_nrm + 2;
-      e.slot_words = (a.tuple_words + e.rm_words + e.src_words + 6) | 1;
+      e.slot_words = (a.tuple_words + e.rm_words + 2 * e.src_words + 5) | 1;
       if (expand_lds_bytes(e) > (size_t)lds_max)
         throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
       hip_check(launch_expand(e, s), "k_expand");
