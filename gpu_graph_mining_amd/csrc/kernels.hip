@@ -1627,6 +1627,7 @@ hipError_t launch_support(const LevelArgs &a, int n_graphs, bool root, hipStream
     if (a.big_grid > 0) {
       LevelArgs b = a;
       b.block = a.big_block;
+      b.table_bits = a.big_table_bits;
       hipLaunchKernelGGL((k_support<false, true>), dim3(a.big_grid), dim3(b.block), level_lds_bytes(b, true), s, b);
     }
   }

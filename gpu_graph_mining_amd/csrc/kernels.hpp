@@ -116,6 +116,7 @@ struct LevelArgs {
   const int32_t *big;
   const unsigned long long *nbig;
   int32_t big_rows, big_block, big_grid;
+  int32_t big_table_bits;  // dedup table of the large-graph launch (>= table_bits)
   int32_t wave_grid;  // workgroups of the persistent per-wave root kernel
   int32_t *nbig_list;
   unsigned long long *nnbig;

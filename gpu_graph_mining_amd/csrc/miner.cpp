@@ -720,6 +720,16 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
       a.table_bits = b.table_bits;
     }
+    // the large-graph launch has few workgroups per CU: its table takes up
+    // to 2^12 entries, so that graphs with thousands of distinct keys do not
+    // split their passes
+    {
+      LevelArgs b = a;
+      b.block = a.big_block;
+      b.table_bits = std::max(a.table_bits, table_bits_for(root ? (uint64_t)db.max_graph_adj : 2ull * last_fill));
+      while (level_lds_bytes(b, true) > (size_t)lds_max && b.table_bits > a.table_bits) --b.table_bits;
+      a.big_table_bits = b.table_bits;
+    }
 
     // 32-bit dedup keys: launch over pattern ranges with range * K < 2^32
     const int64_t chunk = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
