@@ -73,6 +73,21 @@ struct GView {
   __device__ __forceinline__ uint32_t entry(int32_t a) const { return A[a - aoff]; }
 };
 
+// Wall-clock stamps of the extension + support stage: the first workgroups
+// of a stage kernel record ~start (the max of complements is the earliest
+// start), every working workgroup its end; the host reads them from the
+// spread lines instead of timing events, which cost a queue gap each.
+__device__ __forceinline__ void stamp_start(const LevelArgs &a) {
+  if (blockIdx.x < kSpread && threadIdx.x == 0)
+    atomicMax(&a.spread[blockIdx.x * kSpreadStride + kSpreadStart],
+              ~(unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+__device__ __forceinline__ void stamp_end(const LevelArgs &a) {
+  if (threadIdx.x == 0)
+    atomicMax(&a.spread[(blockIdx.x & (kSpread - 1)) * kSpreadStride + kSpreadEnd],
+              (unsigned long long)__builtin_amdgcn_s_memrealtime());
+}
+
 // Degree of an adjacency entry's neighbour (packed in its top byte).
 __device__ __forceinline__ int entry_deg(uint32_t e) { return (int)(e >> 24); }
 
@@ -429,6 +444,7 @@ constexpr int kExpandRows = 64;
 constexpr int kExpandUnroll = 4;  // item windows whose loads are issued together
 
 __global__ void __launch_bounds__(kExpandRows) k_expand(LevelArgs a) {
+  stamp_start(a);
   __shared__ int s_pre[kExpandRows];
   extern __shared__ uint32_t dyn[];
   const int t = threadIdx.x;
@@ -849,6 +865,7 @@ __device__ __forceinline__ int pick_graph(const LevelArgs &a) {
 template <bool ROOT, bool BIG>
 __global__ void __launch_bounds__(kMaxBlock) k_support(LevelArgs a) {
   __shared__ SupportShared sh;
+  if (!BIG) stamp_start(a);  // levels without k_expand start here
   const int g = pick_graph<BIG>(a);
   if (g < 0) return;
   if (ROOT) {
@@ -864,6 +881,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_support(LevelArgs a) {
     support_body<ROOT, true>(a, g, r0, r1, l, sh);
   else
     support_body<ROOT, false>(a, g, r0, r1, l, sh);
+  stamp_end(a);
 }
 
 // Merge one graph's count for key k into a workgroup's accumulation table

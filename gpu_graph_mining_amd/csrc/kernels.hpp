@@ -22,6 +22,8 @@ constexpr int kSpreadCand = 0;     // extension candidates enumerated
 constexpr int kSpreadFill = 1;     // largest dedup table fill (atomicMax)
 constexpr int kSpreadBound = 2;    // sum of the candidates' child bounds
 constexpr int kSpreadPruned = 3;   // candidates dropped by the 2-edge path rule
+constexpr int kSpreadStart = 4;    // ~(earliest start) of the extension + support stage (atomicMax)
+constexpr int kSpreadEnd = 5;      // latest end of the stage (atomicMax); device wall clock
 
 // Everything one level-step kernel reads or writes.  Plain device pointers.
 struct LevelArgs {

@@ -529,16 +529,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   // the root level's lists sit at the adjacency offsets
   if (opt.candidate_lists) grow(w.cand[0], sizeof(uint2) * std::max<int64_t>(db.n_adj, 1), s);
 
-  hipEvent_t ev0, ev1;
-  hip_check(hipEventCreate(&ev0), "hipEventCreate");
-  hip_check(hipEventCreate(&ev1), "hipEventCreate");
-  struct EvGuard {
-    hipEvent_t a, b;
-    ~EvGuard() {
-      (void)hipEventDestroy(a);
-      (void)hipEventDestroy(b);
-    }
-  } evg{ev0, ev1};
   double ext_ms = 0, dev_sec = 0, canon_sec = 0;
   uint64_t ext_launches = 0;
   size_t peak = db.bytes;
@@ -566,8 +556,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   }
   a.big_block = kBigBlock;
   a.stage_a = std::min<int32_t>(db.max_graph_adj, 4096);
-  int dev = 0, lds_max = 0;
+  int dev = 0, lds_max = 0, wall_khz = 1;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
+  if (wall_khz <= 0) wall_khz = 100000;
   hip_check(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
   {
     int cus = 0;
@@ -733,7 +725,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
 
     // 32-bit dedup keys: launch over pattern ranges with range * K < 2^32
     const int64_t chunk = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
-    hip_check(hipEventRecord(ev0, s), "hipEventRecord");
     if (!root && a.use_lists) {
       // list every row's candidates in one parallel pass; the support
       // launches then only deduplicate per graph
@@ -751,7 +742,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.p_hi = (int32_t)std::min<int64_t>(P, lo + chunk);
       hip_check(launch_support(a, G, root, s), "k_support");
     }
-    hip_check(hipEventRecord(ev1, s), "hipEventRecord");
     hip_check(launch_reduce(a, root, s), "k_reduce");
     // counters and (most often all of) the frequent keys in one copy
     const uint32_t guess = std::min<uint32_t>(fcap, std::max<uint32_t>(4096, 2 * last_fcount));
@@ -771,8 +761,14 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       max_fill = std::max(max_fill, cnt.spread[i * kSpreadStride + kSpreadFill]);
       cand_bound += cnt.spread[i * kSpreadStride + kSpreadBound];
     }
-    float ms = 0;
-    hip_check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+    // the stage's span from the kernels' wall-clock stamps
+    unsigned long long t_start = 0, t_end = 0;
+    for (int i = 0; i < kSpread; ++i) {
+      t_start = std::max(t_start, cnt.spread[i * kSpreadStride + kSpreadStart]);
+      t_end = std::max(t_end, cnt.spread[i * kSpreadStride + kSpreadEnd]);
+    }
+    t_start = ~t_start;
+    const double ms = (t_end > t_start && t_start != ~0ull) ? (double)(t_end - t_start) / wall_khz : 0.0;
     if (!root) {
       ext_ms += ms;
       ext_launches++;
@@ -955,10 +951,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       }
       char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
       const char *su = static_cast<const char *>(w.up.dev);
-      hip_check(launch_upload_scatter(du, su, 0, w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(su + keys_off),
-                                      reinterpret_cast<const int32_t *>(su + vals_off), C, s),
-                "scatter child ids");
-      us_crit = t_host.sec() * 1e6;
       grow(w.pid[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.rgraph[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.vt[nxt], (size_t)CS * std::max<int64_t>(child_rows, 1), s);
@@ -996,6 +988,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.nbig_list = w.big[nxt].as<int32_t>();
       a.nnbig = w.nbig.as<unsigned long long>() + nxt;
       a.nrcoff = w.rcoff[nxt].as<uint32_t>();
+      // the selected keys point at their child ids, then the write pass
+      // (launched back to back: the device does not wait on the host between)
+      hip_check(launch_upload_scatter(du, su, 0, w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(su + keys_off),
+                                      reinterpret_cast<const int32_t *>(su + vals_off), C, s),
+                "scatter child ids");
+      us_crit = t_host.sec() * 1e6;
       hip_check(launch_write(a, G, root, s), "k_write");
       counters_clear = true;  // its scan cleared the counters for the next level
       us_select = t_sel.sec() * 1e6;
