@@ -1604,6 +1604,318 @@ __global__ void __launch_bounds__(kScanTile) k_scan_apply(LevelArgs a, int n) {
   if (big) a.nbig_list[at] = g;
 }
 
+// --------------------------------------------------------- ROW-PARALLEL WRITE
+// Later levels whose candidates k_expand listed (every graph with
+// candidates then has lists): the children are counted, placed and written
+// over rows instead of graphs -- one single-wave workgroup per 64
+// consecutive rows whatever their graphs, so a graph of thousands of rows
+// spreads over many waves rather than running serially in one workgroup,
+// and no workgroup barrier is needed.
+//   k_rows<false>  per row: selected candidates and their bound sum (rowcnt,
+//                  rowbsum); per wave: the totals (wrows, wents);
+//   k_wscan        exclusive scans of the wave totals, in place; the grand
+//                  totals go to the cursors;
+//   k_rows<true>   per row: its children at the wave's base plus the row's
+//                  prefix, in list order, their list offsets absolute (the
+//                  next level's per-graph list base is 0, -1 without lists);
+//                  a graph's first and last row record its child-row and
+//                  list-entry ranges (cgstart, gbsum; cgcount, ngcbase);
+//   k_gfix         per graph: child count, list flag, next big-graph list,
+//                  largest graph.
+// Child rows come out graph-major in graph order, as from the per-graph
+// passes.
+constexpr int kRowWave = kRowWaveRows;
+constexpr int kRowUnroll = 4;  // item windows whose loads are issued together
+
+template <bool WRITE>
+__global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
+  __shared__ int s_pre[kRowWave];
+  __shared__ uint32_t s_li[kRowWave];
+  __shared__ int32_t s_p[kRowWave];
+  __shared__ int32_t s_g[kRowWave];
+  __shared__ int32_t s_nvp[kRowWave];
+  __shared__ long long s_pos[kRowWave];
+  __shared__ unsigned long long s_ent[kRowWave];
+  extern __shared__ uint32_t tup[];  // WRITE: row t's tuple at tup[t * TW, + TW)
+  const int t = threadIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.x * kRowWave;
+  const int64_t r = r0 + t;
+  const int nrow = (int)min<int64_t>(kRowWave, a.n_rows - r0);
+  const int TW = a.S >> 2;
+  int n = 0, g = -1;
+  int32_t p = 0;
+  uint32_t li = 0;
+  if (t < nrow) {
+    g = a.rgraph[r];
+    p = a.pid[r];
+    const long long gb = a.gcbase[g];
+    if (gb >= 0) {  // graphs without lists have no candidates
+      n = a.rccnt[r];
+      li = (uint32_t)(gb + a.rcoff[r]);
+    }
+    if (WRITE) load_slot(tup + t * TW, a.vt, r, a.S);
+  }
+  int T;
+  const int pre = wave_exclusive_scan<int>(n, T);
+  s_pre[t] = pre;
+  s_li[t] = li;
+  s_p[t] = p;
+  if (WRITE) {
+    // the row's first child row and first list entry
+    uint32_t rc = 0, rb = 0;
+    int nvp = 0;
+    if (t < nrow) {
+      rc = a.rowcnt[r];
+      rb = a.rowbsum[r];
+      if (rc) nvp = a.meta[p].nv;
+    }
+    unsigned long long tc, tb;
+    const long long pos0 = (long long)a.wrows[blockIdx.x] + (long long)wave_exclusive_scan<unsigned long long>(rc, tc);
+    const unsigned long long ent0 = a.wents[blockIdx.x] + wave_exclusive_scan<unsigned long long>(rb, tb);
+    s_g[t] = g;
+    s_nvp[t] = nvp;
+    s_pos[t] = pos0;
+    s_ent[t] = ent0;
+    // graph boundaries (rows are graph-major): the neighbours' graphs
+    int gprev = __shfl(g, t > 0 ? t - 1 : 0, kRowWave);
+    int gnext = __shfl(g, t < kRowWave - 1 ? t + 1 : t, kRowWave);
+    if (t == 0) gprev = r > 0 ? a.rgraph[r - 1] : -1;
+    if (t == nrow - 1) gnext = r + 1 < a.n_rows ? a.rgraph[r + 1] : -1;
+    if (t < nrow) {
+      if (gprev != g) {
+        a.cgstart[g] = pos0;
+        a.gbsum[g] = ent0;
+      }
+      if (gnext != g) {  // ends; k_gfix turns them into the count and the list flag
+        a.cgcount[g] = pos0 + (long long)rc;
+        a.ngcbase[g] = (long long)(ent0 + rb);
+      }
+    }
+  }
+  wave_sync();
+  const int CW = a.CS >> 2;
+  uint32_t cnt = 0, bsum = 0;  // row t's selected candidates so far, their bound sum
+  for (int jb0 = 0; jb0 < T; jb0 += kRowUnroll * kRowWave) {
+    int kk[kRowUnroll];
+    uint2 c[kRowUnroll];
+    int32_t v[kRowUnroll];
+#pragma unroll
+    for (int u = 0; u < kRowUnroll; ++u) {
+      const int j = jb0 + u * kRowWave + t;
+      kk[u] = 0;
+      c[u] = make_uint2(0, 0);
+      if (j < T) {
+        const int k = find_row(s_pre, j, nrow);
+        kk[u] = k;
+        c[u] = a.cand[(uint64_t)s_li[k] + (uint64_t)(j - s_pre[k])];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRowUnroll; ++u) {
+      v[u] = 0;
+      if (jb0 + u * kRowWave + t < T) {
+        int key, w;
+        cand_decode(false, c[u].x, key, w);
+        v[u] = a.sup[(uint64_t)s_p[kk[u]] * (uint64_t)a.K + (uint64_t)key];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRowUnroll; ++u) {
+      const int jb = jb0 + u * kRowWave;
+      const bool live = jb + t < T;
+      const bool sel = live && v[u] < 0;
+      const unsigned long long mask = __ballot(sel);
+      const uint32_t x = sel ? c[u].y : 0u;
+      uint32_t xt;
+      const uint32_t ex = wave_exclusive_scan<uint32_t>(x, xt);
+      const int k = kk[u];
+      if (WRITE) {
+        const int first = live ? max(s_pre[k], jb) - jb : 0;  // lane of row k's first item here
+        const uint32_t kc = __shfl(cnt, k, kRowWave), kb = __shfl(bsum, k, kRowWave);
+        const uint32_t exf = __shfl(ex, first, kRowWave);
+        if (sel) {
+          const unsigned long long below = mask & ((1ull << t) - 1) & ~((1ull << first) - 1);
+          const long long pos = s_pos[k] + (long long)kc + (long long)__popcll(below);
+          if (pos < a.ccap) {
+            int key, w;
+            cand_decode(false, c[u].x, key, w);
+            a.cpid[pos] = -v[u] - 1;
+            a.crgraph[pos] = s_g[k];
+            const uint32_t *src = tup + k * TW;
+            uint32_t *dst = reinterpret_cast<uint32_t *>(a.cvt + pos * a.CS);
+            const int nvp = s_nvp[k];
+            for (int i = 0; i < CW; ++i) {
+              uint32_t word = i < TW ? src[i] : 0u;
+              if (w >= 0 && (nvp >> 2) == i) {
+                const int sh = (nvp & 3) * 8;
+                word = (word & ~(0xFFu << sh)) | ((uint32_t)w << sh);
+              }
+              dst[i] = word;
+            }
+            if (a.next_lists) a.nrcoff[pos] = (uint32_t)(s_ent[k] + kb + (ex - exf));
+          }
+        }
+      }
+      // row t's items in this window: lanes [lo, hi)
+      const int lo = max(pre, jb) - jb, hi = min(pre + n, jb + kRowWave) - jb;
+      const uint32_t hi_inc = __shfl(ex + x, hi > lo ? hi - 1 : 0, kRowWave);
+      const uint32_t lo_ex = __shfl(ex, hi > lo ? lo : 0, kRowWave);
+      if (hi > lo) {
+        const unsigned long long range = (hi - lo == 64 ? ~0ull : (((1ull << (hi - lo)) - 1) << lo));
+        cnt += (uint32_t)__popcll(mask & range);
+        bsum += hi_inc - lo_ex;
+      }
+    }
+  }
+  if (!WRITE) {
+    if (t < nrow) {
+      a.rowcnt[r] = (uint16_t)cnt;
+      a.rowbsum[r] = bsum;
+    }
+    const unsigned long long wr = wave_sum<unsigned long long>(cnt), we = wave_sum<unsigned long long>(bsum);
+    if (t == 0) {
+      a.wrows[blockIdx.x] = wr;
+      a.wents[blockIdx.x] = we;
+    }
+  }
+}
+
+// Exclusive scans of the wave totals in place (n waves, up to kScanTile *
+// kScanSinglePer in one workgroup: thread t the consecutive waves
+// [t * per, (t + 1) * per)); the totals to the cursors, the next level's
+// big-graph count and largest graph zeroed for k_gfix.
+__global__ void __launch_bounds__(kScanTile) k_wscan_single(LevelArgs a, int n) {
+  __shared__ long long s[kScanTile / 64 + 1];
+  __shared__ unsigned long long su[kScanTile / 64 + 1];
+  const int per = (n + kScanTile - 1) / kScanTile;
+  const int g0 = min(n, (int)threadIdx.x * per), g1 = min(n, g0 + per);
+  unsigned long long rv[kScanSinglePer], ev[kScanSinglePer];
+#pragma unroll
+  for (int k = 0; k < kScanSinglePer; ++k) {
+    const int i = g0 + k;
+    const bool in = i < g1;
+    rv[k] = in ? a.wrows[i] : 0;
+    ev[k] = in ? a.wents[i] : 0;
+  }
+  long long rsum = 0;
+  unsigned long long esum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanSinglePer; ++k) {
+    rsum += (long long)rv[k];
+    esum += ev[k];
+  }
+  long long tr;
+  unsigned long long te;
+  long long rows = block_exclusive_scan<long long>(rsum, tr, s);
+  unsigned long long ents = block_exclusive_scan<unsigned long long>(esum, te, su);
+#pragma unroll
+  for (int k = 0; k < kScanSinglePer; ++k) {
+    const int i = g0 + k;
+    if (i < g1) {
+      a.wrows[i] = (unsigned long long)rows;
+      a.wents[i] = ents;
+    }
+    rows += (long long)rv[k];
+    ents += ev[k];
+  }
+  if (threadIdx.x == 0) {
+    a.cursor[0] = (unsigned long long)tr;
+    *a.ncand_cursor = te;
+    a.cursor[2] = 0;
+    *a.nnbig = 0;
+  }
+  clear_level_counters(a);
+}
+
+// More waves: tile sums, one workgroup over the tiles, per-tile apply (the
+// tile arrays hold a tile's row and list-entry sums, then its offsets).
+__global__ void __launch_bounds__(kScanTile) k_wscan_tiles(LevelArgs a, int n) {
+  __shared__ long long s[kScanTile / 64];
+  __shared__ unsigned long long su[kScanTile / 64];
+  const int i = blockIdx.x * kScanTile + threadIdx.x;
+  const long long rows = block_sum<long long>(i < n ? (long long)a.wrows[i] : 0, s);
+  const unsigned long long ents = block_sum<unsigned long long>(i < n ? a.wents[i] : 0, su);
+  if (threadIdx.x == 0) {
+    a.tile_rows[blockIdx.x] = rows;
+    a.tile_ents[blockIdx.x] = ents;
+  }
+}
+
+__global__ void __launch_bounds__(kScanTile) k_wscan_top(LevelArgs a, int ntiles) {
+  __shared__ long long s[kScanTile / 64 + 1];
+  __shared__ unsigned long long su[kScanTile / 64 + 1];
+  long long crow = 0;
+  unsigned long long cent = 0;
+  for (int base = 0; base < ntiles; base += kScanTile) {
+    const int i = base + threadIdx.x;
+    const bool live = i < ntiles;
+    long long tr;
+    unsigned long long te;
+    const long long r = block_exclusive_scan<long long>(live ? a.tile_rows[i] : 0, tr, s);
+    const unsigned long long e = block_exclusive_scan<unsigned long long>(live ? a.tile_ents[i] : 0, te, su);
+    if (live) {
+      a.tile_rows[i] = crow + r;
+      a.tile_ents[i] = cent + e;
+    }
+    crow += tr;
+    cent += te;
+  }
+  if (threadIdx.x == 0) {
+    a.cursor[0] = (unsigned long long)crow;
+    *a.ncand_cursor = cent;
+    a.cursor[2] = 0;
+    *a.nnbig = 0;
+  }
+  clear_level_counters(a);
+}
+
+__global__ void __launch_bounds__(kScanTile) k_wscan_apply(LevelArgs a, int n) {
+  __shared__ long long s[kScanTile / 64 + 1];
+  __shared__ unsigned long long su[kScanTile / 64 + 1];
+  const int i = blockIdx.x * kScanTile + threadIdx.x;
+  long long tr;
+  unsigned long long te;
+  const long long r = block_exclusive_scan<long long>(i < n ? (long long)a.wrows[i] : 0, tr, s);
+  const unsigned long long e = block_exclusive_scan<unsigned long long>(i < n ? a.wents[i] : 0, te, su);
+  if (i >= n) return;
+  a.wrows[i] = (unsigned long long)(a.tile_rows[blockIdx.x] + r);
+  a.wents[i] = a.tile_ents[blockIdx.x] + e;
+}
+
+// Per graph after k_rows<true>: the child count from the recorded range
+// (graphs without rows at this level: none), the next level's list flag
+// (0: lists at absolute offsets, -1: none), the big-graph list (wave-
+// aggregated append; its order does not matter) and the largest graph.
+__global__ void __launch_bounds__(256) k_gfix(LevelArgs a, int n_graphs) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  long long c = 0;
+  bool lists = false;
+  if (g < n_graphs) {
+    if (a.gcount[g] > 0) {
+      const long long s0 = a.cgstart[g], s1 = a.cgcount[g];
+      const unsigned long long e0 = a.gbsum[g], e1 = (unsigned long long)a.ngcbase[g];
+      c = s1 > a.ccap ? 0 : s1 - s0;  // past the allocation: not written; the host sees the total
+      const unsigned long long b = e1 - e0;
+      lists = a.next_lists && b > 0 && e1 <= 0xFFFFFFFFull && e1 <= (unsigned long long)a.ncand_cap;
+    } else {
+      a.cgstart[g] = 0;
+    }
+    a.cgcount[g] = c;
+    a.ngcbase[g] = lists ? 0 : -1;
+  }
+  const bool big = c > a.big_rows;
+  const unsigned long long bm = __ballot(big);
+  const int lane = threadIdx.x & 63;
+  uint32_t at = 0;
+  const int leader = bm ? __ffsll((long long)bm) - 1 : 0;
+  if (bm && lane == leader) at = (uint32_t)atomicAdd(a.nnbig, (unsigned long long)__popcll(bm));
+  at = __shfl(at, leader, 64);
+  if (big) a.nbig_list[at + (uint32_t)__popcll(bm & ((1ull << lane) - 1))] = g;
+  long long m = c;
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (long long)__shfl_xor(m, o, 64));
+  if (lane == 0 && m > 0) atomicMax(&a.cursor[2], (unsigned long long)m);
+}
+
 // ------------------------------------------------------------ primitives
 // Copies between device memory and mapped pinned host memory on the compute
 // queue (a copy engine transfer between two kernels costs a cross-engine
@@ -1700,6 +2012,23 @@ hipError_t launch_clear_counters(const LevelArgs &a, hipStream_t s) {
 
 hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t s) {
   if (n_graphs <= 0) return hipSuccess;
+  if (a.rows_mode) {
+    const int64_t nw = (a.n_rows + kRowWave - 1) / kRowWave;
+    if (nw <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rows<false>, dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
+    if (nw <= kScanTile * kScanSinglePer) {
+      hipLaunchKernelGGL(k_wscan_single, dim3(1), dim3(kScanTile), 0, s, a, (int)nw);
+    } else {
+      const unsigned tiles = (unsigned)((nw + kScanTile - 1) / kScanTile);
+      hipLaunchKernelGGL(k_wscan_tiles, dim3(tiles), dim3(kScanTile), 0, s, a, (int)nw);
+      hipLaunchKernelGGL(k_wscan_top, dim3(1), dim3(kScanTile), 0, s, a, (int)tiles);
+      hipLaunchKernelGGL(k_wscan_apply, dim3(tiles), dim3(kScanTile), 0, s, a, (int)nw);
+    }
+    hipLaunchKernelGGL(k_rows<true>, dim3((unsigned)nw), dim3(kRowWave), sizeof(uint32_t) * kRowWave * (a.S >> 2), s,
+                       a);
+    hipLaunchKernelGGL(k_gfix, dim3((unsigned)((n_graphs + 255) / 256)), dim3(256), 0, s, a, n_graphs);
+    return hipGetLastError();
+  }
   const size_t lds = level_lds_bytes(a, false);
   LevelArgs b = a;
   b.block = a.big_block;

@@ -13,6 +13,7 @@ namespace gm {
 // 65535 (uint16 counts): a row extends by at most 32 * 256 edges.
 constexpr int kMaxBlock = 256;
 constexpr int kScanTileGraphs = 1024;  // graphs per tile of the placement scan
+constexpr int kRowWaveRows = 64;       // rows per workgroup of the row-parallel children passes
 
 // Statistics counters that every workgroup bumps are spread over kSpread
 // cache lines: atomics on one address serialise across the device.
@@ -122,7 +123,13 @@ struct LevelArgs {
   int32_t wave_grid;  // workgroups of the persistent per-wave root kernel
   int32_t *nbig_list;
   unsigned long long *nnbig;
-  // placement scan: per tile of 1024 graphs, sums then exclusive offsets
+  // Row-parallel children passes (later levels whose candidates k_expand
+  // listed): per wave of kRowWaveRows rows, the child rows and list entries,
+  // scanned in place into the wave's first child row and list entry
+  int32_t rows_mode;
+  unsigned long long *wrows, *wents;
+  // placement scan: per tile of 1024 graphs (row-parallel passes: waves),
+  // sums then exclusive offsets
   long long *tile_rows;
   unsigned long long *tile_ents;
   long long *tile_big;

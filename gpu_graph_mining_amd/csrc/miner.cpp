@@ -988,6 +988,26 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.nbig_list = w.big[nxt].as<int32_t>();
       a.nnbig = w.nbig.as<unsigned long long>() + nxt;
       a.nrcoff = w.rcoff[nxt].as<uint32_t>();
+      // where k_expand listed every row's candidates the children are
+      // counted, placed and written over waves of rows (a large graph
+      // spreads over many waves), else per graph
+      a.rows_mode = (!root && a.lists_ready) ? 1 : 0;
+      {
+        const int64_t nw = a.rows_mode ? (n_rows + kRowWaveRows - 1) / kRowWaveRows : 0;
+        if (a.rows_mode) {
+          auto *wt = static_cast<unsigned long long *>(
+              grow(w.wtot, 2 * sizeof(unsigned long long) * std::max<int64_t>(nw, 1), s));
+          a.wrows = wt;
+          a.wents = wt + nw;
+        }
+        // the placement scan's tiles: of graphs, or of waves of rows
+        const int64_t tiles = (std::max<int64_t>(G, nw) + kScanTileGraphs - 1) / kScanTileGraphs + 1;
+        auto *t = static_cast<long long *>(grow(w.scan_tiles, 4 * sizeof(long long) * tiles, s));
+        a.tile_rows = t;
+        a.tile_ents = reinterpret_cast<unsigned long long *>(t + tiles);
+        a.tile_big = t + 2 * tiles;
+        a.tile_max = t + 3 * tiles;
+      }
       // the selected keys point at their child ids, then the write pass
       // (launched back to back: the device does not wait on the host between)
       hip_check(launch_upload_scatter(du, su, 0, w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(su + keys_off),

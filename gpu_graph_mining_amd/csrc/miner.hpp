@@ -53,6 +53,7 @@ struct Workspace {
   DevBuf gs[2], gc[2], pid[2], rgraph[2], vt[2], upl[2], cand[2], gcb[2], rcoff[2], big[2];
   DevBuf nbig;  // big-graph counts of the two buffered levels
   DevBuf scan_tiles;  // placement scan: per-tile sums and offsets
+  DevBuf wtot;        // row-parallel children passes: per-wave child rows, list entries
   HostBuf up, down;
   // the replica counters hold zeros outside a level step (k_reduce clears
   // what it reads); false after a call that ended early
