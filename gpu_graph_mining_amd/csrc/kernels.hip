@@ -1120,6 +1120,11 @@ __global__ void __launch_bounds__(kWaveBlock) k_root_wave(LevelArgs a, int n_gra
 // (key, support, embeddings).
 template <bool ROOT>
 __global__ void __launch_bounds__(256) k_reduce(LevelArgs a) {
+  // the row-parallel children passes' tile sums start at zero
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n_wtiles; i += gridDim.x * blockDim.x) {
+    a.trows[i] = 0;
+    a.tents[i] = 0;
+  }
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nkeys; i += (int64_t)gridDim.x * blockDim.x) {
     unsigned long long v = 0;
     for (int r = 0; r < a.rep; ++r) {
@@ -1612,10 +1617,12 @@ __global__ void __launch_bounds__(kScanTile) k_scan_apply(LevelArgs a, int n) {
 // spreads over many waves rather than running serially in one workgroup,
 // and no workgroup barrier is needed.
 //   k_rows<false>  per row: selected candidates and their bound sum (rowcnt,
-//                  rowbsum); per wave: the totals (wrows, wents);
-//   k_wscan        exclusive scans of the wave totals, in place; the grand
-//                  totals go to the cursors;
-//   k_rows<true>   per row: its children at the wave's base plus the row's
+//                  rowbsum); per wave: the totals (wrows, wents), also added
+//                  to its tile's sums (trows, tents: 64 waves a tile);
+//   k_rtop         one workgroup: exclusive scans of the tile sums, in
+//                  place; the grand totals go to the cursors;
+//   k_rows<true>   per row: its children at the wave's base (its tile's
+//                  offset plus the tile's earlier waves) plus the row's
 //                  prefix, in list order, their list offsets absolute (the
 //                  next level's per-graph list base is 0, -1 without lists);
 //                  a graph's first and last row record its child-row and
@@ -1669,9 +1676,14 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
       rb = a.rowbsum[r];
       if (rc) nvp = a.meta[p].nv;
     }
+    // the wave's base: its tile's offset plus the tile's earlier waves
+    const int64_t w0 = (int64_t)blockIdx.x & ~(int64_t)63;
+    const bool earlier = t < (int)(blockIdx.x & 63);
+    const unsigned long long br = a.trows[blockIdx.x >> 6] + wave_sum<unsigned long long>(earlier ? a.wrows[w0 + t] : 0);
+    const unsigned long long be = a.tents[blockIdx.x >> 6] + wave_sum<unsigned long long>(earlier ? a.wents[w0 + t] : 0);
     unsigned long long tc, tb;
-    const long long pos0 = (long long)a.wrows[blockIdx.x] + (long long)wave_exclusive_scan<unsigned long long>(rc, tc);
-    const unsigned long long ent0 = a.wents[blockIdx.x] + wave_exclusive_scan<unsigned long long>(rb, tb);
+    const long long pos0 = (long long)br + (long long)wave_exclusive_scan<unsigned long long>(rc, tc);
+    const unsigned long long ent0 = be + wave_exclusive_scan<unsigned long long>(rb, tb);
     s_g[t] = g;
     s_nvp[t] = nvp;
     s_pos[t] = pos0;
@@ -1776,86 +1788,30 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
     if (t == 0) {
       a.wrows[blockIdx.x] = wr;
       a.wents[blockIdx.x] = we;
+      if (wr) atomicAdd(&a.trows[blockIdx.x >> 6], wr);
+      if (we) atomicAdd(&a.tents[blockIdx.x >> 6], we);
     }
   }
 }
 
-// Exclusive scans of the wave totals in place (n waves, up to kScanTile *
-// kScanSinglePer in one workgroup: thread t the consecutive waves
-// [t * per, (t + 1) * per)); the totals to the cursors, the next level's
-// big-graph count and largest graph zeroed for k_gfix.
-__global__ void __launch_bounds__(kScanTile) k_wscan_single(LevelArgs a, int n) {
-  __shared__ long long s[kScanTile / 64 + 1];
-  __shared__ unsigned long long su[kScanTile / 64 + 1];
-  const int per = (n + kScanTile - 1) / kScanTile;
-  const int g0 = min(n, (int)threadIdx.x * per), g1 = min(n, g0 + per);
-  unsigned long long rv[kScanSinglePer], ev[kScanSinglePer];
-#pragma unroll
-  for (int k = 0; k < kScanSinglePer; ++k) {
-    const int i = g0 + k;
-    const bool in = i < g1;
-    rv[k] = in ? a.wrows[i] : 0;
-    ev[k] = in ? a.wents[i] : 0;
-  }
-  long long rsum = 0;
-  unsigned long long esum = 0;
-#pragma unroll
-  for (int k = 0; k < kScanSinglePer; ++k) {
-    rsum += (long long)rv[k];
-    esum += ev[k];
-  }
-  long long tr;
-  unsigned long long te;
-  long long rows = block_exclusive_scan<long long>(rsum, tr, s);
-  unsigned long long ents = block_exclusive_scan<unsigned long long>(esum, te, su);
-#pragma unroll
-  for (int k = 0; k < kScanSinglePer; ++k) {
-    const int i = g0 + k;
-    if (i < g1) {
-      a.wrows[i] = (unsigned long long)rows;
-      a.wents[i] = ents;
-    }
-    rows += (long long)rv[k];
-    ents += ev[k];
-  }
-  if (threadIdx.x == 0) {
-    a.cursor[0] = (unsigned long long)tr;
-    *a.ncand_cursor = te;
-    a.cursor[2] = 0;
-    *a.nnbig = 0;
-  }
-  clear_level_counters(a);
-}
-
-// More waves: tile sums, one workgroup over the tiles, per-tile apply (the
-// tile arrays hold a tile's row and list-entry sums, then its offsets).
-__global__ void __launch_bounds__(kScanTile) k_wscan_tiles(LevelArgs a, int n) {
-  __shared__ long long s[kScanTile / 64];
-  __shared__ unsigned long long su[kScanTile / 64];
-  const int i = blockIdx.x * kScanTile + threadIdx.x;
-  const long long rows = block_sum<long long>(i < n ? (long long)a.wrows[i] : 0, s);
-  const unsigned long long ents = block_sum<unsigned long long>(i < n ? a.wents[i] : 0, su);
-  if (threadIdx.x == 0) {
-    a.tile_rows[blockIdx.x] = rows;
-    a.tile_ents[blockIdx.x] = ents;
-  }
-}
-
-__global__ void __launch_bounds__(kScanTile) k_wscan_top(LevelArgs a, int ntiles) {
+// Exclusive scans of the tile sums in place (n tiles of 64 waves); the
+// totals to the cursors, the next level's big-graph count and largest graph
+// zeroed for k_gfix.
+__global__ void __launch_bounds__(kScanTile) k_rtop(LevelArgs a, int n) {
   __shared__ long long s[kScanTile / 64 + 1];
   __shared__ unsigned long long su[kScanTile / 64 + 1];
   long long crow = 0;
   unsigned long long cent = 0;
-  for (int base = 0; base < ntiles; base += kScanTile) {
+  for (int base = 0; base < n; base += kScanTile) {
     const int i = base + threadIdx.x;
-    const bool live = i < ntiles;
+    const bool live = i < n;
     long long tr;
     unsigned long long te;
-    const long long r = block_exclusive_scan<long long>(live ? a.tile_rows[i] : 0, tr, s);
-    const unsigned long long e = block_exclusive_scan<unsigned long long>(live ? a.tile_ents[i] : 0, te, su);
+    const long long r = block_exclusive_scan<long long>(live ? (long long)a.trows[i] : 0, tr, s);
+    const unsigned long long e = block_exclusive_scan<unsigned long long>(live ? a.tents[i] : 0, te, su);
     if (live) {
-      a.tile_rows[i] = crow + r;
-      a.tile_ents[i] = cent + e;
+      a.trows[i] = (unsigned long long)(crow + r);
+      a.tents[i] = cent + e;
     }
     crow += tr;
     cent += te;
@@ -1867,19 +1823,6 @@ __global__ void __launch_bounds__(kScanTile) k_wscan_top(LevelArgs a, int ntiles
     *a.nnbig = 0;
   }
   clear_level_counters(a);
-}
-
-__global__ void __launch_bounds__(kScanTile) k_wscan_apply(LevelArgs a, int n) {
-  __shared__ long long s[kScanTile / 64 + 1];
-  __shared__ unsigned long long su[kScanTile / 64 + 1];
-  const int i = blockIdx.x * kScanTile + threadIdx.x;
-  long long tr;
-  unsigned long long te;
-  const long long r = block_exclusive_scan<long long>(i < n ? (long long)a.wrows[i] : 0, tr, s);
-  const unsigned long long e = block_exclusive_scan<unsigned long long>(i < n ? a.wents[i] : 0, te, su);
-  if (i >= n) return;
-  a.wrows[i] = (unsigned long long)(a.tile_rows[blockIdx.x] + r);
-  a.wents[i] = a.tile_ents[blockIdx.x] + e;
 }
 
 // Per graph after k_rows<true>: the child count from the recorded range
@@ -2015,15 +1958,9 @@ hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t
   if (a.rows_mode) {
     const int64_t nw = (a.n_rows + kRowWave - 1) / kRowWave;
     if (nw <= 0) return hipSuccess;
+    if ((nw + 63) / 64 > a.n_wtiles) return hipErrorInvalidValue;  // tile sums not sized (or not zeroed) for nw
     hipLaunchKernelGGL(k_rows<false>, dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
-    if (nw <= kScanTile * kScanSinglePer) {
-      hipLaunchKernelGGL(k_wscan_single, dim3(1), dim3(kScanTile), 0, s, a, (int)nw);
-    } else {
-      const unsigned tiles = (unsigned)((nw + kScanTile - 1) / kScanTile);
-      hipLaunchKernelGGL(k_wscan_tiles, dim3(tiles), dim3(kScanTile), 0, s, a, (int)nw);
-      hipLaunchKernelGGL(k_wscan_top, dim3(1), dim3(kScanTile), 0, s, a, (int)tiles);
-      hipLaunchKernelGGL(k_wscan_apply, dim3(tiles), dim3(kScanTile), 0, s, a, (int)nw);
-    }
+    hipLaunchKernelGGL(k_rtop, dim3(1), dim3(kScanTile), 0, s, a, (int)((nw + 63) / 64));
     hipLaunchKernelGGL(k_rows<true>, dim3((unsigned)nw), dim3(kRowWave), sizeof(uint32_t) * kRowWave * (a.S >> 2), s,
                        a);
     hipLaunchKernelGGL(k_gfix, dim3((unsigned)((n_graphs + 255) / 256)), dim3(256), 0, s, a, n_graphs);

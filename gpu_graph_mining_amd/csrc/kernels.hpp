@@ -124,10 +124,12 @@ struct LevelArgs {
   int32_t *nbig_list;
   unsigned long long *nnbig;
   // Row-parallel children passes (later levels whose candidates k_expand
-  // listed): per wave of kRowWaveRows rows, the child rows and list entries,
-  // scanned in place into the wave's first child row and list entry
+  // listed): per wave of kRowWaveRows rows, the child rows and list entries
+  // (wrows, wents); per tile of 64 waves their sums, then exclusive offsets
+  // (trows, tents; n_wtiles tiles, zeroed by k_reduce)
   int32_t rows_mode;
-  unsigned long long *wrows, *wents;
+  int32_t n_wtiles;
+  unsigned long long *wrows, *wents, *trows, *tents;
   // placement scan: per tile of 1024 graphs (row-parallel passes: waves),
   // sums then exclusive offsets
   long long *tile_rows;

@@ -742,6 +742,18 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.p_hi = (int32_t)std::min<int64_t>(P, lo + chunk);
       hip_check(launch_support(a, G, root, s), "k_support");
     }
+    // per-wave totals and 64-wave tile sums of the row-parallel children
+    // passes (k_reduce zeroes the tile sums)
+    a.n_wtiles = 0;
+    if (!root && a.lists_ready) {
+      const int64_t nw = (n_rows + kRowWaveRows - 1) / kRowWaveRows, nt = (nw + 63) / 64;
+      auto *wt = static_cast<unsigned long long *>(grow(w.wtot, sizeof(unsigned long long) * 2 * (nw + nt), s));
+      a.wrows = wt;
+      a.wents = wt + nw;
+      a.trows = wt + 2 * nw;
+      a.tents = wt + 2 * nw + nt;
+      a.n_wtiles = (int32_t)nt;
+    }
     hip_check(launch_reduce(a, root, s), "k_reduce");
     // counters and (most often all of) the frequent keys in one copy
     const uint32_t guess = std::min<uint32_t>(fcap, std::max<uint32_t>(4096, 2 * last_fcount));
@@ -991,23 +1003,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       // where k_expand listed every row's candidates the children are
       // counted, placed and written over waves of rows (a large graph
       // spreads over many waves), else per graph
-      a.rows_mode = (!root && a.lists_ready) ? 1 : 0;
-      {
-        const int64_t nw = a.rows_mode ? (n_rows + kRowWaveRows - 1) / kRowWaveRows : 0;
-        if (a.rows_mode) {
-          auto *wt = static_cast<unsigned long long *>(
-              grow(w.wtot, 2 * sizeof(unsigned long long) * std::max<int64_t>(nw, 1), s));
-          a.wrows = wt;
-          a.wents = wt + nw;
-        }
-        // the placement scan's tiles: of graphs, or of waves of rows
-        const int64_t tiles = (std::max<int64_t>(G, nw) + kScanTileGraphs - 1) / kScanTileGraphs + 1;
-        auto *t = static_cast<long long *>(grow(w.scan_tiles, 4 * sizeof(long long) * tiles, s));
-        a.tile_rows = t;
-        a.tile_ents = reinterpret_cast<unsigned long long *>(t + tiles);
-        a.tile_big = t + 2 * tiles;
-        a.tile_max = t + 3 * tiles;
-      }
+      a.rows_mode = (!root && a.lists_ready && a.n_wtiles > 0) ? 1 : 0;
       // the selected keys point at their child ids, then the write pass
       // (launched back to back: the device does not wait on the host between)
       hip_check(launch_upload_scatter(du, su, 0, w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(su + keys_off),
