@@ -363,9 +363,10 @@ class Pool {
   }
   // Run fn(i) for i in [0, n) on up to `threads` threads (the caller helps);
   // below `serial_below` items the caller runs them alone (waking the
-  // workers costs tens of microseconds).  Callers on different threads
-  // (concurrent mining calls on different handles) take turns: the pool
-  // runs one function at a time.
+  // workers costs tens of microseconds).  The call returns once every item
+  // is done: a worker that wakes after the items are all taken is not
+  // waited for.  Callers on different threads (concurrent mining calls on
+  // different handles) take turns: the pool runs one function at a time.
   void run(int64_t n, int threads, const std::function<void(int64_t)> &fn, int64_t serial_below = 64) {
     if (n <= 0) return;
     if (threads <= 1 || n < serial_below) {
@@ -373,21 +374,22 @@ class Pool {
       return;
     }
     std::lock_guard<std::mutex> one_at_a_time(run_m_);
-    std::unique_lock<std::mutex> lk(m_);
-    while ((int)workers_.size() < threads - 1) workers_.emplace_back([this]() { loop(); });
-    fn_ = &fn;
-    n_ = n;
-    next_ = 0;
-    err_ = nullptr;
-    active_ = (int)workers_.size();
-    ++gen_;
-    lk.unlock();
+    auto job = std::make_shared<Job>();
+    job->fn = &fn;
+    job->n = n;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      while ((int)workers_.size() < threads - 1) workers_.emplace_back([this]() { loop(); });
+      job_ = job;
+      ++gen_;
+    }
     cv_.notify_all();
-    work();
-    lk.lock();
-    done_.wait(lk, [this]() { return active_ == 0; });
-    fn_ = nullptr;
-    if (err_) std::rethrow_exception(err_);
+    job->work();
+    {
+      std::unique_lock<std::mutex> lk(job->m);
+      job->cv.wait(lk, [&]() { return job->finished.load() == job->n; });
+    }
+    if (job->err) std::rethrow_exception(job->err);
   }
   ~Pool() {
     {
@@ -399,43 +401,62 @@ class Pool {
   }
 
  private:
-  void work() {
-    for (;;) {
-      const int64_t i = next_.fetch_add(32);
-      if (i >= n_) break;
-      const int64_t e = std::min(n_, i + 32);
-      try {
-        for (int64_t j = i; j < e; ++j) (*fn_)(j);
-      } catch (...) {
-        std::lock_guard<std::mutex> lk(err_m_);
-        if (!err_) err_ = std::current_exception();
-        next_ = n_;
+  // One run: items are taken 32 at a time; `finished` counts the items
+  // done (or given up after an exception), so the caller waits for items,
+  // not for threads.  A worker holds the job it woke for, so one that runs
+  // late finds every item taken and never calls fn after the run returned.
+  struct Job {
+    const std::function<void(int64_t)> *fn = nullptr;
+    int64_t n = 0;
+    std::atomic<int64_t> next{0}, finished{0};
+    std::mutex m, err_m;
+    std::condition_variable cv;
+    std::exception_ptr err;
+    void done(int64_t k) {
+      if (finished.fetch_add(k) + k == n) {
+        std::lock_guard<std::mutex> lk(m);
+        cv.notify_all();
       }
     }
-  }
+    void work() {
+      for (;;) {
+        const int64_t i = next.fetch_add(32);
+        if (i >= n) break;
+        const int64_t e = std::min(n, i + 32);
+        try {
+          for (int64_t j = i; j < e; ++j) (*fn)(j);
+        } catch (...) {
+          {
+            std::lock_guard<std::mutex> lk(err_m);
+            if (!err) err = std::current_exception();
+          }
+          const int64_t left = next.exchange(n);  // items nobody will take
+          if (left < n) done(n - left);
+        }
+        done(e - i);
+      }
+    }
+  };
   void loop() {
     uint64_t seen = 0;
     for (;;) {
-      std::unique_lock<std::mutex> lk(m_);
-      cv_.wait(lk, [&]() { return stop_ || gen_ != seen; });
-      if (stop_) return;
-      seen = gen_;
-      lk.unlock();
-      work();
-      lk.lock();
-      if (--active_ == 0) done_.notify_all();
+      std::shared_ptr<Job> job;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&]() { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        job = job_;
+      }
+      job->work();
     }
   }
-  std::mutex run_m_, m_, err_m_;
-  std::condition_variable cv_, done_;
+  std::mutex run_m_, m_;
+  std::condition_variable cv_;
   std::vector<std::thread> workers_;
-  const std::function<void(int64_t)> *fn_ = nullptr;
-  std::atomic<int64_t> next_{0};
-  int64_t n_ = 0;
-  int active_ = 0;
+  std::shared_ptr<Job> job_;
   uint64_t gen_ = 0;
   bool stop_ = false;
-  std::exception_ptr err_;
 };
 
 // Device-side counters at the head of the frequent-key buffer (bytes):
