@@ -1122,8 +1122,8 @@ template <bool ROOT>
 __global__ void __launch_bounds__(256) k_reduce(LevelArgs a) {
   // the row-parallel children passes' tile sums start at zero
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n_wtiles; i += gridDim.x * blockDim.x) {
-    a.trows[i] = 0;
-    a.tents[i] = 0;
+    a.trows[(int64_t)i * kTileStride] = 0;
+    a.tents[(int64_t)i * kTileStride] = 0;
   }
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nkeys; i += (int64_t)gridDim.x * blockDim.x) {
     unsigned long long v = 0;
@@ -1679,8 +1679,9 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
     // the wave's base: its tile's offset plus the tile's earlier waves
     const int64_t w0 = (int64_t)blockIdx.x & ~(int64_t)63;
     const bool earlier = t < (int)(blockIdx.x & 63);
-    const unsigned long long br = a.trows[blockIdx.x >> 6] + wave_sum<unsigned long long>(earlier ? a.wrows[w0 + t] : 0);
-    const unsigned long long be = a.tents[blockIdx.x >> 6] + wave_sum<unsigned long long>(earlier ? a.wents[w0 + t] : 0);
+    const int64_t tile = (int64_t)(blockIdx.x >> 6) * kTileStride;
+    const unsigned long long br = a.trows[tile] + wave_sum<unsigned long long>(earlier ? a.wrows[w0 + t] : 0);
+    const unsigned long long be = a.tents[tile] + wave_sum<unsigned long long>(earlier ? a.wents[w0 + t] : 0);
     unsigned long long tc, tb;
     const long long pos0 = (long long)br + (long long)wave_exclusive_scan<unsigned long long>(rc, tc);
     const unsigned long long ent0 = be + wave_exclusive_scan<unsigned long long>(rb, tb);
@@ -1788,8 +1789,8 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
     if (t == 0) {
       a.wrows[blockIdx.x] = wr;
       a.wents[blockIdx.x] = we;
-      if (wr) atomicAdd(&a.trows[blockIdx.x >> 6], wr);
-      if (we) atomicAdd(&a.tents[blockIdx.x >> 6], we);
+      if (wr) atomicAdd(&a.trows[(int64_t)(blockIdx.x >> 6) * kTileStride], wr);
+      if (we) atomicAdd(&a.tents[(int64_t)(blockIdx.x >> 6) * kTileStride], we);
     }
   }
 }
@@ -1807,11 +1808,12 @@ __global__ void __launch_bounds__(kScanTile) k_rtop(LevelArgs a, int n) {
     const bool live = i < n;
     long long tr;
     unsigned long long te;
-    const long long r = block_exclusive_scan<long long>(live ? (long long)a.trows[i] : 0, tr, s);
-    const unsigned long long e = block_exclusive_scan<unsigned long long>(live ? a.tents[i] : 0, te, su);
+    const int64_t at = (int64_t)i * kTileStride;
+    const long long r = block_exclusive_scan<long long>(live ? (long long)a.trows[at] : 0, tr, s);
+    const unsigned long long e = block_exclusive_scan<unsigned long long>(live ? a.tents[at] : 0, te, su);
     if (live) {
-      a.trows[i] = (unsigned long long)(crow + r);
-      a.tents[i] = cent + e;
+      a.trows[at] = (unsigned long long)(crow + r);
+      a.tents[at] = cent + e;
     }
     crow += tr;
     cent += te;

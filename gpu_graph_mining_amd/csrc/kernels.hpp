@@ -14,6 +14,7 @@ namespace gm {
 constexpr int kMaxBlock = 256;
 constexpr int kScanTileGraphs = 1024;  // graphs per tile of the placement scan
 constexpr int kRowWaveRows = 64;       // rows per workgroup of the row-parallel children passes
+constexpr int kTileStride = 16;        // 64-bit words between tile sums: one cache line each (atomics on a line serialise)
 
 // Statistics counters that every workgroup bumps are spread over kSpread
 // cache lines: atomics on one address serialise across the device.
@@ -126,7 +127,8 @@ struct LevelArgs {
   // Row-parallel children passes (later levels whose candidates k_expand
   // listed): per wave of kRowWaveRows rows, the child rows and list entries
   // (wrows, wents); per tile of 64 waves their sums, then exclusive offsets
-  // (trows, tents; n_wtiles tiles, zeroed by k_reduce)
+  // (trows, tents: tile i at [i * kTileStride]; n_wtiles tiles, zeroed by
+  // k_reduce)
   int32_t rows_mode;
   int32_t n_wtiles;
   unsigned long long *wrows, *wents, *trows, *tents;

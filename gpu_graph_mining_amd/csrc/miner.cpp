@@ -768,11 +768,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     a.n_wtiles = 0;
     if (!root && a.lists_ready) {
       const int64_t nw = (n_rows + kRowWaveRows - 1) / kRowWaveRows, nt = (nw + 63) / 64;
-      auto *wt = static_cast<unsigned long long *>(grow(w.wtot, sizeof(unsigned long long) * 2 * (nw + nt), s));
+      const int64_t tw = nt * kTileStride;  // one cache line per tile sum
+      auto *wt = static_cast<unsigned long long *>(grow(w.wtot, sizeof(unsigned long long) * 2 * (nw + tw), s));
       a.wrows = wt;
       a.wents = wt + nw;
       a.trows = wt + 2 * nw;
-      a.tents = wt + 2 * nw + nt;
+      a.tents = wt + 2 * nw + tw;
       a.n_wtiles = (int32_t)nt;
     }
     hip_check(launch_reduce(a, root, s), "k_reduce");
