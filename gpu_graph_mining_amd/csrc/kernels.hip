@@ -1523,7 +1523,7 @@ __global__ void __launch_bounds__(kScanTile) k_scan_top(LevelArgs a, int ntiles)
 // The three scan kernels in one workgroup, for up to kScanTile *
 // kScanSinglePer graphs: thread t places the consecutive graphs
 // [t * per, (t + 1) * per).
-constexpr int kScanSinglePer = 16;
+constexpr int kScanSinglePer = kScanSingleGraphs / kScanTile;
 
 __global__ void __launch_bounds__(kScanTile) k_scan_single(LevelArgs a, int n) {
   __shared__ long long s[kScanTile / 64 + 1];
@@ -1616,6 +1616,8 @@ __global__ void __launch_bounds__(kScanTile) k_scan_apply(LevelArgs a, int n) {
 // consecutive rows whatever their graphs, so a graph of thousands of rows
 // spreads over many waves rather than running serially in one workgroup,
 // and no workgroup barrier is needed.
+// (The root level runs the same passes with the database's vertices as
+// rows: their lists sit at their adjacency offsets.)
 //   k_rows<false>  per row: selected candidates and their bound sum (rowcnt,
 //                  rowbsum); per wave: the totals (wrows, wents), also added
 //                  to its tile's sums (trows, tents: 64 waves a tile);
@@ -1634,8 +1636,10 @@ __global__ void __launch_bounds__(kScanTile) k_scan_apply(LevelArgs a, int n) {
 constexpr int kRowWave = kRowWaveRows;
 constexpr int kRowUnroll = 4;  // item windows whose loads are issued together
 
-template <bool WRITE>
+template <bool ROOT, bool WRITE>
 __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
+  // root level: the rows are the database's vertices (graph from vgraph,
+  // list at the vertex's adjacency offset, keys per job, child tuple (u, w))
   __shared__ int s_pre[kRowWave];
   __shared__ uint32_t s_li[kRowWave];
   __shared__ int32_t s_p[kRowWave];
@@ -1652,15 +1656,24 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
   int n = 0, g = -1;
   int32_t p = 0;
   uint32_t li = 0;
+  int uv = 0;  // root level: the vertex's index in its graph
   if (t < nrow) {
-    g = a.rgraph[r];
-    p = a.pid[r];
-    const long long gb = a.gcbase[g];
-    if (gb >= 0) {  // graphs without lists have no candidates
+    if (ROOT) {
+      g = a.vgraph[r];
+      p = a.gjob[g];
       n = a.rccnt[r];
-      li = (uint32_t)(gb + a.rcoff[r]);
+      li = (uint32_t)a.vadj[r];
+      uv = (int)(r - a.gvbase[g]);
+    } else {
+      g = a.rgraph[r];
+      p = a.pid[r];
+      const long long gb = a.gcbase[g];
+      if (gb >= 0) {  // graphs without lists have no candidates
+        n = a.rccnt[r];
+        li = (uint32_t)(gb + a.rcoff[r]);
+      }
+      if (WRITE) load_slot(tup + t * TW, a.vt, r, a.S);
     }
-    if (WRITE) load_slot(tup + t * TW, a.vt, r, a.S);
   }
   int T;
   const int pre = wave_exclusive_scan<int>(n, T);
@@ -1674,7 +1687,10 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
     if (t < nrow) {
       rc = a.rowcnt[r];
       rb = a.rowbsum[r];
-      if (rc) nvp = a.meta[p].nv;
+      if (ROOT)
+        nvp = uv;  // the child's tuple starts with the vertex
+      else if (rc)
+        nvp = a.meta[p].nv;
     }
     // the wave's base: its tile's offset plus the tile's earlier waves
     const int64_t w0 = (int64_t)blockIdx.x & ~(int64_t)63;
@@ -1692,8 +1708,9 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
     // graph boundaries (rows are graph-major): the neighbours' graphs
     int gprev = __shfl(g, t > 0 ? t - 1 : 0, kRowWave);
     int gnext = __shfl(g, t < kRowWave - 1 ? t + 1 : t, kRowWave);
-    if (t == 0) gprev = r > 0 ? a.rgraph[r - 1] : -1;
-    if (t == nrow - 1) gnext = r + 1 < a.n_rows ? a.rgraph[r + 1] : -1;
+    const int32_t *rg = ROOT ? a.vgraph : a.rgraph;
+    if (t == 0) gprev = r > 0 ? rg[r - 1] : -1;
+    if (t == nrow - 1) gnext = r + 1 < a.n_rows ? rg[r + 1] : -1;
     if (t < nrow) {
       if (gprev != g) {
         a.cgstart[g] = pos0;
@@ -1728,7 +1745,7 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
       v[u] = 0;
       if (jb0 + u * kRowWave + t < T) {
         int key, w;
-        cand_decode(false, c[u].x, key, w);
+        cand_decode(ROOT, c[u].x, key, w);
         v[u] = a.sup[(uint64_t)s_p[kk[u]] * (uint64_t)a.K + (uint64_t)key];
       }
     }
@@ -1751,19 +1768,24 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
           const long long pos = s_pos[k] + (long long)kc + (long long)__popcll(below);
           if (pos < a.ccap) {
             int key, w;
-            cand_decode(false, c[u].x, key, w);
+            cand_decode(ROOT, c[u].x, key, w);
             a.cpid[pos] = -v[u] - 1;
             a.crgraph[pos] = s_g[k];
-            const uint32_t *src = tup + k * TW;
             uint32_t *dst = reinterpret_cast<uint32_t *>(a.cvt + pos * a.CS);
             const int nvp = s_nvp[k];
-            for (int i = 0; i < CW; ++i) {
-              uint32_t word = i < TW ? src[i] : 0u;
-              if (w >= 0 && (nvp >> 2) == i) {
-                const int sh = (nvp & 3) * 8;
-                word = (word & ~(0xFFu << sh)) | ((uint32_t)w << sh);
+            if (ROOT) {  // (u, w)
+              dst[0] = (uint32_t)nvp | ((uint32_t)w << 8);
+              for (int i = 1; i < CW; ++i) dst[i] = 0;
+            } else {
+              const uint32_t *src = tup + k * TW;
+              for (int i = 0; i < CW; ++i) {
+                uint32_t word = i < TW ? src[i] : 0u;
+                if (w >= 0 && (nvp >> 2) == i) {
+                  const int sh = (nvp & 3) * 8;
+                  word = (word & ~(0xFFu << sh)) | ((uint32_t)w << sh);
+                }
+                dst[i] = word;
               }
-              dst[i] = word;
             }
             if (a.next_lists) a.nrcoff[pos] = (uint32_t)(s_ent[k] + kb + (ex - exf));
           }
@@ -1831,12 +1853,14 @@ __global__ void __launch_bounds__(kScanTile) k_rtop(LevelArgs a, int n) {
 // (graphs without rows at this level: none), the next level's list flag
 // (0: lists at absolute offsets, -1: none), the big-graph list (wave-
 // aggregated append; its order does not matter) and the largest graph.
+template <bool ROOT>
 __global__ void __launch_bounds__(256) k_gfix(LevelArgs a, int n_graphs) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   long long c = 0;
   bool lists = false;
   if (g < n_graphs) {
-    if (a.gcount[g] > 0) {
+    const bool rows = ROOT ? a.gvbase[g + 1] > a.gvbase[g] : a.gcount[g] > 0;
+    if (rows) {
       const long long s0 = a.cgstart[g], s1 = a.cgcount[g];
       const unsigned long long e0 = a.gbsum[g], e1 = (unsigned long long)a.ngcbase[g];
       c = s1 > a.ccap ? 0 : s1 - s0;  // past the allocation: not written; the host sees the total
@@ -1961,11 +1985,19 @@ hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t
     const int64_t nw = (a.n_rows + kRowWave - 1) / kRowWave;
     if (nw <= 0) return hipSuccess;
     if ((nw + 63) / 64 > a.n_wtiles) return hipErrorInvalidValue;  // tile sums not sized (or not zeroed) for nw
-    hipLaunchKernelGGL(k_rows<false>, dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
+    const size_t tup_lds = root ? 0 : sizeof(uint32_t) * kRowWave * (a.S >> 2);
+    if (root)
+      hipLaunchKernelGGL((k_rows<true, false>), dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_rows<false, false>), dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
     hipLaunchKernelGGL(k_rtop, dim3(1), dim3(kScanTile), 0, s, a, (int)((nw + 63) / 64));
-    hipLaunchKernelGGL(k_rows<true>, dim3((unsigned)nw), dim3(kRowWave), sizeof(uint32_t) * kRowWave * (a.S >> 2), s,
-                       a);
-    hipLaunchKernelGGL(k_gfix, dim3((unsigned)((n_graphs + 255) / 256)), dim3(256), 0, s, a, n_graphs);
+    if (root) {
+      hipLaunchKernelGGL((k_rows<true, true>), dim3((unsigned)nw), dim3(kRowWave), tup_lds, s, a);
+      hipLaunchKernelGGL(k_gfix<true>, dim3((unsigned)((n_graphs + 255) / 256)), dim3(256), 0, s, a, n_graphs);
+    } else {
+      hipLaunchKernelGGL((k_rows<false, true>), dim3((unsigned)nw), dim3(kRowWave), tup_lds, s, a);
+      hipLaunchKernelGGL(k_gfix<false>, dim3((unsigned)((n_graphs + 255) / 256)), dim3(256), 0, s, a, n_graphs);
+    }
     return hipGetLastError();
   }
   const size_t lds = level_lds_bytes(a, false);
@@ -1979,7 +2011,7 @@ hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t
     hipLaunchKernelGGL((k_children<false, false, false>), dim3(n_graphs), dim3(a.block), lds, s, a);
     if (big) hipLaunchKernelGGL((k_children<false, false, true>), dim3(a.big_grid), dim3(b.block), lds_big, s, b);
   }
-  if (n_graphs <= kScanTile * kScanSinglePer) {
+  if (n_graphs <= kScanSingleGraphs) {
     hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanTile), 0, s, a, n_graphs);
   } else {
     const unsigned tiles = (unsigned)((n_graphs + kScanTile - 1) / kScanTile);

@@ -13,6 +13,7 @@ namespace gm {
 // 65535 (uint16 counts): a row extends by at most 32 * 256 edges.
 constexpr int kMaxBlock = 256;
 constexpr int kScanTileGraphs = 1024;  // graphs per tile of the placement scan
+constexpr int kScanSingleGraphs = 16 * kScanTileGraphs;  // up to this many graphs: one-workgroup placement scan
 constexpr int kRowWaveRows = 64;       // rows per workgroup of the row-parallel children passes
 constexpr int kTileStride = 16;        // 64-bit words between tile sums: one cache line each (atomics on a line serialise)
 
@@ -35,6 +36,7 @@ struct LevelArgs {
   const uint32_t *adj;
   const uint8_t *vlab;
   const int32_t *gjob;
+  const int32_t *vgraph;  // graph of each vertex
   // current level: rows of graph g are [gstart[g], gstart[g] + gcount[g])
   const int64_t *gstart;
   const int64_t *gcount;

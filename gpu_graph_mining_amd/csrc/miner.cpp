@@ -144,19 +144,21 @@ void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk
   auto vmap = [&](int32_t l) { return (int32_t)(std::lower_bound(vls.begin(), vls.end(), l) - vls.begin()); };
   auto emap = [&](int32_t l) { return (int32_t)(std::lower_bound(els.begin(), els.end(), l) - els.begin()); };
 
-  std::vector<int32_t> &gvbase = pk.gvbase, &vadj = pk.vadj, &gjob = pk.gjob;
+  std::vector<int32_t> &gvbase = pk.gvbase, &vadj = pk.vadj, &gjob = pk.gjob, &vgraph = pk.vgraph;
   std::vector<uint32_t> &adj = pk.adj;
   std::vector<uint8_t> &vlab = pk.vlab;
   pk.max_graph_v = pk.max_graph_adj = 0;
   gvbase.clear();
   vadj.clear();
   gjob.clear();
+  vgraph.clear();
   adj.clear();
   vlab.clear();
   gvbase.reserve(ng + 1);
   vadj.reserve(nv + 1);
   adj.reserve(2 * ne);
   vlab.reserve(nv);
+  vgraph.reserve(nv);
   gjob.reserve(ng);
   out.job_first_graph.assign(1, 0);
   std::vector<int32_t> deg, fill, dl;
@@ -183,6 +185,7 @@ void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk
       for (int32_t v = 0; v < n; ++v) {
         dl[v] = vmap(db->vlabel[v0 + v]);
         vlab.push_back((uint8_t)dl[v]);
+        vgraph.push_back((int32_t)gjob.size() - 1);
       }
       // adjacency in input edge order, both directions, as Graph::read
       // builds it (src/globals/graph_types.cpp:94-107)
@@ -229,8 +232,10 @@ void upload_db(const PackedDB &pk, DeviceDB &out, hipStream_t s) {
   upload(out.adj, pk.adj, s);
   upload(out.vlab, pk.vlab, s);
   upload(out.gjob, pk.gjob, s);
+  upload(out.vgraph, pk.vgraph, s);
   hip_check(hipStreamSynchronize(s), "upload database");
-  out.bytes = 4 * pk.gvbase.size() + 4 * pk.vadj.size() + 4 * pk.adj.size() + pk.vlab.size() + 4 * pk.gjob.size();
+  out.bytes = 4 * pk.gvbase.size() + 4 * pk.vadj.size() + 4 * pk.adj.size() + pk.vlab.size() + 4 * pk.gjob.size() +
+              4 * pk.vgraph.size();
 }
 
 namespace {
@@ -561,6 +566,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   a.adj = db.adj.as<uint32_t>();
   a.vlab = db.vlab.as<uint8_t>();
   a.gjob = db.gjob.as<int32_t>();
+  a.vgraph = db.vgraph.as<int32_t>();
   a.NEL = NEL;
   a.NVL = NVL;
   // LDS: stage the largest graph's adjacency when it fits 16 KB (larger
@@ -765,9 +771,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     }
     // per-wave totals and 64-wave tile sums of the row-parallel children
     // passes (k_reduce zeroes the tile sums)
+    // (the root level only for many graphs: up to kScanSingleGraphs the
+    // per-graph passes place the children with one single-workgroup scan
+    // and are as fast)
     a.n_wtiles = 0;
-    if (!root && a.lists_ready) {
-      const int64_t nw = (n_rows + kRowWaveRows - 1) / kRowWaveRows, nt = (nw + 63) / 64;
+    if (root ? (a.use_lists != 0 && G > kScanSingleGraphs) : a.lists_ready != 0) {
+      const int64_t nw = ((root ? db.n_vertices : n_rows) + kRowWaveRows - 1) / kRowWaveRows, nt = (nw + 63) / 64;
       const int64_t tw = nt * kTileStride;  // one cache line per tile sum
       auto *wt = static_cast<unsigned long long *>(grow(w.wtot, sizeof(unsigned long long) * 2 * (nw + tw), s));
       a.wrows = wt;
@@ -1022,10 +1031,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.nbig_list = w.big[nxt].as<int32_t>();
       a.nnbig = w.nbig.as<unsigned long long>() + nxt;
       a.nrcoff = w.rcoff[nxt].as<uint32_t>();
-      // where k_expand listed every row's candidates the children are
-      // counted, placed and written over waves of rows (a large graph
-      // spreads over many waves), else per graph
-      a.rows_mode = (!root && a.lists_ready && a.n_wtiles > 0) ? 1 : 0;
+      // where every row's candidates are listed (k_expand at later levels,
+      // the root pass at the root) the children are counted, placed and
+      // written over waves of rows (a large graph spreads over many waves),
+      // else per graph
+      a.rows_mode = a.n_wtiles > 0 ? 1 : 0;
+      if (root && a.rows_mode) a.n_rows = db.n_vertices;  // the root level's rows are the vertices
       // the selected keys point at their child ids, then the write pass
       // (launched back to back: the device does not wait on the host between)
       hip_check(launch_upload_scatter(du, su, 0, w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(su + keys_off),

@@ -68,7 +68,7 @@ struct DeviceDB {
   int32_t NVL = 0, NEL = 0;
   std::vector<int32_t> vl_values, el_values;  // dense label -> original
   std::vector<int32_t> job_first_graph;       // n_jobs + 1
-  DevBuf gvbase, vadj, adj, vlab, gjob;
+  DevBuf gvbase, vadj, adj, vlab, gjob, vgraph;
   size_t bytes = 0;
   // mining state; one mining call at a time per database
   std::unique_ptr<Workspace> ws;
@@ -78,7 +78,7 @@ struct DeviceDB {
 // Host image of the device layout (device.hpp).
 struct PackedDB {
   int32_t max_graph_v = 0, max_graph_adj = 0;
-  std::vector<int32_t> gvbase, vadj, gjob;
+  std::vector<int32_t> gvbase, vadj, gjob, vgraph;
   std::vector<uint32_t> adj;
   std::vector<uint8_t> vlab;
 };

@@ -73,6 +73,19 @@ def test_gpu_batch_equals_individual_runs(golden):
     assert st["n_patterns"] == sum(golden[n]["n_patterns"] for n in names)
 
 
+def test_gpu_many_graphs_equal_oracle():
+    """20,000 graphs (more than one placement-scan workgroup covers): the
+    root level's children are counted and written over vertex waves too;
+    patterns, supports and embedding counts equal the oracle's."""
+    db, _ = wl.config0(seed=5, n_graphs=20000)
+    ms = 200
+    pats, st = gm.mine(db, ms, collect_proj=True)
+    opats, ost = po.mine_arrays(db.arrays(), ms)
+    assert len(opats) > 4000
+    assert _by_code(pats) == _by_code(opats)
+    assert (st["e_child"], st["e_root"], st["e_cand"]) == (ost["e_child"], ost["e_root"], ost["e_cand"])
+
+
 def test_gpu_deterministic_across_runs(golden):
     case = golden["config1_1k"]
     db = case_db(case)
