@@ -8,6 +8,7 @@
 //   adj[a]         packed adjacency entry: bits 0-7 neighbour's local id,
 //                  8-15 dense edge label, 16-23 neighbour's dense label
 //   gjob[g]        job index of graph g (0 unless several jobs are batched)
+//   vgraph[v]      graph of global vertex v
 // Local vertex ids fit a byte, so a graph has at most 256 vertices.  This is
 // the padded-CSR graph_database_cuda of the reference
 // (src/globals/cuda_graph_types.hpp:16-166) without the padding and with the
