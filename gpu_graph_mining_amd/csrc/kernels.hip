@@ -1647,6 +1647,7 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
   __shared__ int32_t s_nvp[kRowWave];
   __shared__ long long s_pos[kRowWave];
   __shared__ unsigned long long s_ent[kRowWave];
+  __shared__ int32_t s_ms[kRowWave];  // count by support: the row's job's min support
   extern __shared__ uint32_t tup[];  // WRITE: row t's tuple at tup[t * TW, + TW)
   const int t = threadIdx.x;
   const int64_t r0 = (int64_t)blockIdx.x * kRowWave;
@@ -1680,6 +1681,10 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
   s_pre[t] = pre;
   s_li[t] = li;
   s_p[t] = p;
+  // a speculative level's count runs before the child ids are scattered:
+  // every key reaching its job's min support becomes a child
+  const bool by_support = !WRITE && a.count_by_support;
+  if (by_support) s_ms[t] = n ? a.minsup_job[ROOT ? p : a.meta[p].job] : 0;
   if (WRITE) {
     // the row's first child row and first list entry
     uint32_t rc = 0, rb = 0;
@@ -1753,7 +1758,7 @@ __global__ void __launch_bounds__(kRowWave) k_rows(LevelArgs a) {
     for (int u = 0; u < kRowUnroll; ++u) {
       const int jb = jb0 + u * kRowWave;
       const bool live = jb + t < T;
-      const bool sel = live && v[u] < 0;
+      const bool sel = live && (by_support ? v[u] >= s_ms[kk[u]] : v[u] < 0);
       const unsigned long long mask = __ballot(sel);
       const uint32_t x = sel ? c[u].y : 0u;
       uint32_t xt;
@@ -1954,6 +1959,29 @@ hipError_t launch_reduce(const LevelArgs &a, bool root, hipStream_t s) {
 
 static unsigned copy_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 2048)); }
 
+// One word to mapped host memory, released at system scope after every
+// earlier kernel of the stream: the host polls it instead of synchronising
+// the stream, so work queued after it keeps running.
+__global__ void k_signal(unsigned long long *flag, unsigned long long v) {
+  __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t launch_signal(unsigned long long *flag, unsigned long long v, hipStream_t s) {
+  hipLaunchKernelGGL(k_signal, dim3(1), dim3(1), 0, s, flag, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_count(const LevelArgs &a, bool root, hipStream_t s) {
+  const int64_t nw = (a.n_rows + kRowWave - 1) / kRowWave;
+  if (nw <= 0) return hipSuccess;
+  if ((nw + 63) / 64 > a.n_wtiles) return hipErrorInvalidValue;  // tile sums not sized (or not zeroed) for nw
+  if (root)
+    hipLaunchKernelGGL((k_rows<true, false>), dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_rows<false, false>), dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s) {
   if (bytes % 8) return hipErrorInvalidValue;
   const int64_t n = (int64_t)(bytes / 8);
@@ -1986,10 +2014,10 @@ hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t
     if (nw <= 0) return hipSuccess;
     if ((nw + 63) / 64 > a.n_wtiles) return hipErrorInvalidValue;  // tile sums not sized (or not zeroed) for nw
     const size_t tup_lds = root ? 0 : sizeof(uint32_t) * kRowWave * (a.S >> 2);
-    if (root)
-      hipLaunchKernelGGL((k_rows<true, false>), dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_rows<false, false>), dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
+    if (!a.count_done) {
+      const hipError_t e = launch_count(a, root, s);
+      if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_rtop, dim3(1), dim3(kScanTile), 0, s, a, (int)((nw + 63) / 64));
     if (root) {
       hipLaunchKernelGGL((k_rows<true, true>), dim3((unsigned)nw), dim3(kRowWave), tup_lds, s, a);

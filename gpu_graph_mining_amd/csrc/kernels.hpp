@@ -133,6 +133,10 @@ struct LevelArgs {
   // k_reduce)
   int32_t rows_mode;
   int32_t n_wtiles;
+  // speculative levels: the count pass runs before the child ids are
+  // scattered, selecting keys by support >= the job's min support
+  // (count_by_support), and launch_write then skips it (count_done)
+  int32_t count_by_support, count_done;
   unsigned long long *wrows, *wents, *trows, *tents;
   // placement scan: per tile of 1024 graphs (row-parallel passes: waves),
   // sums then exclusive offsets
@@ -151,6 +155,11 @@ hipError_t launch_expand(const LevelArgs &a, hipStream_t s);
 // Copy `bytes` (a multiple of 8) between device memory and mapped pinned
 // host memory with a kernel on stream s.
 hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s);
+// Write v to *flag (mapped pinned host memory) once the stream's earlier
+// work is done and visible to the host.
+hipError_t launch_signal(unsigned long long *flag, unsigned long long v, hipStream_t s);
+// The row-parallel count pass alone (k_rows<*, false>), ahead of the write.
+hipError_t launch_count(const LevelArgs &a, bool root, hipStream_t s);
 // Copy the children's upload (bytes, a multiple of 8) and set
 // sup[keys[i]] = vals[i] for i < C, in one kernel.
 hipError_t launch_upload_scatter(void *dst, const void *src, size_t bytes, int32_t *sup, const uint64_t *keys,

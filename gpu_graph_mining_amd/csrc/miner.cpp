@@ -785,6 +785,19 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.tents = wt + 2 * nw + tw;
       a.n_wtiles = (int32_t)nt;
     }
+    // Speculative levels (every frequent key becomes a child): the
+    // row-parallel count pass needs no child ids, so it is launched as soon
+    // as the frequent keys are on the host and runs beside their sort and
+    // decode (the same condition as `spec` below, with the bound on the
+    // children's vertices)
+    a.count_done = 0;
+    const bool early_count = !root && a.n_wtiles > 0 && !opt.queries &&
+                             (opt.speculate == 2 || (opt.speculate == 0 && dead_share <= 0.3)) &&
+                             !(opt.max_edges > 0 && L.len + 1 > opt.max_edges) && L.max_nv + 1 <= kMaxPatV;
+    if (early_count) {
+      a.rowcnt = static_cast<uint16_t *>(grow(w.rowcnt, sizeof(uint16_t) * std::max<int64_t>(list_rows, 1), s));
+      a.rowbsum = static_cast<uint32_t *>(grow(w.rowbsum, sizeof(uint32_t) * std::max<int64_t>(list_rows, 1), s));
+    }
     hip_check(launch_reduce(a, root, s), "k_reduce");
     // counters and (most often all of) the frequent keys in one copy
     const uint32_t guess = std::min<uint32_t>(fcap, std::max<uint32_t>(4096, 2 * last_fcount));
@@ -792,7 +805,37 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     char *hbuf = static_cast<char *>(w.down.ensure(head));
     hip_check(launch_copy8(w.down.dev, dctr, head, s), "download counters");
     Timer t_sync;
-    hip_check(hipStreamSynchronize(s), "k_support");
+    // a short frequent-key list (the last level's fitted the first download
+    // twice over) is all but certain to come down in this one copy: the
+    // count pass is queued right behind it and the host polls a completion
+    // word instead of waiting for the stream; longer lists may need a second
+    // copy, which would wait for the count, so their count follows the copies
+    const bool queued_count = early_count && 2 * (uint64_t)last_fcount < guess;
+    if (queued_count) {
+      if (w.seq == 0) {
+        w.flag.ensure(64);
+        std::memset(w.flag.p, 0, 64);
+      }
+      const unsigned long long want = ++w.seq;
+      hip_check(launch_signal(static_cast<unsigned long long *>(w.flag.dev), want, s), "signal");
+      a.count_by_support = 1;
+      hip_check(launch_count(a, root, s), "k_count");
+      a.count_by_support = 0;
+      a.count_done = 1;
+      const unsigned long long *fl = static_cast<const unsigned long long *>(w.flag.p);
+      unsigned spins = 0;
+      while (__atomic_load_n(fl, __ATOMIC_ACQUIRE) != want) {
+        if ((++spins & 1023) == 0 && t_sync.sec() > 2.0) {  // surfaces a device error, if any
+          hip_check(hipStreamSynchronize(s), "k_support");
+          break;
+        }
+#if defined(__x86_64__)
+        __builtin_ia32_pause();
+#endif
+      }
+    } else {
+      hip_check(hipStreamSynchronize(s), "k_support");
+    }
     const double us_sync = t_sync.sec() * 1e6;
     Timer t_host;  // host work between the level's download and its upload
     Counters cnt;
@@ -843,6 +886,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         hip_check(hipStreamSynchronize(s), "download frequent keys");
         std::memcpy(frall.data() + have, h2, rest);
         src = frall.data();
+      }
+      if (early_count && !a.count_done) {
+        a.count_by_support = 1;
+        hip_check(launch_count(a, root, s), "k_count");
+        a.count_by_support = 0;
+        a.count_done = 1;
       }
       sort_freq(src, F, (uint64_t)nkeys, fr, frtmp);
     }
@@ -914,6 +963,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       for (int64_t i = 0; i < F; ++i) max_nv_exact = std::max(max_nv_exact, (int)knv[i]);
     const bool spec = want_spec && !root && !opt.queries && !past_max &&
                       (early_decode ? max_nv_exact : max_nv_bound) <= kMaxPatV;
+    if (a.count_done && !spec) throw Error(GM_E_DEVICE, "internal: early count on a level that is not speculative");
     const bool late_decode = spec && !early_decode;
     if (!spec) {
       if (!early_decode) Pool::get().run(F, threads, decode, 4096);

@@ -55,6 +55,8 @@ struct Workspace {
   DevBuf scan_tiles;  // placement scan: per-tile sums and offsets
   DevBuf wtot;        // row-parallel children passes: per-wave child rows, list entries
   HostBuf up, down;
+  HostBuf flag;                // the level download's completion word (k_signal)
+  unsigned long long seq = 0;  // its last value
   // the replica counters hold zeros outside a level step (k_reduce clears
   // what it reads); false after a call that ended early
   bool replicas_zero = false;
