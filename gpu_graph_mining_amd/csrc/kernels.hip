@@ -1,19 +1,22 @@
 // kernels.hip -- HIP kernels of the level step (gfx950, wave64).
 //
-// One workgroup owns one database graph for the whole level step, so every
-// support question ("how many distinct graphs have an embedding of P with
-// extension x") is answered by deduplicating (pattern, key) pairs inside the
-// workgroup (an LDS hash table) and adding 1 per distinct pair to a global
-// counter.  Two kernels per level:
-//   k_support  enumerates every extension candidate of every row once:
-//              distinct-graph support, embedding count and child candidate
-//              bound per (pattern, key), plus each row's candidate list;
-//   k_children the child rows (pattern id + tuple) of the selected keys,
-//              read from the candidate lists (or, for a graph whose lists
-//              were not kept, from a fresh enumeration): a count pass, a
-//              tiled scan (k_scan_*) and a write pass, which also lay
-//              out the next level's candidate lists.
-// The first level runs the same two kernels over 1-edge keys.
+// Support: one workgroup (at the root, one wave) owns one database graph,
+// so every support question ("how many distinct graphs have an embedding of
+// P with extension x") is answered by deduplicating (pattern, key) pairs
+// inside it (an LDS hash table) and adding 1 per distinct pair to a global
+// counter.  Per level:
+//   k_expand   lists every row's extension candidates with their child
+//              bounds, row-parallel (later levels);
+//   k_support  deduplicates each graph's candidates (from the lists, or by
+//              enumerating rows where no lists are kept): distinct-graph
+//              support and embedding count per (pattern, key); the root
+//              level runs k_root_wave over 1-edge keys instead;
+//   k_reduce   sums the counter replicas and lists the frequent keys;
+//   children   the child rows (pattern id + tuple) of the selected keys:
+//              a count pass, a placement scan and a write pass that also
+//              lays out the next level's candidate lists -- row-parallel
+//              (k_rows, k_rtop, k_gfix) where every candidate is listed,
+//              per graph (k_children, k_scan_*) otherwise.
 //
 // Extension rules (pattern side precomputed in PatMeta, graph side read
 // from the packed adjacency):

@@ -12,13 +12,18 @@
 // reports (src/gspan/gspan.cpp:172-274); only the emission order differs.
 //
 // Level step for patterns P_1..P_n with embeddings in graph-major rows:
-//   1. k_support   : distinct-graph support of every (pattern, extension key)
-//                    and the list of keys whose support reached min support;
+//   1. k_expand    : every row's extension candidates, listed (row-parallel);
+//      k_support   : distinct-graph support and embedding count of every
+//                    (pattern, extension key), one workgroup per graph;
+//      k_reduce    : the keys whose support reached their job's min support;
 //   2. host        : decode each frequent key into a child code, keep the
-//                    minimal ones (canonical.cpp), number the children;
-//   3. k_write     : child rows of the chosen keys, each graph's rows
-//                    reserved from a level cursor (sized exactly by the
-//                    embedding counts k_support gathered).
+//                    minimal ones (canonical.cpp; on speculative levels every
+//                    frequent key becomes a child and the check runs while
+//                    the device writes), number the children in key order;
+//   3. children    : count, place and write the child rows (row-parallel
+//                    passes where every candidate is listed, per-graph passes
+//                    otherwise), sized exactly by the embedding counts
+//                    k_support gathered.
 #include "miner.hpp"
 
 #include <algorithm>
