@@ -1982,6 +1982,8 @@ hipError_t launch_count(const LevelArgs &a, bool root, hipStream_t s) {
     hipLaunchKernelGGL((k_rows<true, false>), dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
   else
     hipLaunchKernelGGL((k_rows<false, false>), dim3((unsigned)nw), dim3(kRowWave), 0, s, a);
+  // the placement scan needs only the count's tile sums
+  hipLaunchKernelGGL(k_rtop, dim3(1), dim3(kScanTile), 0, s, a, (int)((nw + 63) / 64));
   return hipGetLastError();
 }
 
@@ -2017,11 +2019,10 @@ hipError_t launch_write(const LevelArgs &a, int n_graphs, bool root, hipStream_t
     if (nw <= 0) return hipSuccess;
     if ((nw + 63) / 64 > a.n_wtiles) return hipErrorInvalidValue;  // tile sums not sized (or not zeroed) for nw
     const size_t tup_lds = root ? 0 : sizeof(uint32_t) * kRowWave * (a.S >> 2);
-    if (!a.count_done) {
+    if (!a.count_done) {  // count pass and placement scan
       const hipError_t e = launch_count(a, root, s);
       if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_rtop, dim3(1), dim3(kScanTile), 0, s, a, (int)((nw + 63) / 64));
     if (root) {
       hipLaunchKernelGGL((k_rows<true, true>), dim3((unsigned)nw), dim3(kRowWave), tup_lds, s, a);
       hipLaunchKernelGGL(k_gfix<true>, dim3((unsigned)((n_graphs + 255) / 256)), dim3(256), 0, s, a, n_graphs);

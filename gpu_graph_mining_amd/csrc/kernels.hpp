@@ -158,7 +158,8 @@ hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s)
 // Write v to *flag (mapped pinned host memory) once the stream's earlier
 // work is done and visible to the host.
 hipError_t launch_signal(unsigned long long *flag, unsigned long long v, hipStream_t s);
-// The row-parallel count pass alone (k_rows<*, false>), ahead of the write.
+// The row-parallel count pass (k_rows<*, false>) and its placement scan
+// (k_rtop), ahead of the write.
 hipError_t launch_count(const LevelArgs &a, bool root, hipStream_t s);
 // Copy the children's upload (bytes, a multiple of 8) and set
 // sup[keys[i]] = vals[i] for i < C, in one kernel.

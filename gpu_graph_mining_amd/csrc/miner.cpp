@@ -802,6 +802,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     if (early_count) {
       a.rowcnt = static_cast<uint16_t *>(grow(w.rowcnt, sizeof(uint16_t) * std::max<int64_t>(list_rows, 1), s));
       a.rowbsum = static_cast<uint32_t *>(grow(w.rowbsum, sizeof(uint32_t) * std::max<int64_t>(list_rows, 1), s));
+      a.nnbig = w.nbig.as<unsigned long long>() + (cur ^ 1);  // the placement scan zeroes the next level's count
     }
     hip_check(launch_reduce(a, root, s), "k_reduce");
     // counters and (most often all of) the frequent keys in one copy
