@@ -1129,8 +1129,21 @@ __global__ void __launch_bounds__(256) k_reduce(LevelArgs a) {
     a.tents[(int64_t)i * kTileStride] = 0;
   }
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nkeys; i += (int64_t)gridDim.x * blockDim.x) {
+    // eight replicas' loads in flight before any of them is cleared (a
+    // load after a store to the same array waits for it)
     unsigned long long v = 0;
-    for (int r = 0; r < a.rep; ++r) {
+    int r = 0;
+    for (; r + 8 <= a.rep; r += 8) {
+      unsigned long long x[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) x[q] = a.acc_rep[(int64_t)(r + q) * a.nkeys + i];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        v += x[q];
+        a.acc_rep[(int64_t)(r + q) * a.nkeys + i] = 0;
+      }
+    }
+    for (; r < a.rep; ++r) {
       const int64_t j = (int64_t)r * a.nkeys + i;
       v += a.acc_rep[j];
       a.acc_rep[j] = 0;
