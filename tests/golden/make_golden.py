@@ -13,8 +13,14 @@ support-recording sink of oracle/ref_harness.cpp) on:
 Each case stores the sorted "code<TAB>support" lines the reference emits
 (code in its -1 convention) in full when small, else their sha256 and count.
 
+The full-size planted-pattern case (100,000 transactions, 1 %) also records
+the oracle's embedding and byte counts (E_child, E_cand, E_root, B_ext,
+B_sup, B_child): the GPU tests check E_child / E_cand against them and the
+benchmark prices its roofline with the byte counts.
+
 Usage (needs /root/reference; run from the repo root):
     make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+    python tests/golden/make_golden.py --only config2   # add / replace cases
 """
 from __future__ import annotations
 
@@ -31,6 +37,7 @@ sys.path.insert(0, str(ROOT))
 
 import gpu_graph_mining_amd as gm  # noqa: E402
 from gpu_graph_mining_amd import workloads as wl  # noqa: E402
+from oracle import py_oracle as po  # noqa: E402
 
 REF = ROOT / "oracle" / "_ref" / "gspan_ref"
 FULL_LIMIT = 1200  # store listings up to this many lines in full
@@ -53,9 +60,45 @@ def case_record(name: str, lines: list, **meta) -> dict:
     return rec
 
 
+ORACLE_KEYS = ("e_child", "e_cand", "e_root", "b_ext", "b_sup", "b_child")
+
+
+def oracle_counts(db, minsup: int) -> dict:
+    """Embedding and byte counts of the whole run from the pinned oracle."""
+    _, st = po.mine_arrays(db.arrays(), minsup, collect=False)
+    return {k: int(st[k]) for k in ORACLE_KEYS}
+
+
+def big_cases():
+    """Cases built on demand with --only (minutes of reference time each)."""
+    db, ms = wl.config2()
+    yield "config2", db, ms, {"gen": "config2", "seed": 3, "n_graphs": 100000}
+
+
+def add_only(names: set) -> None:
+    path = HERE / "golden.json"
+    doc = json.loads(path.read_text())
+    cases = [c for c in doc["cases"] if c["name"] not in names]
+    with tempfile.TemporaryDirectory() as td:
+        for name, db, ms, spec in big_cases():
+            if name not in names:
+                continue
+            p = Path(td) / f"{name}.txt"
+            db.write_txt(p)
+            rec = case_record(name, run_ref(p, "txt", ms), db=spec, db_sha256=db.sha256(), minsup=ms)
+            rec["oracle"] = oracle_counts(db, ms)
+            cases.append(rec)
+            print(name, rec["n_patterns"], rec["oracle"], flush=True)
+    doc["cases"] = cases
+    path.write_text(json.dumps(doc, indent=0))
+
+
 def main() -> None:
     if not REF.exists():
         raise SystemExit("build the reference first: make -f oracle/Makefile.ref")
+    only = set(sys.argv[sys.argv.index("--only") + 1].split(",")) if "--only" in sys.argv else None
+    if only:
+        return add_only(only)
     cases = []
     chem = HERE / "Chemical_340.txt"
     for s in (170, 68, 34, 17, 10):

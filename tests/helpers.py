@@ -45,6 +45,8 @@ def case_db(case: dict) -> gm.GraphDB:
         db, _ = wl.config0(seed=spec["seed"], n_graphs=spec["n_graphs"])
     elif g == "config1":
         db, _ = wl.config1(seed=spec["seed"], n_graphs=spec["n_graphs"])
+    elif g == "config2":
+        db, _ = wl.config2(seed=spec["seed"], n_graphs=spec["n_graphs"])
     elif g == "config4":
         db = wl.config4(seed=spec["seed"], n_graphs=spec["n_graphs"])
     elif g == "config3":

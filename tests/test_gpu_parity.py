@@ -18,6 +18,8 @@ def _by_code(patterns):
 
 def test_gpu_matches_reference_on_every_golden_case(golden):
     for name, case in golden.items():
+        if "oracle" in case:  # the full-size workloads have tests of their own
+            continue
         print("case", name, flush=True)
         db = case_db(case)
         pats, st = gm.mine(db, case["minsup"])
