@@ -1,13 +1,18 @@
 """Benchmark of the GPU gSpan miner on one MI355X.
 
 A step is one complete mining run (every level step, every pattern) of the
-configuration-1 workload -- 10,000 planted-pattern transactions (seed 2),
-absolute min support 500 (5 %) -- with the database already resident in HBM.
+largest single-GPU workload -- 100,000 planted-pattern transactions (seed 3,
+Poisson(30) edges, 50 seed patterns, 5 vertex / 5 edge labels), absolute
+min support 1,000 (1 %) -- with the database already resident in HBM.
 The line reports extended embeddings/s (E_child: embedding rows of every
 reported pattern with >= 2 edges, per step), frequent patterns/s and the
-per-step mining seconds, plus the roofline of the dominant kernel (the
-extension + support kernel) and the CPU baseline (the oracle restatement of
-the reference's sequential gSpan, one core, same input).
+per-step mining seconds, plus the roofline of the dominant stage (the
+extension + support kernels, timed with HIP events on the mining stream)
+and the CPU baseline (the oracle restatement of the reference's sequential
+gSpan, one core, on a bounded sample of the same workload).  Parity: the
+first run's pattern listing must equal the reference's own (sha256 of its
+sorted "code<TAB>support" lines in tests/golden/golden.json), and E_child /
+E_cand must equal the oracle's.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|4] [--no-cpu] [--jobs N]
 """
@@ -59,10 +64,27 @@ def _traffic(kernel: str, workload: str):
         return None
 
 
-def _prefix(arrays: dict, n: int) -> dict:
-    vb, eb = arrays["graph_vbegin"], arrays["graph_ebegin"]
-    return {"graph_vbegin": vb[:n + 1], "vlabels": arrays["vlabels"][:vb[n]], "graph_ebegin": eb[:n + 1],
-            "efrom": arrays["efrom"][:eb[n]], "eto": arrays["eto"][:eb[n]], "elabel": arrays["elabel"][:eb[n]]}
+def _golden_case(cfg: int):
+    """The reference's listing of the workload (tests/golden/golden.json):
+    sha256 and count of its sorted "code<TAB>support" lines, plus the
+    oracle's embedding and byte counts for the full-size workload."""
+    name = {1: "config1", 2: "config2"}.get(cfg)
+    if name is None:
+        return None
+    cases = json.loads((ROOT / "tests" / "golden" / "golden.json").read_text())["cases"]
+    return next((c for c in cases if c["name"] == name), None)
+
+
+def _code_str(code) -> str:
+    """A code in the sequential gSpan's -1 convention, as to_string(false)
+    prints it (src/globals/dfs_code.cpp:199-230)."""
+    return ";".join("(%d %d %d %d %d)" % (d[0], d[1], d[2] if i == 0 else -1, d[3], d[4] if d[0] < d[1] else -1)
+                    for i, d in enumerate(code))
+
+
+def _sha(lines) -> str:
+    import hashlib
+    return hashlib.sha256((("\n".join(lines) + "\n") if lines else "").encode()).hexdigest()
 
 
 def _cpu_model() -> str:
@@ -148,9 +170,12 @@ def _copy_calibration(mib: int) -> None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--cpu-root-stride", type=int, default=0,
+                    help="CPU baseline sample: grow every Nth 1-edge root of the oracle's pattern tree "
+                         "(0: by workload -- the whole run for config 1, every 6th root for config 2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--jobs", type=int, default=256,
                     help="batching: the first N jobs of the 256-job mix, batched vs one at a time (0: skip)")
@@ -177,20 +202,22 @@ def main() -> int:
     upload_s = time.perf_counter() - t0
     counts = db.counts()
     config.update({"n_vertices": counts["n_vertices"], "n_edges": counts["n_edges"]})
+    golden = _golden_case(args.config)
 
     # reference-checked first run (outside the timed region)
     pats, st0 = gm.mine(db, minsup)
     print(f"# first run: {st0['n_patterns']} patterns, E_child={st0['e_child']}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
-        gm.mine(db, minsup, count_only=True)
-    ext_s = 0.0
+        gm.mine(db, minsup, count_only=True, stage_events=True)
+    ext_s = ev_s = 0.0
     launches = 0
     canon_s = 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        _, st = gm.mine(db, minsup, count_only=True)
+        _, st = gm.mine(db, minsup, count_only=True, stage_events=True)
         ext_s += st["seconds_ext_kernel"]
+        ev_s += st["seconds_stage_events"]
         launches += st["ext_kernel_launches"]
         canon_s += st["seconds_canon"]
     elapsed = time.perf_counter() - t0
@@ -220,40 +247,56 @@ def main() -> int:
         "mine_seconds_with_upload": step_s + upload_s,
         "levels": st["n_levels"],
         "canon_seconds_per_step": canon_s / args.steps,
-        "ext_kernel_seconds_per_step": ext_s / args.steps,
+        "stage_seconds_per_step": ev_s / args.steps,
+        "stage_seconds_per_step_stamps": ext_s / args.steps,
         "split_passes": st["split_passes"],
         "max_graph_rows": st["max_graph_rows"],
         "max_table_fill": st["max_table_fill"],
     }
 
     roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-            "traffic": _traffic("k_support", config["workload"]), "kernel": "k_expand + k_support (extension + support stage of a level step)",
+            "traffic": _traffic("stage", config["workload"]),
+            "kernel": "k_expand + k_support (extension + support stage of a level step)",
             "launches_per_step": launches / args.steps,
-            "avg_launch_ms": (ext_s / launches * 1e3) if launches else None}
+            "avg_launch_ms": (ev_s / launches * 1e3) if launches else None}
+    # parity: the reference's listing (sha256 + count), the oracle's counts
+    ok = True
+    if golden is not None:
+        lines = sorted(f"{_code_str(c)}\t{s}" for c, s, _ in pats)
+        ok = len(lines) == golden["n_patterns"] and _sha(lines) == golden["sha256"]
+    oracle = golden.get("oracle") if golden else None
+    cpu = None
     if not args.no_cpu:
         from oracle import py_oracle as po
         arrays = db.arrays()
-        sample = "the whole workload"
-        if args.config == 2:  # bound the CPU leg to ~10-30 s: the first 10,000 transactions
-            n = 10000
-            arrays = _prefix(arrays, n)
-            sample = f"first {n} of 100000 transactions at the same 1 % support"
-        ms_cpu = minsup if args.config != 2 else max(1, minsup * 10000 // 100000)
-        opats, ost = po.mine_arrays(arrays, ms_cpu)
-        if args.config != 2:
-            got = sorted((po.normalize(c), s) for c, s, _ in pats)
-            want = sorted((c, s) for c, s, _ in opats)
-            line["parity"] = "identical" if (got == want and ost["e_child"] == e_child) else "MISMATCH"
-            # algorithmic bytes of the extension + support stage over the
-            # same patterns (formula in oracle/cpu_gspan.cpp)
-            b = ost["b_ext"] + ost["b_sup"]
-            per_step_ext = ext_s / args.steps
-            roof["algorithmic_bytes_per_step"] = b
-            roof["achieved"] = b / per_step_ext / 1e9
-            roof["frac"] = roof["achieved"] / HBM_PEAK_GBS
-        line["cpu_baseline"] = {"value": ost["e_child"] / ost["seconds"], "unit": "embeddings/s", "cores": 1,
-                                "kind": "port", "sample": sample, "seconds": ost["seconds"],
-                                "patterns": ost["n_patterns"], "cpu": _cpu_model()}
+        stride = args.cpu_root_stride or (6 if args.config == 2 else 1)
+        if stride == 1:
+            opats, ost = po.mine_arrays(arrays, minsup)
+            sample = "the whole workload"
+            if oracle is None:
+                got = sorted((po.normalize(c), s) for c, s, _ in pats)
+                want = sorted((c, s) for c, s, _ in opats)
+                ok = ok and got == want
+                oracle = ost
+        else:
+            ost = po.mine_sampled(arrays, minsup, stride)
+            sample = (f"every {stride}th 1-edge root of the oracle's pattern tree, each subtree in full, on the "
+                      f"whole database: {ost['n_patterns']} patterns, E_child {ost['e_child']}")
+        cpu = {"value": ost["e_child"] / ost["seconds"], "unit": "embeddings/s", "cores": 1, "kind": "port",
+               "sample": sample, "seconds": ost["seconds"], "patterns": ost["n_patterns"], "cpu": _cpu_model()}
+    if oracle is not None:
+        ok = ok and oracle["e_child"] == e_child and oracle["e_cand"] == st["e_cand"]
+        # algorithmic bytes of the extension + support stage over the whole
+        # run (formula in oracle/cpu_gspan.cpp), per launch = per level step
+        b = oracle["b_ext"] + oracle["b_sup"]
+        per_launch = b / (launches / args.steps)
+        roof["algorithmic_bytes_per_step"] = b
+        roof["algorithmic_bytes_per_launch"] = per_launch
+        roof["achieved"] = b / (ev_s / args.steps) / 1e9
+        roof["frac"] = roof["achieved"] / HBM_PEAK_GBS
+    line["parity"] = ("identical" if ok else "MISMATCH") if (golden is not None or oracle is not None) else None
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     line["roofline"] = roof
     if args.jobs:
         _batch_section(args.jobs, line)

@@ -53,7 +53,8 @@ class gm_cfg(C.Structure):
     _fields_ = [
         ("label_style", C.c_int32), ("max_edges", C.c_int32), ("canon_threads", C.c_int32),
         ("block_threads", C.c_int32), ("mem_budget_bytes", C.c_int64), ("collect_proj", C.c_int32),
-        ("candidate_lists", C.c_int32), ("path_pruning", C.c_int32), ("speculate", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("candidate_lists", C.c_int32), ("path_pruning", C.c_int32), ("speculate", C.c_int32), ("stage_events", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -64,7 +65,7 @@ class gm_stats(C.Structure):
         ("seconds_device", C.c_double), ("seconds_canon", C.c_double), ("seconds_ext_kernel", C.c_double),
         ("ext_kernel_launches", C.c_uint64), ("peak_device_bytes", C.c_uint64), ("split_passes", C.c_uint64),
         ("max_graph_rows", C.c_uint64), ("max_table_fill", C.c_uint64),
-        ("pruned", C.c_uint64), ("reserved", C.c_uint64 * 2),
+        ("pruned", C.c_uint64), ("seconds_stage_events", C.c_double), ("reserved", C.c_uint64 * 1),
     ]
 
     def as_dict(self) -> dict:
@@ -417,7 +418,7 @@ class graph_collecting_output(graph_output):
 
 # ------------------------------------------------------------------- mining
 def _make_cfg(label_style=GM_LABELS_FULL, max_edges=0, canon_threads=0, collect_proj=False, candidate_lists=True,
-              block_threads=0, path_pruning=True, speculate=0) -> gm_cfg:
+              block_threads=0, path_pruning=True, speculate=0, stage_events=False) -> gm_cfg:
     cfg = gm_cfg()
     load_library().gm_cfg_init(C.byref(cfg))
     cfg.label_style = label_style
@@ -428,6 +429,7 @@ def _make_cfg(label_style=GM_LABELS_FULL, max_edges=0, canon_threads=0, collect_
     cfg.block_threads = block_threads
     cfg.path_pruning = 0 if path_pruning else 1
     cfg.speculate = speculate
+    cfg.stage_events = 1 if stage_events else 0
     return cfg
 
 
@@ -435,7 +437,7 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
          collect_proj: bool = False, canon_threads: int = 0,
          on_pattern: Optional[Callable[[DFSCode, int, int], Optional[bool]]] = None, count_only: bool = False,
          candidate_lists: bool = True, block_threads: int = 0, path_pruning: bool = True,
-         speculate: int = 0):
+         speculate: int = 0, stage_events: bool = False):
     """Mine db on the GPU.  Returns (patterns, stats) where patterns is a
     list of (DFSCode, support, |Proj|) in emission order (empty when
     on_pattern is given; it then receives each pattern, and returning False
@@ -446,7 +448,9 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
     path_pruning=False keeps the candidates whose new edge forms an
     infrequent 2-edge path; speculate (0 adaptive, 1 never, 2 always)
     controls whether a level's children are written before their canonical
-    check (the results are the same either way)."""
+    check (the results are the same either way); stage_events=True times
+    every level's extension + support stage with HIP events on the mining
+    stream (stats["seconds_stage_events"])."""
     lib = load_library()
     out: list = []
 
@@ -459,7 +463,7 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
 
     ccb = PATTERN_CB() if count_only else PATTERN_CB(cb)  # PATTERN_CB() is a NULL pointer
     cfg = _make_cfg(label_style, max_edges, canon_threads, collect_proj, candidate_lists, block_threads,
-                    path_pruning, speculate)
+                    path_pruning, speculate, stage_events)
     st = gm_stats()
     _check(lib.gm_mine(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st)))
     return out, st.as_dict()

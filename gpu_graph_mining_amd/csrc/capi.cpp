@@ -103,6 +103,7 @@ gm::MineOptions options(const gm_cfg *cfg) {
     o.path_pruning = cfg->path_pruning == 0;
     if (cfg->speculate < 0 || cfg->speculate > 2) throw gm::Error(GM_E_ARG, "speculate must be 0, 1 or 2");
     o.speculate = cfg->speculate;
+    o.stage_events = cfg->stage_events != 0;
     if (cfg->block_threads != 0 && cfg->block_threads != 64 && cfg->block_threads != 128 &&
         cfg->block_threads != 256)
       throw gm::Error(GM_E_ARG, "block_threads must be 0, 64, 128 or 256");

@@ -560,7 +560,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   // the root level's lists sit at the adjacency offsets
   if (opt.candidate_lists) grow(w.cand[0], sizeof(uint2) * std::max<int64_t>(db.n_adj, 1), s);
 
-  double ext_ms = 0, dev_sec = 0, canon_sec = 0;
+  double ext_ms = 0, dev_sec = 0, canon_sec = 0, ev_stage_ms = 0;
   uint64_t ext_launches = 0;
   size_t peak = db.bytes;
 
@@ -757,6 +757,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
 
     // 32-bit dedup keys: launch over pattern ranges with range * K < 2^32
     const int64_t chunk = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
+    const bool time_stage = opt.stage_events && !root;
+    if (time_stage) {
+      for (hipEvent_t &e : w.ev)
+        if (!e) hip_check(hipEventCreate(&e), "hipEventCreate");
+      hip_check(hipEventRecord(w.ev[0], s), "hipEventRecord");
+    }
     if (!root && a.use_lists) {
       // list every row's candidates in one parallel pass; the support
       // launches then only deduplicate per graph
@@ -774,6 +780,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.p_hi = (int32_t)std::min<int64_t>(P, lo + chunk);
       hip_check(launch_support(a, G, root, s), "k_support");
     }
+    if (time_stage) hip_check(hipEventRecord(w.ev[1], s), "hipEventRecord");
     // per-wave totals and 64-wave tile sums of the row-parallel children
     // passes (k_reduce zeroes the tile sums)
     // (the root level only for many graphs: up to kScanSingleGraphs the
@@ -865,6 +872,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       ext_ms += ms;
       ext_launches++;
       st.e_cand += ecand;
+    }
+    if (time_stage) {
+      float ev_ms = 0.f;
+      hip_check(hipEventSynchronize(w.ev[1]), "hipEventSynchronize");
+      hip_check(hipEventElapsedTime(&ev_ms, w.ev[0], w.ev[1]), "hipEventElapsedTime");
+      ev_stage_ms += ev_ms;
     }
     st.split_passes += cnt.splits;
     st.pruned += pruned;
@@ -1287,6 +1300,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   st.seconds_device = dev_sec;
   st.seconds_canon = canon_sec;
   st.seconds_ext_kernel = ext_ms * 1e-3;
+  st.seconds_stage_events = ev_stage_ms * 1e-3;
   st.ext_kernel_launches = ext_launches;
   st.peak_device_bytes = peak;
   if (stats) *stats = st;

@@ -56,6 +56,11 @@ struct Workspace {
   DevBuf wtot;        // row-parallel children passes: per-wave child rows, list entries
   HostBuf up, down;
   HostBuf flag;                // the level download's completion word (k_signal)
+  hipEvent_t ev[2] = {nullptr, nullptr};  // stage timing (MineOptions::stage_events)
+  ~Workspace() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
   unsigned long long seq = 0;  // its last value
   // the replica counters hold zeros outside a level step (k_reduce clears
   // what it reads); false after a call that ended early
@@ -121,6 +126,7 @@ struct MineOptions {
   bool candidate_lists = true;  // k_support stores candidates for k_write
   bool path_pruning = true;     // k_expand drops candidates with an infrequent 2-edge sub-path
   int32_t speculate = 0;        // 0 adaptive, 1 never, 2 always (gm_cfg::speculate)
+  bool stage_events = false;    // time the extension + support stages with HIP events
   // Query mode: instead of mining, follow the codes of `queries` (every
   // prefix of a minimum code is one) and store each query's support in
   // query_support; the report callback is not called.

@@ -64,7 +64,9 @@ typedef struct gm_cfg {
   int32_t speculate;        /* children written before their canonical check (the check overlaps the
                                write; non-minimal ones are dropped at the next level): 0 = when the
                                last level's non-minimal share was small (default), 1 = never, 2 = always */
-  int32_t reserved[4];
+  int32_t stage_events;     /* 1 = time each level's extension + support stage with HIP events on the
+                               mining stream (gm_stats.seconds_stage_events); 0 = off (default) */
+  int32_t reserved[3];
 } gm_cfg;
 
 /* Per-run statistics (filled when a non-NULL pointer is passed). */
@@ -85,7 +87,8 @@ typedef struct gm_stats {
   uint64_t max_graph_rows; /* most embeddings one graph held at one level   */
   uint64_t max_table_fill; /* most distinct extensions one graph produced in one pass */
   uint64_t pruned;         /* candidates dropped by the 2-edge path rule (counted in e_cand) */
-  uint64_t reserved[2];
+  double seconds_stage_events; /* extension + support stages timed by HIP events (cfg.stage_events) */
+  uint64_t reserved[1];
 } gm_stats;
 
 /* Pattern callback: n_edges rows of (from, to, fromlabel, elabel, tolabel),

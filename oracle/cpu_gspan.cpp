@@ -249,6 +249,9 @@ class Miner {
   void *user = nullptr;
   Stats st;
   bool want_balg = true;
+  // Sampling for a bounded CPU timing: only the 1-edge roots whose ordinal
+  // (in the order below) is a multiple of root_stride are grown (1 = all).
+  unsigned root_stride = 1;
 
   void run() {
     // gspan.cpp:321-332: every directed edge with fromlabel <= tolabel.
@@ -260,9 +263,11 @@ class Miner {
         if (get_forward_root(g, g[from], edges))
           for (Edge *e : edges) root[g[from].label][e->elabel][g[e->to].label].push(id, e, nullptr);
     }
+    unsigned ordinal = 0;
     for (auto &fl : root)
       for (auto &el : fl.second)
         for (auto &tl : el.second) {
+          if (ordinal++ % root_stride != 0) continue;
           code_.push(0, 1, fl.first, el.first, tl.first);
           project(tl.second);
           code_.pop();
@@ -628,6 +633,26 @@ int orc_mine_capped(int32_t n_graphs, const int32_t *graph_vbegin, const int32_t
   orc::build_db_from_arrays(n_graphs, graph_vbegin, vlabels, graph_ebegin, efrom, eto, elabel, m.db);
   m.minsup = (unsigned)minsup;
   m.max_edges = (unsigned)max_edges;
+  m.cb = cb;
+  m.user = user;
+  auto t0 = std::chrono::steady_clock::now();
+  m.run();
+  double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(m.st, secs, stats);
+  return 0;
+}
+
+// orc_mine over a sample of the pattern tree: only every root_stride-th
+// 1-edge root (in the miner's root order) is grown, each subtree in full.
+// Used to time the CPU baseline on a bounded part of a large workload.
+int orc_mine_sampled(int32_t n_graphs, const int32_t *graph_vbegin, const int32_t *vlabels,
+                     const int32_t *graph_ebegin, const int32_t *efrom, const int32_t *eto, const int32_t *elabel,
+                     int32_t minsup, int32_t root_stride, orc::pattern_cb cb, void *user, orc_stats *stats) {
+  if (n_graphs < 0 || minsup < 1 || root_stride < 1) return -1;
+  orc::Miner m;
+  orc::build_db_from_arrays(n_graphs, graph_vbegin, vlabels, graph_ebegin, efrom, eto, elabel, m.db);
+  m.minsup = (unsigned)minsup;
+  m.root_stride = (unsigned)root_stride;
   m.cb = cb;
   m.user = user;
   auto t0 = std::chrono::steady_clock::now();

@@ -49,6 +49,9 @@ def lib() -> C.CDLL:
         l.orc_mine_capped.restype = C.c_int
         l.orc_mine_capped.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p, C.c_int32, C.c_int32, CB,
                                       C.c_void_p, C.POINTER(orc_stats)]
+        l.orc_mine_sampled.restype = C.c_int
+        l.orc_mine_sampled.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p, C.c_int32, C.c_int32, CB,
+                                       C.c_void_p, C.POINTER(orc_stats)]
         l.orc_mine_file.restype = C.c_int
         l.orc_mine_file.argtypes = [C.c_char_p, C.c_char_p, C.c_int32, CB, C.c_void_p, C.POINTER(orc_stats)]
         l.orc_is_min.restype = C.c_int
@@ -85,6 +88,21 @@ def mine_arrays(arrays: dict, minsup: int, collect: bool = True, max_edges: int 
     if rc != 0:
         raise RuntimeError(f"orc_mine failed: {rc}")
     return out, st.as_dict()
+
+
+def mine_sampled(arrays: dict, minsup: int, root_stride: int) -> dict:
+    """Stats of a run that grows only every root_stride-th 1-edge root (in
+    the oracle's root order), each subtree in full: a bounded sample of the
+    same workload for timing the CPU baseline."""
+    a = {k: np.ascontiguousarray(v, dtype=np.int32) for k, v in arrays.items()}
+    ccb = CB(lambda *args: None)
+    st = orc_stats()
+    rc = lib().orc_mine_sampled(len(a["graph_vbegin"]) - 1, _p(a["graph_vbegin"]), _p(a["vlabels"]),
+                                _p(a["graph_ebegin"]), _p(a["efrom"]), _p(a["eto"]), _p(a["elabel"]), int(minsup),
+                                int(root_stride), ccb, None, C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_mine_sampled failed: {rc}")
+    return st.as_dict()
 
 
 def mine_file(path, filetype: str, minsup: int) -> Tuple[List[Pattern], dict]:
