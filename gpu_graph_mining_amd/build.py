@@ -8,6 +8,7 @@ go to gpu_graph_mining_amd/_build/ (git-ignored).  Run as a module:
 from __future__ import annotations
 
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -36,11 +37,28 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def _run(cmd: list[str]) -> None:
+def _run(cmd: list[str]) -> str:
     proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l)
     if proc.returncode != 0:
         raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def check_no_scratch(remarks: str) -> None:
+    """Fail when a kernel of the gfx950 code object uses scratch memory
+    (from the -Rpass-analysis=kernel-resource-usage remarks)."""
+    kernel = None
+    bad = []
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            kernel = m.group(1)
+        m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+        if m and int(m.group(1)) != 0:
+            bad.append(f"{kernel}: {m.group(1)} bytes/lane")
+    if bad:
+        raise RuntimeError("kernels use scratch memory:\n  " + "\n  ".join(bad))
 
 
 def build(verbose: bool = False, out: Path = LIB, build_dir: Path = BUILD, defines: tuple = ()) -> Path:
@@ -53,8 +71,11 @@ def build(verbose: bool = False, out: Path = LIB, build_dir: Path = BUILD, defin
     objs = []
     for src in DEVICE_SOURCES:
         obj = build_dir / (src + ".o")
-        jobs.append([hipcc, f"--offload-arch={ARCH}", *COMMON, *extra, *DEVICE_FLAGS, "-c", str(CSRC / src), "-o",
-                     str(obj)])
+        # the resource-usage remarks let the build fail on any kernel that
+        # uses scratch memory (per-lane arrays the compiler could not keep in
+        # registers)
+        jobs.append([hipcc, f"--offload-arch={ARCH}", *COMMON, *extra, *DEVICE_FLAGS,
+                     "-Rpass-analysis=kernel-resource-usage", "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = build_dir / (src + ".o")
@@ -65,7 +86,9 @@ def build(verbose: bool = False, out: Path = LIB, build_dir: Path = BUILD, defin
         for cmd in jobs:
             if verbose:
                 print(" ".join(cmd), flush=True)
-        list(ex.map(_run, jobs))
+        outs = list(ex.map(_run, jobs))
+    for out_text in outs[:len(DEVICE_SOURCES)]:
+        check_no_scratch(out_text)
     tmp = out.with_suffix(".so.tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, out)

@@ -99,7 +99,6 @@ gm::MineOptions options(const gm_cfg *cfg) {
     o.max_edges = cfg->max_edges;
     o.canon_threads = cfg->canon_threads;
     o.collect_proj = cfg->collect_proj != 0;
-    o.candidate_lists = cfg->candidate_lists == 0;
     o.path_pruning = cfg->path_pruning == 0;
     if (cfg->speculate < 0 || cfg->speculate > 2) throw gm::Error(GM_E_ARG, "speculate must be 0, 1 or 2");
     o.speculate = cfg->speculate;
@@ -107,7 +106,6 @@ gm::MineOptions options(const gm_cfg *cfg) {
     if (cfg->block_threads != 0 && cfg->block_threads != 64 && cfg->block_threads != 128 &&
         cfg->block_threads != 256)
       throw gm::Error(GM_E_ARG, "block_threads must be 0, 64, 128 or 256");
-    o.block_threads = cfg->block_threads;
   }
   return o;
 }

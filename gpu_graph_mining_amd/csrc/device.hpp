@@ -15,11 +15,15 @@
 // neighbour label folded into the entry, so one 4-byte read answers every
 // question the extension rules ask about a neighbour.
 //
-// Embedding store, one level at a time, graph-major:
-//   grow[g]..grow[g+1]   rows of graph g (int64 offsets)
+// Embedding store, one level at a time, pattern-major: the rows of a
+// pattern are contiguous, and within them the rows of one graph (a segment):
 //   pid[r]               pattern of row r
+//   rgraph[r]            graph of row r
 //   vt[r*S .. r*S+S)     the row's vertex tuple: byte i = local id of pattern
 //                        vertex i (DFS discovery order), S = stride in bytes
+//   rbound[r]            the row's item count (adjacency entries of its
+//                        rightmost vertex and rightmost-path sources): the
+//                        size of its candidate-list region
 // A row is one embedding of its pattern (|Proj(P)| rows per pattern), the
 // counterpart of the reference's column store (embedding_lists.hpp:12-85)
 // with whole tuples instead of back-linked columns.
@@ -46,7 +50,11 @@ struct alignas(16) PatMeta {
   uint32_t bwd_mask; // bit t: the pattern already has edge (rightmost, t)
   uint32_t flags;    // kPatDead: the code is not minimal (its rows are not extended)
   uint32_t rm[kMaxPatV];
+  uint32_t bwd_from;     // bit f: f is the from vertex of rightmost-path edge q >= 1 (a backward target)
+  uint8_t rmq[kMaxPatV]; // that q, by vertex
+  uint32_t pad[3];
 };
+static_assert(sizeof(PatMeta) == 192, "PatMeta layout");
 
 // Extension key of one pattern at a level whose tuple stride is S:
 //   backward to vertex t with edge label e   : t*NEL + e

@@ -166,6 +166,7 @@ void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk
   vgraph.reserve(nv);
   gjob.reserve(ng);
   out.job_first_graph.assign(1, 0);
+  out.job_first_vertex.assign(1, 0);
   std::vector<int32_t> deg, fill, dl;
   std::vector<std::pair<int32_t, int32_t>> pairs;
   int32_t job = 0;
@@ -221,6 +222,7 @@ void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk
     }
     ++job;
     out.job_first_graph.push_back((int32_t)gvbase.size());
+    out.job_first_vertex.push_back((int64_t)vlab.size());
   }
   gvbase.push_back((int32_t)vlab.size());
   vadj.push_back((int32_t)adj.size());
@@ -278,9 +280,11 @@ struct Level {
   Arr<uint32_t> sup;
   Arr<int32_t> node;   // query mode: trie node of the code
   Arr<uint8_t> alive;  // the code is minimal (a dead pattern's rows are not extended)
+  Arr<int64_t> row0;   // n + 1: pattern i's rows are [row0[i], row0[i + 1]) (pattern-major rows)
   void resize(int64_t k, int l) {
     n = k;
     len = l;
+    row0.resize(k + 1);
     code.resize(k * l);
     vlab.resize(k * kMaxPatV);
     nv.resize(k);
@@ -314,6 +318,12 @@ PatMeta make_meta(const DFS *code, int len, const uint8_t *vlab, int nv, int32_t
   m.nrm = (uint8_t)nrm;
   for (int i = 0; i < len; ++i)
     if (code[i].from == nv - 1 && code[i].to < code[i].from) m.bwd_mask |= 1u << code[i].to;
+  // backward targets: the from vertices of rightmost-path edges q >= 1
+  for (int q = 1; q < nrm; ++q) {
+    const int f = (int)(m.rm[q] & 0xFF);
+    m.bwd_from |= 1u << f;
+    m.rmq[f] = (uint8_t)q;
+  }
   return m;
 }
 
@@ -469,28 +479,19 @@ class Pool {
   bool stop_ = false;
 };
 
-// Device-side counters at the head of the frequent-key buffer (bytes):
-//   0 frequent keys (u32)   8 dedup error (i32)   16 split passes (u64)
-//   24 child rows placed (u64)   32 next-level list entries asked (u64)
-// then kSpread lines of spread statistics (candidates, largest table fill).
-// The level-start clear covers all but the child-row words, which k_scan
-// overwrites and the next level's download reads back.
+// Device-side counters at the head of the frequent-key buffer: frequent
+// keys, failed chunks, error bits, the level's list cursor, then kSpread
+// lines of spread statistics.  The level-start clear zeroes all but the
+// written-row slots (k_write of the previous level adds to them; the host
+// compares their running total with the rows it asked for).
 struct Counters {
-  uint32_t fcount, pad0;
-  int32_t err, pad1;
-  unsigned long long splits, wcursor, ncursor, max_rows;  // max_rows: largest graph's child rows (k_scan)
-  unsigned long long pad[2];
+  uint32_t fcount, nfail;
+  int32_t err, pad0;
+  unsigned long long list_cursor;
+  unsigned long long pad[5];
   unsigned long long spread[kSpread * kSpreadStride];
 };
 static_assert(sizeof(Counters) == 64 + 8 * kSpread * kSpreadStride, "counter block layout");
-
-// The write kernel must have produced exactly the rows the embedding
-// counts announced.
-void check_write(const Counters &c, int64_t rows) {
-  if (c.wcursor != (unsigned long long)rows)
-    throw Error(GM_E_DEVICE, "child rows written (" + std::to_string(c.wcursor) +
-                                 ") differ from the embedding count (" + std::to_string(rows) + ")");
-}
 
 // Grow a workspace buffer; the stream is drained first because the old
 // allocation may still be in use by queued kernels.
@@ -500,37 +501,21 @@ void *grow(DevBuf &b, size_t bytes, hipStream_t s) {
   return b.ensure(bytes);
 }
 
-// Grow a replica buffer and clear it (its zero invariant covers all of it).
+// Grow the packed counters and clear them (their zero invariant covers all
+// of the buffer).
 void grow_zero(DevBuf &b, size_t bytes, hipStream_t s) {
   if (bytes <= b.cap && b.p) return;
   grow(b, bytes, s);
-  hip_check(hipMemsetAsync(b.p, 0, b.cap, s), "memset replicas");
+  hip_check(hipMemsetAsync(b.p, 0, b.cap, s), "memset counters");
 }
 
-// Dedup table bits for an expected number of distinct keys per graph: the
-// table passes are split when 3/4 full.
-int table_bits_for(uint64_t keys) {
-  int b = 8;
-  while (b < 12 && ((1ull << b) * 3) / 4 < keys) ++b;
-  return b;
-}
-
-// Graphs with more rows than this at a level run in a second launch with
-// kBigBlock-thread workgroups, so that one large graph does not hold the
-// level up while a 64-lane workgroup walks its rows.
-constexpr int kBigRows = 128;
-constexpr int kBigBlock = 256;
-
-// Candidate-list entries (8 bytes) a level may keep before the write kernel
-// falls back to enumeration.
-constexpr uint64_t kMaxListEntries = 1ull << 32;
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace
 
 bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats *stats, hipStream_t s) {
   std::lock_guard<std::mutex> lock(db.mu);
   Timer total;
-  const int32_t G = db.n_graphs;
   const int32_t NEL = db.NEL, NVL = db.NVL;
   int threads = opt.canon_threads > 0 ? opt.canon_threads
                                       : (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
@@ -538,33 +523,18 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   std::memset(&st, 0, sizeof(st));
   if (!db.ws) db.ws.reset(new Workspace());
   Workspace &w = *db.ws;
-  if (!w.replicas_zero) {
-    if (w.acc_rep.p) hip_check(hipMemsetAsync(w.acc_rep.p, 0, w.acc_rep.cap, s), "memset replicas");
-  }
-  w.replicas_zero = false;  // until this call ends normally
+  if (!w.acc_zero && w.acc.p) hip_check(hipMemsetAsync(w.acc.p, 0, w.acc.cap, s), "memset counters");
+  w.acc_zero = false;  // until this call ends normally
 
   grow(w.minsup, sizeof(int32_t) * opt.minsup.size(), s);
   hip_check(hipMemcpyAsync(w.minsup.p, opt.minsup.data(), sizeof(int32_t) * opt.minsup.size(),
                            hipMemcpyHostToDevice, s), "upload min supports");
-  for (int i = 0; i < 2; ++i) {
-    grow(w.gs[i], sizeof(int64_t) * std::max(G, 1), s);
-    grow(w.gc[i], sizeof(int64_t) * std::max(G, 1), s);
-    grow(w.gcb[i], sizeof(int64_t) * std::max(G, 1), s);
-    grow(w.big[i], sizeof(int32_t) * std::max(G, 1), s);
-  }
-  grow(w.nbig, 2 * sizeof(unsigned long long), s);
-  {
-    const int64_t tiles = (G + kScanTileGraphs - 1) / kScanTileGraphs + 1;
-    grow(w.scan_tiles, 4 * sizeof(long long) * tiles, s);
-  }
-  // the root level's lists sit at the adjacency offsets
-  if (opt.candidate_lists) grow(w.cand[0], sizeof(uint2) * std::max<int64_t>(db.n_adj, 1), s);
 
   double ext_ms = 0, dev_sec = 0, canon_sec = 0, ev_stage_ms = 0;
   uint64_t ext_launches = 0;
   size_t peak = db.bytes;
 
-  LevelArgs a;
+  StepArgs a;
   std::memset(&a, 0, sizeof(a));
   a.gvbase = db.gvbase.as<int32_t>();
   a.vadj = db.vadj.as<int32_t>();
@@ -574,30 +544,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   a.vgraph = db.vgraph.as<int32_t>();
   a.NEL = NEL;
   a.NVL = NVL;
-  // LDS: stage the largest graph's adjacency when it fits 16 KB (larger
-  // graphs are read from HBM)
-  a.stage_v = db.max_graph_v + 1;
-  a.big_rows = kBigRows;
-  {
-    const int64_t tiles = (G + kScanTileGraphs - 1) / kScanTileGraphs + 1;
-    long long *t = w.scan_tiles.as<long long>();
-    a.tile_rows = t;
-    a.tile_ents = reinterpret_cast<unsigned long long *>(t + tiles);
-    a.tile_big = t + 2 * tiles;
-    a.tile_max = t + 3 * tiles;
-  }
-  a.big_block = kBigBlock;
-  a.stage_a = std::min<int32_t>(db.max_graph_adj, 4096);
-  int dev = 0, lds_max = 0, wall_khz = 1;
+  int dev = 0, wall_khz = 1;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   hip_check(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
   if (wall_khz <= 0) wall_khz = 100000;
-  hip_check(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
-  {
-    int cus = 0;
-    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
-    a.wave_grid = std::max(1, cus) * 8;  // 8 workgroups of 4 waves per CU
-  }
 
   Level L, N;             // patterns of the current and of the next level
   Arr<Freq> frall, fr, frtmp;
@@ -611,14 +561,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   int level = 0;          // 0 = root stage
   int S = 0;              // tuple stride of the current level
   int64_t n_rows = 0;
-  int cur = 0;                // which of the double buffers holds the current level
-  int64_t pending_rows = -1;  // child rows the last write must have produced
+  int cur = 0;                   // which of the double buffers holds the current level
+  unsigned long long written_expect = 0;  // child rows asked of every k_write so far
+  uint64_t list_cap = (uint64_t)std::max<int64_t>(db.n_adj, 1);  // this level's list entries (root: 2E)
   uint32_t last_fcount = 0;
-  uint32_t last_fill = 0;     // largest dedup table fill of the last level
-  int max_nrm = 0;            // longest rightmost path among this level's patterns
-  // the counter block is zero but for the child-row words (the write pass's
-  // scan of the last level cleared it)
-  bool counters_clear = false;
+  int table_bits = 9;
   const Counters *dctr_last = nullptr;
   // Path pruning: t2[((job * NVL) + centre label) * AR + arm] is the mask of
   // the arms (elabel * NVL + neighbour label) that form a frequent 2-edge
@@ -636,10 +583,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   size_t mask_off = 0;  // this level's masks in its pattern upload (stride 0: none)
   int mask_stride = 0;
   double dead_share = 0;  // the last level's rows of non-minimal children / its child rows
-  bool lists_now = opt.candidate_lists;  // this level's rows have candidate lists
   // GM_LEVEL_TRACE=1: one line per level step on stderr (sizes, timings)
   const char *trace_env = getenv("GM_LEVEL_TRACE");
   const bool trace_levels = trace_env && trace_env[0] == '1';
+  // root "patterns" are the jobs: their rows are the vertices of their graphs
+  const std::vector<int64_t> &job_row0 = db.job_first_vertex;
 
   // query mode: record the supports of the queries that end at `node`; the
   // child is kept (its rows written) only when longer queries continue
@@ -649,6 +597,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     return !nd.kids.empty();
   };
   std::vector<DFS> orig;  // a reported code in the database's own labels
+  bool counters_fresh = true;  // the counter block must be zeroed (first level, or it moved)
 
   for (;;) {
     const bool root = level == 0;
@@ -657,267 +606,161 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const int64_t P = root ? db.n_jobs : L.n;
     const int64_t K = root ? (int64_t)NVL * NEL * NVL : (int64_t)S * NEL * (NVL + 1);
     const int64_t nkeys = P * K;
-    // Packed support / embedding counters: `rep` replicas (graph g counts
-    // into replica g mod rep; k_reduce sums and clears them) spread the
-    // atomics on the keys every graph hits -- one address takes ~12 ns per
-    // atomic.  The replicas take at most 16 bytes of k_reduce traffic per
-    // row of the level (8-256 MB).
     const int64_t rows_now = root ? db.n_vertices : n_rows;
-    const int64_t rep_budget = std::min<int64_t>(256ll << 20, std::max<int64_t>(8ll << 20, 16 * rows_now));
-    int rep = 1;
-    while (rep < 64 && (int64_t)rep * 2 * nkeys * 8 <= rep_budget) rep *= 2;
-    grow(w.sup, sizeof(int32_t) * nkeys, s);
-    grow_zero(w.acc_rep, sizeof(unsigned long long) * nkeys * rep, s);
-    const uint32_t fcap = (uint32_t)std::min<int64_t>(nkeys, std::max<int64_t>(1 << 20, P * 64));
-    const size_t freq_bytes = sizeof(Counters) + 3 * sizeof(unsigned long long) * fcap;
-    if (w.freq.p && freq_bytes > w.freq.cap && pending_rows >= 0) {
-      // the counter block moves: settle the last write's check first
-      Counters c;
-      hip_check(hipMemcpyAsync(&c, w.freq.p, sizeof(c), hipMemcpyDeviceToHost, s), "download counters");
-      hip_check(hipStreamSynchronize(s), "k_write");
-      check_write(c, pending_rows);
-      pending_rows = -1;
-    }
-    if (grow(w.freq, freq_bytes, s) != (void *)dctr_last) counters_clear = false;  // a new block
-    Counters *dctr = w.freq.as<Counters>();
-    dctr_last = dctr;
-    const int64_t list_rows = root ? db.n_vertices : n_rows;
-    a.use_lists = lists_now ? 1 : 0;
-    if (a.use_lists) grow(w.rccnt, sizeof(uint16_t) * std::max<int64_t>(list_rows, 1), s);
-    a.cand = w.cand[cur].as<uint2>();
-    a.gcbase = w.gcb[cur].as<int64_t>();
-    a.rcoff = w.rcoff[cur].as<uint32_t>();
-    a.rccnt = w.rccnt.as<uint16_t>();
-    a.big = w.big[cur].as<int32_t>();
-    a.nbig = w.nbig.as<unsigned long long>() + cur;
-    // at most rows / (big_rows + 1) graphs can be big (none at the root)
-    a.big_grid = root ? 0 : (int32_t)std::min<int64_t>(G, n_rows / (kBigRows + 1));
-    a.gstart = root ? nullptr : w.gs[cur].as<int64_t>();
-    a.gcount = root ? nullptr : w.gc[cur].as<int64_t>();
-    a.pid = root ? nullptr : w.pid[cur].as<int32_t>();
-    a.rgraph = root ? nullptr : w.rgraph[cur].as<int32_t>();
-    a.n_rows = root ? 0 : n_rows;
-    a.lists_ready = 0;
-    a.vt = root ? nullptr : w.vt[cur].as<uint8_t>();
-    a.S = S;
-    a.meta = root ? nullptr : reinterpret_cast<const PatMeta *>(w.upl[cur].as<char>());
-    a.mask_stride = root ? 0 : mask_stride;
-    a.pmask = a.mask_stride ? reinterpret_cast<const unsigned long long *>(w.upl[cur].as<char>() + mask_off) : nullptr;
-    a.K = K;
-    a.sup = w.sup.as<int32_t>();
-    a.nkeys = nkeys;
-    a.acc_rep = w.acc_rep.as<unsigned long long>();
-    a.rep = rep;
-    a.minsup_job = w.minsup.as<int32_t>();
-    a.freq = reinterpret_cast<unsigned long long *>(dctr + 1);
-    a.freq_cap = fcap;
-    a.freq_count = &dctr->fcount;
-    a.spread = dctr->spread;
-    a.error_flag = &dctr->err;
-    a.split_passes = &dctr->splits;
-    a.cursor = &dctr->wcursor;
-    a.ncand_cursor = &dctr->ncursor;
-    if (!counters_clear) hip_check(launch_clear_counters(a, s), "clear counters");
-    counters_clear = false;  // this level's kernels count into it
-    // workgroup size: enough lanes for the average graph's rows (or vertices)
-    if (opt.block_threads > 0) {
-      a.block = opt.block_threads;
-    } else {
-      const double per_graph =
-          root ? (double)db.n_vertices / std::max(1, G) : (double)n_rows / (double)std::max(1, G);
-      a.block = per_graph > 128 ? 256 : per_graph > 64 ? 128 : 64;
-    }
-    // dedup table: sized for twice the last level's fullest graph (root
-    // level: the largest adjacency), at most 2^10 entries so that many
-    // workgroups fit a CU's LDS; larger graphs split their passes
-    a.table_bits = std::min(10, table_bits_for(root ? (uint64_t)db.max_graph_adj : 2ull * last_fill));
-    // per-thread slots: the tuple, then the row's rightmost path (at least
-    // 2 words for the list replay; odd word stride against bank conflicts)
-    const int rm_words = (int)round4(max_nrm);
-    a.tuple_words = S / 4;
-    a.slot_words = std::max(2, a.tuple_words + rm_words) | 1;
-    {
-      LevelArgs b = a;
-      b.block = std::max(a.block, a.big_block);
-      while (level_lds_bytes(b, true) > (size_t)lds_max && b.table_bits > 6) --b.table_bits;
-      if (level_lds_bytes(b, true) > (size_t)lds_max)
-        throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
-      a.table_bits = b.table_bits;
-    }
-    // the large-graph launch has few workgroups per CU: its table takes up
-    // to 2^12 entries, so that graphs with thousands of distinct keys do not
-    // split their passes
-    {
-      LevelArgs b = a;
-      b.block = a.big_block;
-      b.table_bits = std::max(a.table_bits, table_bits_for(root ? (uint64_t)db.max_graph_adj : 2ull * last_fill));
-      while (level_lds_bytes(b, true) > (size_t)lds_max && b.table_bits > a.table_bits) --b.table_bits;
-      a.big_table_bits = b.table_bits;
-    }
-
-    // 32-bit dedup keys: launch over pattern ranges with range * K < 2^32
-    const int64_t chunk = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
-    const bool time_stage = opt.stage_events && !root;
-    if (time_stage) {
-      for (hipEvent_t &e : w.ev)
-        if (!e) hip_check(hipEventCreate(&e), "hipEventCreate");
-      hip_check(hipEventRecord(w.ev[0], s), "hipEventRecord");
-    }
-    if (!root && a.use_lists) {
-      // list every row's candidates in one parallel pass; the support
-      // launches then only deduplicate per graph
-      LevelArgs e = a;
-      e.rm_words = rm_words;
-      e.src_words = max_nrm + 2;
-      e.slot_words = (a.tuple_words + e.rm_words + 2 * e.src_words + 5) | 1;
-      if (expand_lds_bytes(e) > (size_t)lds_max)
-        throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
-      hip_check(launch_expand(e, s), "k_expand");
-      a.lists_ready = 1;
-    }
-    for (int64_t lo = 0; lo < P; lo += chunk) {
-      a.p_lo = (int32_t)lo;
-      a.p_hi = (int32_t)std::min<int64_t>(P, lo + chunk);
-      hip_check(launch_support(a, G, root, s), "k_support");
-    }
-    if (time_stage) hip_check(hipEventRecord(w.ev[1], s), "hipEventRecord");
-    // per-wave totals and 64-wave tile sums of the row-parallel children
-    // passes (k_reduce zeroes the tile sums)
-    // (the root level only for many graphs: up to kScanSingleGraphs the
-    // per-graph passes place the children with one single-workgroup scan
-    // and are as fast)
-    a.n_wtiles = 0;
-    if (root ? (a.use_lists != 0 && G > kScanSingleGraphs) : a.lists_ready != 0) {
-      const int64_t nw = ((root ? db.n_vertices : n_rows) + kRowWaveRows - 1) / kRowWaveRows, nt = (nw + 63) / 64;
-      const int64_t tw = nt * kTileStride;  // one cache line per tile sum
-      auto *wt = static_cast<unsigned long long *>(grow(w.wtot, sizeof(unsigned long long) * 2 * (nw + tw), s));
-      a.wrows = wt;
-      a.wents = wt + nw;
-      a.trows = wt + 2 * nw;
-      a.tents = wt + 2 * nw + tw;
-      a.n_wtiles = (int32_t)nt;
-    }
-    // Speculative levels (every frequent key becomes a child): the
-    // row-parallel count pass needs no child ids, so it is launched as soon
-    // as the frequent keys are on the host and runs beside their sort and
-    // decode (the same condition as `spec` below, with the bound on the
-    // children's vertices)
-    a.count_done = 0;
-    const bool early_count = !root && a.n_wtiles > 0 && !opt.queries &&
-                             (opt.speculate == 2 || (opt.speculate == 0 && dead_share <= 0.3)) &&
-                             !(opt.max_edges > 0 && L.len + 1 > opt.max_edges) && L.max_nv + 1 <= kMaxPatV;
-    if (early_count) {
-      a.rowcnt = static_cast<uint16_t *>(grow(w.rowcnt, sizeof(uint16_t) * std::max<int64_t>(list_rows, 1), s));
-      a.rowbsum = static_cast<uint32_t *>(grow(w.rowbsum, sizeof(uint32_t) * std::max<int64_t>(list_rows, 1), s));
-      a.nnbig = w.nbig.as<unsigned long long>() + (cur ^ 1);  // the placement scan zeroes the next level's count
-    }
-    hip_check(launch_reduce(a, root, s), "k_reduce");
-    // counters and (most often all of) the frequent keys in one copy
-    const uint32_t guess = std::min<uint32_t>(fcap, std::max<uint32_t>(4096, 2 * last_fcount));
-    const size_t head = sizeof(Counters) + 3 * sizeof(unsigned long long) * guess;
-    char *hbuf = static_cast<char *>(w.down.ensure(head));
-    hip_check(launch_copy8(w.down.dev, dctr, head, s), "download counters");
-    Timer t_sync;
-    // a short frequent-key list (the last level's fitted the first download
-    // twice over) is all but certain to come down in this one copy: the
-    // count pass is queued right behind it and the host polls a completion
-    // word instead of waiting for the stream; longer lists may need a second
-    // copy, which would wait for the count, so their count follows the copies
-    const bool queued_count = early_count && 2 * (uint64_t)last_fcount < guess;
-    if (queued_count) {
-      if (w.seq == 0) {
-        w.flag.ensure(64);
-        std::memset(w.flag.p, 0, 64);
-      }
-      const unsigned long long want = ++w.seq;
-      hip_check(launch_signal(static_cast<unsigned long long *>(w.flag.dev), want, s), "signal");
-      a.count_by_support = 1;
-      hip_check(launch_count(a, root, s), "k_count");
-      a.count_by_support = 0;
-      a.count_done = 1;
-      const unsigned long long *fl = static_cast<const unsigned long long *>(w.flag.p);
-      unsigned spins = 0;
-      while (__atomic_load_n(fl, __ATOMIC_ACQUIRE) != want) {
-        if ((++spins & 1023) == 0 && t_sync.sec() > 2.0) {  // surfaces a device error, if any
-          hip_check(hipStreamSynchronize(s), "k_support");
-          break;
-        }
-#if defined(__x86_64__)
-        __builtin_ia32_pause();
-#endif
-      }
-    } else {
-      hip_check(hipStreamSynchronize(s), "k_support");
-    }
-    const double us_sync = t_sync.sec() * 1e6;
-    Timer t_host;  // host work between the level's download and its upload
+    auto prow = [&](int64_t p) { return root ? job_row0[p] : L.row0[p]; };
+    // launches over pattern ranges with range * K < 2^32 (32-bit table keys)
+    const int64_t prange = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
+    int64_t total_chunks = 0;
+    for (int64_t lo = 0; lo < P; lo += prange)
+      total_chunks += ceil_div(prow(std::min<int64_t>(P, lo + prange)) - prow(lo), kChunkRows);
+    grow(w.sup, sizeof(int32_t) * std::max<int64_t>(nkeys, 1), s);
+    grow_zero(w.acc, sizeof(unsigned long long) * std::max<int64_t>(nkeys, 1), s);
+    grow(w.chunks, sizeof(ChunkInfo) * std::max<int64_t>(total_chunks, 1), s);
+    grow(w.fail, sizeof(uint32_t) * std::max<int64_t>(total_chunks, 1), s);
+    grow(w.rccnt, sizeof(uint16_t) * std::max<int64_t>(rows_now, 1), s);
+    grow(w.cand, sizeof(uint2) * std::max<uint64_t>(list_cap, 1), s);
+    uint32_t fcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(nkeys, 1), std::max<int64_t>(1 << 20, P * 64));
+    Counters *dctr = nullptr;
     Counters cnt;
-    std::memcpy(&cnt, hbuf, sizeof(cnt));
-    unsigned long long ecand = 0, max_fill = 0, cand_bound = 0, pruned = 0;
-    for (int i = 0; i < kSpread; ++i) {
-      ecand += cnt.spread[i * kSpreadStride + kSpreadCand];
-      pruned += cnt.spread[i * kSpreadStride + kSpreadPruned];
-      max_fill = std::max(max_fill, cnt.spread[i * kSpreadStride + kSpreadFill]);
-      cand_bound += cnt.spread[i * kSpreadStride + kSpreadBound];
+    const char *hbuf = nullptr;
+    double ms = 0;
+    unsigned long long ecand = 0, max_fill = 0, cand_bound = 0, pruned = 0, failed = 0;
+    uint32_t guess = 0;  // frequent keys in the first download
+    // the stage; rerun with a longer frequent-key list when it overflowed
+    for (int attempt = 0;; ++attempt) {
+      const size_t freq_bytes = sizeof(Counters) + 3 * sizeof(unsigned long long) * fcap;
+      if (grow(w.freq, freq_bytes, s) != (const void *)dctr_last) counters_fresh = true;  // a new block
+      dctr = w.freq.as<Counters>();
+      if (counters_fresh) {
+        // written rows of an earlier level live in the old block: checked already
+        hip_check(hipMemsetAsync(dctr, 0, sizeof(Counters), s), "clear counters");
+        written_expect = 0;
+        counters_fresh = false;
+      }
+      dctr_last = dctr;
+      a.rccnt = w.rccnt.as<uint16_t>();
+      a.cand = w.cand.as<uint2>();
+      a.list_cap = list_cap;
+      a.list_cursor = &dctr->list_cursor;
+      a.chunks = w.chunks.as<ChunkInfo>();
+      a.fail_list = w.fail.as<uint32_t>();
+      a.fail_count = &dctr->nfail;
+      a.acc = w.acc.as<unsigned long long>();
+      a.table_bits = table_bits;
+      a.spread = dctr->spread;
+      a.error_flag = &dctr->err;
+      a.sup = w.sup.as<int32_t>();
+      a.nkeys = nkeys;
+      a.K = K;
+      a.minsup_job = w.minsup.as<int32_t>();
+      a.freq = reinterpret_cast<unsigned long long *>(dctr + 1);
+      a.freq_cap = fcap;
+      a.freq_count = &dctr->fcount;
+      a.pid = root ? nullptr : w.pid[cur].as<int32_t>();
+      a.rgraph = root ? nullptr : w.rgraph[cur].as<int32_t>();
+      a.vt = root ? nullptr : w.vt[cur].as<uint8_t>();
+      a.rbound = root ? nullptr : w.rbound[cur].as<uint32_t>();
+      a.S = S;
+      a.meta = root ? nullptr : reinterpret_cast<const PatMeta *>(w.upl[cur].as<char>());
+      a.mask_stride = root ? 0 : mask_stride;
+      a.pmask = a.mask_stride ? reinterpret_cast<const unsigned long long *>(w.upl[cur].as<char>() + mask_off)
+                              : nullptr;
+      hip_check(launch_clear_counters(a, s), "clear counters");
+      const bool time_stage = opt.stage_events && !root;
+      if (time_stage) {
+        for (hipEvent_t &e : w.ev)
+          if (!e) hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipEventRecord(w.ev[0], s), "hipEventRecord");
+      }
+      int64_t chunk0 = 0;
+      for (int64_t lo = 0; lo < P; lo += prange) {
+        const int64_t hi = std::min<int64_t>(P, lo + prange);
+        a.p_lo = (int32_t)lo;
+        a.p_hi = (int32_t)hi;
+        a.row_lo = prow(lo);
+        a.row_hi = prow(hi);
+        a.chunk0 = chunk0;
+        a.n_chunks = ceil_div(a.row_hi - a.row_lo, kChunkRows);
+        hip_check(launch_level(a, root, s), "k_level");
+        chunk0 += a.n_chunks;
+      }
+      // the recount and the write see the whole level (segment starts are
+      // pattern changes at the launch boundaries anyway)
+      a.row_lo = 0;
+      a.row_hi = rows_now;
+      a.chunk0 = 0;
+      a.n_chunks = total_chunks;
+      hip_check(launch_level_fix(a, root, s), "k_level_fix");
+      if (time_stage) hip_check(hipEventRecord(w.ev[1], s), "hipEventRecord");
+      hip_check(launch_reduce(a, root, s), "k_reduce");
+      // counters and (most often all of) the frequent keys in one copy
+      guess = std::min<uint32_t>(fcap, std::max<uint32_t>(4096, 2 * last_fcount));
+      const size_t head = sizeof(Counters) + 3 * sizeof(unsigned long long) * guess;
+      hbuf = static_cast<const char *>(w.down.ensure(head));
+      hip_check(launch_copy8(w.down.dev, dctr, head, s), "download counters");
+      hip_check(hipStreamSynchronize(s), "k_level");
+      std::memcpy(&cnt, hbuf, sizeof(cnt));
+      if (cnt.err) throw Error(GM_E_DEVICE, "internal: level step error " + std::to_string(cnt.err));
+      ecand = max_fill = cand_bound = pruned = failed = 0;
+      unsigned long long written = 0, t_start = 0, t_end = 0;
+      for (int i = 0; i < kSpread; ++i) {
+        const unsigned long long *line = cnt.spread + i * kSpreadStride;
+        ecand += line[kSpreadCand];
+        pruned += line[kSpreadPruned];
+        max_fill = std::max(max_fill, line[kSpreadFill]);
+        cand_bound += line[kSpreadBound];
+        failed += line[kSpreadFailed];
+        written += line[kSpreadWritten];
+        t_start = std::max(t_start, line[kSpreadStart]);
+        t_end = std::max(t_end, line[kSpreadEnd]);
+      }
+      if (written != written_expect)
+        throw Error(GM_E_DEVICE, "child rows written (" + std::to_string(written) +
+                                     ") differ from the embedding counts (" + std::to_string(written_expect) + ")");
+      t_start = ~t_start;
+      ms = (t_end > t_start && t_start != ~0ull) ? (double)(t_end - t_start) / wall_khz : 0.0;
+      if (time_stage) {
+        float ev_ms = 0.f;
+        hip_check(hipEventElapsedTime(&ev_ms, w.ev[0], w.ev[1]), "hipEventElapsedTime");
+        ev_stage_ms += ev_ms;
+      }
+      if (cnt.fcount <= fcap) break;
+      // the list was too short: k_reduce cleared the counters, so the stage
+      // is run again with room for every frequent key
+      if (attempt > 0) throw Error(GM_E_DEVICE, "internal: frequent-key count changed between runs");
+      fcap = cnt.fcount;
     }
-    // the stage's span from the kernels' wall-clock stamps
-    unsigned long long t_start = 0, t_end = 0;
-    for (int i = 0; i < kSpread; ++i) {
-      t_start = std::max(t_start, cnt.spread[i * kSpreadStride + kSpreadStart]);
-      t_end = std::max(t_end, cnt.spread[i * kSpreadStride + kSpreadEnd]);
-    }
-    t_start = ~t_start;
-    const double ms = (t_end > t_start && t_start != ~0ull) ? (double)(t_end - t_start) / wall_khz : 0.0;
     if (!root) {
       ext_ms += ms;
       ext_launches++;
       st.e_cand += ecand;
     }
-    if (time_stage) {
-      float ev_ms = 0.f;
-      hip_check(hipEventSynchronize(w.ev[1]), "hipEventSynchronize");
-      hip_check(hipEventElapsedTime(&ev_ms, w.ev[0], w.ev[1]), "hipEventElapsedTime");
-      ev_stage_ms += ev_ms;
-    }
-    st.split_passes += cnt.splits;
+    st.split_passes += failed;
     st.pruned += pruned;
-    if (pending_rows >= 0) {
-      check_write(cnt, pending_rows);
-      st.max_graph_rows = std::max<uint64_t>(st.max_graph_rows, cnt.max_rows);
-    }
     st.max_table_fill = std::max<uint64_t>(st.max_table_fill, max_fill);
-    if (cnt.err) throw Error(GM_E_UNSUPPORTED, "per-graph deduplication table overflow");
-    if (cnt.fcount > fcap)
-      throw Error(GM_E_OOM, "frequent-key list overflow (" + std::to_string(cnt.fcount) + " keys)");
+    // a level with recounted chunks wants a larger table next time
+    if (failed * 64 > (unsigned long long)total_chunks && table_bits < 11) ++table_bits;
     last_fcount = cnt.fcount;
-    last_fill = (uint32_t)max_fill;
     const int64_t F = cnt.fcount;
     if (F) {
-      const uint32_t have = std::min(cnt.fcount, guess);
       const Freq *src = reinterpret_cast<const Freq *>(hbuf + sizeof(Counters));
-      if (cnt.fcount > have) {  // the rest of the list, after the part downloaded
+      if (F > (int64_t)guess) {  // the rest of the list, after the part downloaded
         frall.resize(F);
-        std::memcpy(frall.data(), src, sizeof(Freq) * have);
-        const size_t rest = sizeof(Freq) * (cnt.fcount - have);
+        std::memcpy(frall.data(), src, sizeof(Freq) * guess);
+        const size_t rest = sizeof(Freq) * (F - guess);
         char *h2 = static_cast<char *>(w.down.ensure(rest));
-        hip_check(launch_copy8(w.down.dev, reinterpret_cast<const char *>(a.freq) + sizeof(Freq) * have, rest, s),
+        hip_check(launch_copy8(w.down.dev, reinterpret_cast<const char *>(a.freq) + sizeof(Freq) * guess, rest, s),
                   "download frequent keys");
         hip_check(hipStreamSynchronize(s), "download frequent keys");
-        std::memcpy(frall.data() + have, h2, rest);
+        std::memcpy(frall.data() + guess, h2, rest);
         src = frall.data();
-      }
-      if (early_count && !a.count_done) {
-        a.count_by_support = 1;
-        hip_check(launch_count(a, root, s), "k_count");
-        a.count_by_support = 0;
-        a.count_done = 1;
       }
       sort_freq(src, F, (uint64_t)nkeys, fr, frtmp);
     }
     dev_sec += tdev.sec();
+    Timer t_host;
 
     // decode the frequent keys into child tuples
-    const double us_prep = t_host.sec() * 1e6;
     Timer tcan;
     const int new_len = root ? 1 : L.len + 1;
     kidx.resize(F);
@@ -982,7 +825,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       for (int64_t i = 0; i < F; ++i) max_nv_exact = std::max(max_nv_exact, (int)knv[i]);
     const bool spec = want_spec && !root && !opt.queries && !past_max &&
                       (early_decode ? max_nv_exact : max_nv_bound) <= kMaxPatV;
-    if (a.count_done && !spec) throw Error(GM_E_DEVICE, "internal: early count on a level that is not speculative");
     const bool late_decode = spec && !early_decode;
     if (!spec) {
       if (!early_decode) Pool::get().run(F, threads, decode, 4096);
@@ -1013,10 +855,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     }
     canon_sec += tcan.sec();
     double us_canon = tcan.sec() * 1e6;
-    Timer t_sel;
     st.n_candidates += root ? 0 : F;
 
-    // children in key order
+    // children in key order; child c's rows are [row0[c], row0[c + 1])
     int64_t C = 0, child_rows = 0;
     int max_nv = 0;
     for (int64_t i = 0; i < F; ++i) {
@@ -1028,93 +869,66 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     if (late_decode) max_nv = max_nv_bound;
     N.resize(C, new_len);
     nproj.resize(C);
-    double us_select = 0, us_host = 0, us_overlap = 0, us_crit = 0;
+    double us_host = 0, us_overlap = 0;
     int64_t alive = 0, alive_rows = 0;  // minimal children and their embeddings
     int CS = 4;
-    pending_rows = -1;
-    bool lists_next = false;
     if (C > 0) {
-      // The embedding counts size the child level exactly; the write kernel
-      // reserves each graph's rows from a cursor.  The selected keys point
-      // at their child ids first (read from the mapped staging buffer) and
-      // the write pass starts; the children's metadata (the next level's)
+      // The embedding counts size the child level exactly: child c's rows
+      // start at the sum of the earlier children's counts, k_write reserves
+      // them per chunk from the child's cursor.  The selected keys point at
+      // their child ids first (read from the mapped staging buffer) and the
+      // write pass starts; the children's metadata (the next level's)
       // follows in one upload.
       Timer tdev2;
       CS = (int)round4(max_nv);
       const int nxt = cur ^ 1;
       const size_t meta_bytes = sizeof(PatMeta) * C;
       const size_t keys_off = (meta_bytes + 7) & ~size_t(7);
-      const size_t vals_off = keys_off + sizeof(unsigned long long) * C;
+      const size_t base_off = keys_off + sizeof(unsigned long long) * C;
       // path masks of the children with >= 2 edges: a forward and a
       // backward word per rightmost-path vertex (at most max_nv of them)
       const bool masks = prune && new_len >= 2;
       const int mstride = masks ? max_nv : 0;
-      const size_t masks_off = (vals_off + sizeof(int32_t) * C + 7) & ~size_t(7);
+      const size_t masks_off = base_off + sizeof(unsigned long long) * C;
       const size_t up_bytes = masks_off + sizeof(uint64_t) * 2 * mstride * C;
       char *hu = static_cast<char *>(w.up.ensure(up_bytes));
       PatMeta *hm = reinterpret_cast<PatMeta *>(hu);
       unsigned long long *hk = reinterpret_cast<unsigned long long *>(hu + keys_off);
-      int32_t *hv = reinterpret_cast<int32_t *>(hu + vals_off);
+      unsigned long long *hb = reinterpret_cast<unsigned long long *>(hu + base_off);
+      int64_t run = 0;
       for (int64_t i = 0; i < F; ++i) {
         const int64_t c = kidx[i];
         if (c < 0) continue;
         hk[c] = fr[i].key;
-        hv[c] = -(int32_t)(c + 1);
+        hb[c] = (unsigned long long)run;
+        N.row0[c] = run;
+        run += (int64_t)fr[i].proj;
       }
+      N.row0[C] = run;
       char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
       const char *su = static_cast<const char *>(w.up.dev);
       grow(w.pid[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.rgraph[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.vt[nxt], (size_t)CS * std::max<int64_t>(child_rows, 1), s);
-      grow(w.rowcnt, sizeof(uint16_t) * std::max<int64_t>(list_rows, 1), s);
-      grow(w.rowbsum, sizeof(uint32_t) * std::max<int64_t>(list_rows, 1), s);
-      // the children's candidate lists: at most the bound sum of all this
-      // level's candidates (the selected ones need less)
-      lists_next = opt.candidate_lists && cand_bound > 0 && cand_bound < kMaxListEntries;
-      if (lists_next) {
-        grow(w.cand[nxt], sizeof(uint2) * cand_bound, s);
-        grow(w.rcoff[nxt], sizeof(uint32_t) * std::max<int64_t>(child_rows, 1), s);
-      }
-      grow(w.gbsum, sizeof(unsigned long long) * std::max(G, 1), s);
-      a.rowcnt = w.rowcnt.as<uint16_t>();
-      a.rowbsum = w.rowbsum.as<uint32_t>();
-      a.gbsum = w.gbsum.as<unsigned long long>();
+      grow(w.rbound[nxt], sizeof(uint32_t) * std::max<int64_t>(child_rows, 1), s);
+      grow(w.cursor, sizeof(unsigned long long) * C, s);
+      a.cursor = w.cursor.as<unsigned long long>();
       a.ccap = child_rows;
-      a.cgstart = w.gs[nxt].as<int64_t>();
-      a.cgcount = w.gc[nxt].as<int64_t>();
       a.cpid = w.pid[nxt].as<int32_t>();
       a.crgraph = w.rgraph[nxt].as<int32_t>();
       a.cvt = w.vt[nxt].as<uint8_t>();
+      a.cbound = w.rbound[nxt].as<uint32_t>();
       a.CS = CS;
-      a.next_lists = lists_next ? 1 : 0;
-      a.tuple_words = std::max(S, CS) / 4;
-      a.slot_words = (a.tuple_words + rm_words) | 1;
-      {
-        LevelArgs b = a;
-        b.block = std::max(a.block, a.big_block);
-        if (level_lds_bytes(b, false) > (size_t)lds_max)
-          throw Error(GM_E_UNSUPPORTED, "level step needs more LDS than a workgroup has");
-      }
-      a.ncand_cap = lists_next ? (int64_t)(w.cand[nxt].cap / sizeof(uint2)) : 0;
-      a.ngcbase = w.gcb[nxt].as<int64_t>();
-      a.nbig_list = w.big[nxt].as<int32_t>();
-      a.nnbig = w.nbig.as<unsigned long long>() + nxt;
-      a.nrcoff = w.rcoff[nxt].as<uint32_t>();
-      // where every row's candidates are listed (k_expand at later levels,
-      // the root pass at the root) the children are counted, placed and
-      // written over waves of rows (a large graph spreads over many waves),
-      // else per graph
-      a.rows_mode = a.n_wtiles > 0 ? 1 : 0;
-      if (root && a.rows_mode) a.n_rows = db.n_vertices;  // the root level's rows are the vertices
-      // the selected keys point at their child ids, then the write pass
-      // (launched back to back: the device does not wait on the host between)
+      a.chunk0 = 0;
+      a.n_chunks = total_chunks;
+      // the selected keys point at their child ids and the children's first
+      // rows go to their cursors, then the write pass (launched back to
+      // back: the device does not wait on the host between)
       hip_check(launch_upload_scatter(du, su, 0, w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(su + keys_off),
-                                      reinterpret_cast<const int32_t *>(su + vals_off), C, s),
+                                      a.cursor, reinterpret_cast<const unsigned long long *>(su + base_off), C, s),
                 "scatter child ids");
-      us_crit = t_host.sec() * 1e6;
-      hip_check(launch_write(a, G, root, s), "k_write");
-      counters_clear = true;  // its scan cleared the counters for the next level
-      us_select = t_sel.sec() * 1e6;
+      hip_check(launch_write(a, root, s), "k_write");
+      written_expect += (unsigned long long)child_rows;
       us_host = t_host.sec() * 1e6;
 
       // while the device writes the children: the canonical check of the
@@ -1222,11 +1036,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         canon_sec += tc2.sec();
         us_canon += tc2.sec() * 1e6;
       }
-      int next_nrm = 0;  // over every child: LDS slots hold any row's rightmost path
       int64_t dead_rows = 0;
       N.max_nv = 0;
       for (int64_t c = 0; c < C; ++c) {
-        next_nrm = std::max(next_nrm, (int)hm[c].nrm);
         N.max_nv = std::max(N.max_nv, (int)N.nv[c]);
         if (!N.alive[c]) {
           dead_rows += (int64_t)nproj[c];
@@ -1244,19 +1056,16 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       dead_share = (double)nonmin_rows / (double)std::max<int64_t>(1, alive_rows + nonmin_rows);
       hip_check(launch_copy8(du, su, up_bytes, s), "upload children");
       us_overlap = t_over.sec() * 1e6;
-      pending_rows = child_rows;
       dev_sec += tdev2.sec();
-      size_t live = db.bytes + w.sup.cap + w.acc_rep.cap + w.freq.cap +
-                    w.rowcnt.cap + w.rowbsum.cap + w.rccnt.cap;
+      size_t live = db.bytes + w.sup.cap + w.acc.cap + w.freq.cap + w.rccnt.cap + w.cand.cap + w.chunks.cap +
+                    w.fail.cap + w.cursor.cap;
       for (int i = 0; i < 2; ++i)
-        live += w.pid[i].cap + w.rgraph[i].cap + w.vt[i].cap + w.upl[i].cap + w.cand[i].cap + w.rcoff[i].cap;
+        live += w.pid[i].cap + w.rgraph[i].cap + w.vt[i].cap + w.rbound[i].cap + w.upl[i].cap;
       peak = std::max(peak, live);
       cur = nxt;
-      max_nrm = next_nrm;
       mask_off = masks_off;
       mask_stride = masks && t2_ready ? mstride : 0;
     }
-    lists_now = lists_next;
     if (root)
       st.e_root += alive_rows;
     else
@@ -1279,23 +1088,35 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     }
     if (trace_levels)
       fprintf(stderr,
-              "[gm level %d] patterns %lld rows %lld K %lld rep %d block %d table_bits %d support_us %.1f "
-              "candidates %llu pruned %llu frequent %u children %lld child_rows %lld minimal %lld spec %d splits %llu "
-              "max_fill %llu lists %d "
-              "host_us prep %.1f canon %.1f select %.1f total %.1f overlap %.1f sync %.1f crit %.1f\n",
-              level, (long long)P, (long long)(root ? db.n_vertices : n_rows), (long long)K, rep, a.block,
-              a.table_bits, ms * 1e3, ecand, pruned, cnt.fcount, (long long)C, (long long)child_rows, (long long)alive,
-              spec ? 1 : 0, cnt.splits, max_fill,
-              a.use_lists, us_prep, us_canon, us_select, us_host, us_overlap, us_sync, us_crit);
+              "[gm level %d] patterns %lld rows %lld K %lld chunks %lld table_bits %d stage_us %.1f "
+              "candidates %llu pruned %llu frequent %u children %lld child_rows %lld minimal %lld spec %d "
+              "failed %llu max_fill %llu list_cap %llu host_us canon %.1f host %.1f overlap %.1f\n",
+              level, (long long)P, (long long)rows_now, (long long)K, (long long)total_chunks, table_bits, ms * 1e3,
+              ecand, pruned, cnt.fcount, (long long)C, (long long)child_rows, (long long)alive, spec ? 1 : 0, failed,
+              max_fill, (unsigned long long)list_cap, us_canon, us_host, us_overlap);
     st.n_levels++;
     if (stopped) break;
     std::swap(L, N);
     S = CS;
     n_rows = child_rows;
+    // the children's lists: at most the item counts of every listed
+    // candidate of this level (the selected ones need less)
+    list_cap = std::max<uint64_t>(cand_bound, 1);
     ++level;
   }
+  // the last write's row count
+  if (written_expect) {
+    Counters c;
+    hip_check(hipMemcpyAsync(&c, w.freq.p, sizeof(c), hipMemcpyDeviceToHost, s), "download counters");
+    hip_check(hipStreamSynchronize(s), "mining");
+    unsigned long long written = 0;
+    for (int i = 0; i < kSpread; ++i) written += c.spread[i * kSpreadStride + kSpreadWritten];
+    if (written != written_expect && !stopped)
+      throw Error(GM_E_DEVICE, "child rows written (" + std::to_string(written) +
+                                   ") differ from the embedding counts (" + std::to_string(written_expect) + ")");
+  }
   hip_check(hipStreamSynchronize(s), "mining");
-  w.replicas_zero = true;
+  w.acc_zero = true;
   st.seconds_total = total.sec();
   st.seconds_device = dev_sec;
   st.seconds_canon = canon_sec;

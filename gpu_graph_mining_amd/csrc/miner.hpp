@@ -47,24 +47,19 @@ struct HostBuf {
 // Device buffers of the level steps, kept across mining calls on one
 // database so that repeated calls allocate nothing once warm.
 struct Workspace {
-  DevBuf minsup, sup, acc_rep, freq, rowcnt, rowbsum, gbsum, rccnt;
-  // double-buffered by level: rows (gs/gc/pid/rgraph/vt), pattern metadata and
-  // selected keys (upl), candidate lists (cand/gcb/rcoff), big-graph lists
-  DevBuf gs[2], gc[2], pid[2], rgraph[2], vt[2], upl[2], cand[2], gcb[2], rcoff[2], big[2];
-  DevBuf nbig;  // big-graph counts of the two buffered levels
-  DevBuf scan_tiles;  // placement scan: per-tile sums and offsets
-  DevBuf wtot;        // row-parallel children passes: per-wave child rows, list entries
+  DevBuf minsup, sup, acc, freq, rccnt, cursor, fail, chunks, cand;
+  // double-buffered by level: rows (pid/rgraph/vt/rbound) and the pattern
+  // upload (metadata, path masks)
+  DevBuf pid[2], rgraph[2], vt[2], rbound[2], upl[2];
   HostBuf up, down;
-  HostBuf flag;                // the level download's completion word (k_signal)
   hipEvent_t ev[2] = {nullptr, nullptr};  // stage timing (MineOptions::stage_events)
+  // the packed counters hold zeros outside a level step (k_reduce clears
+  // what it reads); false after a call that ended early
+  bool acc_zero = false;
   ~Workspace() {
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
   }
-  unsigned long long seq = 0;  // its last value
-  // the replica counters hold zeros outside a level step (k_reduce clears
-  // what it reads); false after a call that ended early
-  bool replicas_zero = false;
 };
 
 // The packed database of device.hpp on the GPU, for one or more jobs.
@@ -75,6 +70,7 @@ struct DeviceDB {
   int32_t NVL = 0, NEL = 0;
   std::vector<int32_t> vl_values, el_values;  // dense label -> original
   std::vector<int32_t> job_first_graph;       // n_jobs + 1
+  std::vector<int64_t> job_first_vertex;      // n_jobs + 1 (the root level's rows of each job)
   DevBuf gvbase, vadj, adj, vlab, gjob, vgraph;
   size_t bytes = 0;
   // mining state; one mining call at a time per database
@@ -121,10 +117,8 @@ struct MineOptions {
   std::vector<int32_t> minsup;  // per job
   int32_t max_edges = 0;
   int32_t canon_threads = 0;
-  int32_t block_threads = 0;   // 0 = by rows per graph
   bool collect_proj = false;
-  bool candidate_lists = true;  // k_support stores candidates for k_write
-  bool path_pruning = true;     // k_expand drops candidates with an infrequent 2-edge sub-path
+  bool path_pruning = true;     // k_level drops candidates with an infrequent 2-edge sub-path
   int32_t speculate = 0;        // 0 adaptive, 1 never, 2 always (gm_cfg::speculate)
   bool stage_events = false;    // time the extension + support stages with HIP events
   // Query mode: instead of mining, follow the codes of `queries` (every
