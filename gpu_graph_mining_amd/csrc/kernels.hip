@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
   chunk_bounds<ROOT>(a, ci, start, end);
   ChunkInfo *info = a.chunks + a.chunk0 + ci;
   if (start >= end) {
-    if (lane == 0) *info = ChunkInfo{start, start, 0ull, 0u, 0u};
+    if (lane == 0) *info = ChunkInfo{start, start, 0ull, 0u, 0u, 0, {0u, 0u, 0u}};
     return;
   }
   // the chunk's list region: one atomic per chunk
@@ -328,11 +328,11 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
   if (lbase + items > a.list_cap) {  // sized by the host from the parent level's bound sum
     if (lane == 0) {
       atomicOr(a.error_flag, 2);
-      *info = ChunkInfo{start, start, 0ull, 0u, 0u};
+      *info = ChunkInfo{start, start, 0ull, 0u, 0u, 0, {0u, 0u, 0u}};
     }
     return;
   }
-  if (lane == 0) *info = ChunkInfo{start, end, lbase, 0u, 0u};
+  if (lane == 0) *info = ChunkInfo{start, end, lbase, 0u, 0u, a.p_lo, {0u, 0u, 0u}};
 
   const CountTable tab = count_table(dyn, a.table_bits, s_scal);
   for (uint32_t i = lane; i < tab.cap; i += 64) {
@@ -364,30 +364,41 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
     const uint32_t rpre = wave_exclusive_scan<uint32_t>(rb, btot);
     const unsigned long long lo = lbase + rpre;
     int cnt = 0;  // candidates listed for this row
+    // Each step a lane takes up to 4 items of its current source, their
+    // adjacency entries loaded together, then evaluates them in order.
     if (ROOT) {
-      int32_t a0 = 0, n = 0, lu = 0, job = 0;
+      int32_t ca = 0, ce = 0, lu = 0, job = 0, n = 0;
       if (live) {
         const int32_t g = a.vgraph[r];
         job = a.gjob[g];
         lu = a.vlab[r];
-        a0 = a.vadj[r];
+        ca = a.vadj[r];
         n = (int)rb;
+        ce = ca + n;
       }
       const uint32_t kb = (uint32_t)((int64_t)(job - a.p_lo) * a.K);
-      const int maxn = wave_max(n);
-      for (int i = 0; i < maxn; ++i) {
-        if (i < n) {
-          const uint32_t e = a.adj[a0 + i];
-          const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
-          if (lu <= nl) {
-            const int key = (lu * NEL + el) * NVL + nl;
-            const uint32_t bound = (uint32_t)n + (e >> 24);
-            a.cand[lo + cnt] = make_uint2(cand_code(true, key, w), bound);
-            ++cnt;
-            ++n_cand;
-            bsum += bound;
-            count_insert(tab, kb + (uint32_t)key, head);
-          }
+      auto eval = [&](uint32_t e) {
+        const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
+        if (lu <= nl) {
+          const int key = (lu * NEL + el) * NVL + nl;
+          const uint32_t bound = (uint32_t)n + (e >> 24);
+          a.cand[lo + cnt] = make_uint2(cand_code(true, key, w), bound);
+          ++cnt;
+          ++n_cand;
+          bsum += bound;
+          count_insert(tab, kb + (uint32_t)key, head);
+        }
+      };
+      while (__ballot(ca < ce)) {
+        if (ca < ce) {
+          const int c4 = min(4, ce - ca);
+          uint32_t e[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) e[u] = u < c4 ? a.adj[ca + u] : 0u;
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (u < c4) eval(e[u]);
+          ca += c4;
         }
       }
     } else {
@@ -409,7 +420,7 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
       const PatMeta *pm = a.meta + p;
       const unsigned long long *mkp = ms ? a.pmask + (int64_t)p * 2 * ms : nullptr;
       unsigned long long mk = ~0ull;
-      int n = 0;
+      const int n = (int)rb;
       if (work) {
         t.load(a.vt, r, a.S, nv);
         vb = a.gvbase[g];
@@ -424,79 +435,91 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
           ne = a.vadj[vb + u + 1];
         }
         if (mkp) mk = mkp[0];
-        n = (int)rb;
       }
       const uint32_t kb = (uint32_t)((int64_t)(p - a.p_lo) * a.K);
       const int maxtoc = nv - 1;
       int s = 0;      // current source: 0 the rightmost vertex, q >= 1 rm[q - 1].from
       int pre_s = 0;  // items before the current source
-      const int maxn = wave_max(n);
-      for (int i = 0; i < maxn; ++i) {
-        if (i < n) {
-          if (ca >= ce) {  // next source (each source has at least one item)
-            ++s;
-            if (s > nrm) {
-              err = 1;
-              n = i;  // stop this row
-              continue;
-            }
-            pre_s = i;
-            ca = na;
-            ce = ne;
-            rme = nrme;
-            if (mkp) mk = mkp[s];
-            if (s < nrm) {
-              nrme = pm->rm[s];
-              const int u = t.get(nrme & 0xFF);
-              na = a.vadj[vb + u];
-              ne = a.vadj[vb + u + 1];
-            }
-          }
-          const uint32_t e = a.adj[ca++];
-          const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
-          const uint32_t dw = e >> 24;
-          const int fw = t.find(w);  // pattern vertex already mapped to w, or -1
-          int key = -1, wout = 0;
-          uint32_t bound = 0;
-          bool keep = true;
-          if (s == 0) {
-            if (fw < 0) {
-              if (nl >= minlabel) {
-                key = fwd_base + (maxtoc * NEL + el) * NVL + nl;
-                wout = w;
-                bound = (uint32_t)n + dw;
-                keep = (mk >> (el * NVL + nl)) & 1ull;
-              }
-            } else if ((bwd_from >> fw) & 1u) {
-              // backward to a rightmost-path vertex f = rm[q].from, q >= 1
-              const int q = pm->rmq[fw];
-              const uint32_t rq = pm->rm[q];
-              const int e1 = (rq >> 8) & 0xFF, l1 = (rq >> 16) & 0xFF;
-              if (!((bwd_mask >> fw) & 1u) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
-                key = fw * NEL + el;
-                wout = -1;
-                bound = (uint32_t)n;
-                if (mkp) keep = (mkp[ms + 1 + q] >> el) & 1ull;
-              }
-            }
-          } else {
-            const int f = rme & 0xFF, e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
-            if (fw < 0 && nl >= minlabel && (e1 < el || (e1 == el && l1 <= nl))) {
-              key = fwd_base + (f * NEL + el) * NVL + nl;
+      int32_t cb = ca;  // first item of the current source
+      auto eval = [&](uint32_t e) {
+        const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
+        const uint32_t dw = e >> 24;
+        const int fw = t.find(w);  // pattern vertex already mapped to w, or -1
+        int key = -1, wout = 0;
+        uint32_t bound = 0;
+        bool keep = true;
+        if (s == 0) {
+          if (fw < 0) {
+            if (nl >= minlabel) {
+              key = fwd_base + (maxtoc * NEL + el) * NVL + nl;
               wout = w;
-              bound = (uint32_t)(n - pre_s) + dw;  // sources s - 1 .. root, plus w
+              bound = (uint32_t)n + dw;
               keep = (mk >> (el * NVL + nl)) & 1ull;
             }
+          } else if ((bwd_from >> fw) & 1u) {
+            // backward to a rightmost-path vertex f = rm[q].from, q >= 1
+            const int q = pm->rmq[fw];
+            const uint32_t rq = pm->rm[q];
+            const int e1 = (rq >> 8) & 0xFF, l1 = (rq >> 16) & 0xFF;
+            if (!((bwd_mask >> fw) & 1u) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
+              key = fw * NEL + el;
+              wout = -1;
+              bound = (uint32_t)n;
+              if (mkp) keep = (mkp[ms + 1 + q] >> el) & 1ull;
+            }
           }
-          if (key >= 0) {
-            ++n_cand;
-            if (!keep) {
-              ++n_pruned;  // counted as a candidate, never listed
+        } else {
+          const int f = rme & 0xFF, e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
+          if (fw < 0 && nl >= minlabel && (e1 < el || (e1 == el && l1 <= nl))) {
+            key = fwd_base + (f * NEL + el) * NVL + nl;
+            wout = w;
+            bound = (uint32_t)(n - pre_s) + dw;  // sources s - 1 .. root, plus w
+            keep = (mk >> (el * NVL + nl)) & 1ull;
+          }
+        }
+        if (key >= 0) {
+          ++n_cand;
+          if (!keep) {
+            ++n_pruned;  // counted as a candidate, never listed
+          } else if (cnt < n) {
+            a.cand[lo + cnt] = make_uint2(cand_code(false, key, wout), bound);
+            ++cnt;
+            bsum += bound;
+            count_insert(tab, kb + (uint32_t)key, head);
+          } else {
+            err = 1;  // more candidates than items: the row's item count is wrong
+          }
+        }
+      };
+      bool more = work && ca < ce;
+      while (__ballot(more)) {
+        if (more) {
+          const int c4 = min(4, ce - ca);
+          uint32_t e[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) e[u] = u < c4 ? a.adj[ca + u] : 0u;
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (u < c4) eval(e[u]);
+          ca += c4;
+          if (ca >= ce) {  // next source (each source has at least one item)
+            pre_s += ce - cb;
+            ++s;
+            if (s > nrm) {
+              more = false;
+              if (pre_s != n) err = 1;  // the row's item count disagrees with its sources
             } else {
-              a.cand[lo + cnt] = make_uint2(cand_code(false, key, wout), bound);
-              ++cnt;
-              bsum += bound;
-              count_insert(tab, kb + (uint32_t)key, head);
+              ca = cb = na;
+              ce = ne;
+              rme = nrme;
+              if (mkp) mk = mkp[s];
+              if (s < nrm) {
+                nrme = pm->rm[s];
+                const int u = t.get(nrme & 0xFF);
+                na = a.vadj[vb + u];
+                ne = a.vadj[vb + u + 1];
+              }
+              more = ca < ce;
             }
           }
         }
@@ -512,12 +535,22 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
   }
   const bool failed = s_scal[1] != 0;
   if (!failed) {
-    for (uint32_t i = lane; i < tab.cap; i += 64) {
+    // commit the counts; keep the chunk's (key, embeddings) records
+    uint2 *rec = a.chunk_keys + (int64_t)(a.chunk0 + ci) * a.chunk_key_cap;
+    uint32_t nrec = 0;
+    for (uint32_t i0 = 0; i0 < tab.cap; i0 += 64) {
+      const uint32_t i = i0 + lane;
       const uint32_t k = tab.key[i];
-      if (k == kEmpty32) continue;
-      atomicAdd(&a.acc[(int64_t)a.p_lo * a.K + (int64_t)k],
-                ((unsigned long long)tab.sup[i] << kSupShift) | (unsigned long long)tab.emb[i]);
+      const bool used = k != kEmpty32;
+      const uint64_t m = __ballot(used);
+      if (used) {
+        atomicAdd(&a.acc[(int64_t)a.p_lo * a.K + (int64_t)k],
+                  ((unsigned long long)tab.sup[i] << kSupShift) | (unsigned long long)tab.emb[i]);
+        rec[nrec + (uint32_t)__popcll(m & lanes_below(lane))] = make_uint2(k, tab.emb[i]);
+      }
+      nrec += (uint32_t)__popcll(m);
     }
+    if (lane == 0) info->nkeys = nrec;
   } else if (lane == 0) {
     const uint32_t at = atomicAdd(a.fail_count, 1u);
     a.fail_list[at] = (uint32_t)(a.chunk0 + ci);
@@ -682,16 +715,20 @@ __global__ void __launch_bounds__(256) k_reduce(StepArgs a) {
 }
 
 // ------------------------------------------------------------------ WRITE
-// One wave per chunk (the chunks k_level recorded).  A pass over the
-// children c with c mod `mod` == res: count the chunk's candidates per
-// child in an LDS table (lane per row), reserve each child's rows with one
-// atomic on its cursor, then write them with the candidates taken 64 at a
-// time in list order (rows in order, each row's list in order): a lane's
-// position is its child's reserved base plus the child's running count
-// plus its rank among the window's lanes of the same child.  The child's
-// rows of one chunk are thus consecutive and in the parent's row order,
-// so every child's rows stay graph-contiguous.  A pass whose table fills up
-// is split in two by child residue before anything is reserved.
+// One wave per chunk (the chunks k_level recorded).  Each child's rows of
+// the chunk are reserved with one atomic on the child's cursor -- the
+// counts come from the chunk's (key, embeddings) records -- then written
+// with the candidates taken 64 at a time in list order (rows in order, each
+// row's list in order): a lane's position is its child's reserved base plus
+// the child's running count plus its rank among the window's lanes of the
+// same child.  The child's rows of one chunk are thus consecutive and in the
+// parent's row order, so every child's rows stay graph-contiguous.
+// A chunk that k_level_fix recounted has no records: it is counted here, a
+// pass over the children c with c mod `mod` == res at a time (lane per row),
+// and a pass whose table fills up is split in two by child residue before
+// anything is reserved.
+constexpr int kWriteUnroll = 2;  // candidate windows whose loads are issued together
+
 template <bool ROOT, int TW>
 __global__ void __launch_bounds__(64) k_write(StepArgs a) {
   constexpr uint32_t cap = 1u << kWriteTableBits;
@@ -708,6 +745,7 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
   const int lane = threadIdx.x;
   const ChunkInfo c = a.chunks[blockIdx.x];
   if (c.start >= c.end) return;
+  bool counted = !(c.flags & kChunkFailed);
   const int CW = a.CS >> 2;
   unsigned long long written = 0;
   int top = 1;
@@ -717,8 +755,8 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
   }
   wave_sync();
   // child selected by the candidate (key of pattern / job p), or -1
-  auto child_of = [&](int p, int key) -> int32_t {
-    const int32_t v = a.sup[(int64_t)p * a.K + key];
+  auto child_of = [&](int64_t pk) -> int32_t {
+    const int32_t v = a.sup[pk];
     return v < 0 ? -v - 1 : -1;
   };
   auto lookup = [&](uint32_t ch) -> uint32_t {
@@ -728,6 +766,20 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
       h = (h + 1) & (cap - 1);
     }
     return 0;  // unreachable: every child of the pass was counted
+  };
+  // add n rows of child ch to the table (false: the table is full)
+  auto table_add = [&](uint32_t ch, uint32_t n) -> bool {
+    uint32_t h = hash_slot32(ch, kWriteTableBits);
+    for (uint32_t probe = 0; probe < cap; ++probe) {
+      const uint32_t old = atomicCAS(&ckey[h], kEmpty32, ch);
+      if (old == kEmpty32 || old == ch) {
+        atomicAdd(&ccnt[h], n);
+        if (old == kEmpty32 && (uint32_t)atomicAdd(&s_fill, 1) >= cap - cap / 8) return false;
+        return true;
+      }
+      h = (h + 1) & (cap - 1);
+    }
+    return false;
   };
   while (top > 0) {
     const uint32_t mod = stack_mod[top - 1], res = stack_res[top - 1];
@@ -741,46 +793,50 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
       s_over = 0;
     }
     wave_sync();
-    // count
-    unsigned long long lbase = c.list_base;
-    for (int64_t b = c.start; b < c.end;) {
-      const int nb = (int)min<int64_t>(64, c.end - b);
-      const int64_t r = b + lane;
-      const bool live = lane < nb;
-      const uint32_t rb = live ? row_items<ROOT>(a, r) : 0u;
-      uint32_t btot;
-      const unsigned long long lo = lbase + wave_exclusive_scan<uint32_t>(rb, btot);
-      int n = 0, p = 0;
-      if (live) {
-        n = a.rccnt[r];
-        p = ROOT ? a.gjob[a.vgraph[r]] : a.pid[r];
+    if (counted) {
+      // the chunk's records: one per key, so one per child
+      const uint2 *rec = a.chunk_keys + (int64_t)blockIdx.x * a.chunk_key_cap;
+      for (uint32_t i = lane; i < c.nkeys; i += 64) {
+        const uint2 kr = rec[i];
+        const int32_t ch = child_of((int64_t)c.p_lo * a.K + (int64_t)kr.x);
+        if (ch >= 0 && !table_add((uint32_t)ch, kr.y)) atomicOr(&s_over, 1);
       }
-      const int maxn = wave_max(n);
-      for (int j = 0; j < maxn; ++j) {
-        if (j >= n) continue;
-        int key, w;
-        cand_decode(ROOT, a.cand[lo + j].x, key, w);
-        const int32_t ch = child_of(p, key);
-        if (ch < 0 || (mod > 1 && ((uint32_t)ch & (mod - 1)) != res)) continue;
-        if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
-        uint32_t h = hash_slot32((uint32_t)ch, kWriteTableBits);
-        bool done = false;
-        for (uint32_t probe = 0; probe < cap && !done; ++probe) {
-          const uint32_t old = atomicCAS(&ckey[h], kEmpty32, (uint32_t)ch);
-          if (old == kEmpty32 || old == (uint32_t)ch) {
-            atomicAdd(&ccnt[h], 1u);
-            if (old == kEmpty32 && (uint32_t)atomicAdd(&s_fill, 1) >= cap - cap / 4) atomicOr(&s_over, 1);
-            done = true;
-          }
-          h = (h + 1) & (cap - 1);
+    } else {
+      // count: lane per row
+      unsigned long long lbase = c.list_base;
+      for (int64_t b = c.start; b < c.end;) {
+        const int nb = (int)min<int64_t>(64, c.end - b);
+        const int64_t r = b + lane;
+        const bool live = lane < nb;
+        const uint32_t rb = live ? row_items<ROOT>(a, r) : 0u;
+        uint32_t btot;
+        const unsigned long long lo = lbase + wave_exclusive_scan<uint32_t>(rb, btot);
+        int n = 0, p = 0;
+        if (live) {
+          n = a.rccnt[r];
+          p = ROOT ? a.gjob[a.vgraph[r]] : a.pid[r];
         }
-        if (!done) atomicOr(&s_over, 1);
+        const int maxn = wave_max(n);
+        for (int j = 0; j < maxn; ++j) {
+          if (j >= n) continue;
+          int key, w;
+          cand_decode(ROOT, a.cand[lo + j].x, key, w);
+          const int32_t ch = child_of((int64_t)p * a.K + key);
+          if (ch < 0 || (mod > 1 && ((uint32_t)ch & (mod - 1)) != res)) continue;
+          if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
+          if (!table_add((uint32_t)ch, 1u)) atomicOr(&s_over, 1);
+        }
+        lbase += btot;
+        b += nb;
       }
-      lbase += btot;
-      b += nb;
     }
     wave_sync();
     if (s_over) {
+      if (counted) {  // more children than the table holds: count them in split passes
+        counted = false;
+        wave_sync();
+        continue;
+      }
       if (mod >= (1u << 30) || top + 1 > 32) {  // cannot happen: a pass of one child fits
         if (lane == 0) atomicOr(a.error_flag, 8);
         break;
@@ -800,7 +856,7 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
       if (ckey[i] != kEmpty32) cbase[i] = atomicAdd(&a.cursor[ckey[i]], (unsigned long long)ccnt[i]);
     wave_sync();
     // write, candidates 64 at a time in list order
-    lbase = c.list_base;
+    unsigned long long lbase = c.list_base;
     for (int64_t b = c.start; b < c.end;) {
       const int nb = (int)min<int64_t>(64, c.end - b);
       const int64_t r = b + lane;
@@ -819,10 +875,12 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
         } else {
           p = a.pid[r];
           g = a.rgraph[r];
-          nvp = a.meta[p].nv;
-          const uint32_t *src = reinterpret_cast<const uint32_t *>(a.vt + r * a.S);
+          if (n) {
+            nvp = a.meta[p].nv;
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(a.vt + r * a.S);
 #pragma unroll
-          for (int q = 0; q < TW; ++q) s_tup[lane * TW + q] = src[q];
+            for (int q = 0; q < TW; ++q) s_tup[lane * TW + q] = src[q];
+          }
         }
       }
       int T;
@@ -833,61 +891,79 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
       s_g[lane] = g;
       s_nvp[lane] = nvp;
       wave_sync();
-      for (int jb = 0; jb < T; jb += 64) {
-        const int j = jb + lane;
-        int k = 0, key = 0, w = 0;
-        uint32_t bound = 0;
-        int32_t ch = -1;
-        if (j < T) {
-          int lo_k = 0, hi_k = nb;  // last row k with s_pre[k] <= j
-          while (hi_k - lo_k > 1) {
-            const int mid = (lo_k + hi_k) >> 1;
-            if (s_pre[mid] <= j) lo_k = mid;
-            else hi_k = mid;
+      for (int jb0 = 0; jb0 < T; jb0 += 64 * kWriteUnroll) {
+        int kk[kWriteUnroll];
+        uint2 ent[kWriteUnroll];
+        int32_t chv[kWriteUnroll];
+#pragma unroll
+        for (int u = 0; u < kWriteUnroll; ++u) {
+          const int j = jb0 + u * 64 + lane;
+          kk[u] = 0;
+          ent[u] = make_uint2(0, 0);
+          if (j < T) {
+            int lo_k = 0, hi_k = nb;  // last row k with s_pre[k] <= j
+            while (hi_k - lo_k > 1) {
+              const int mid = (lo_k + hi_k) >> 1;
+              if (s_pre[mid] <= j) lo_k = mid;
+              else hi_k = mid;
+            }
+            kk[u] = lo_k;
+            ent[u] = a.cand[s_lo[lo_k] + (unsigned long long)(j - s_pre[lo_k])];
           }
-          k = lo_k;
-          const uint2 e = a.cand[s_lo[k] + (unsigned long long)(j - s_pre[k])];
-          cand_decode(ROOT, e.x, key, w);
-          bound = e.y;
-          ch = child_of(s_p[k], key);
-          if (ch >= 0 && mod > 1 && ((uint32_t)ch & (mod - 1)) != res) ch = -1;
         }
-        const uint32_t h = ch >= 0 ? lookup((uint32_t)ch) : 0u;
-        uint64_t active = __ballot(ch >= 0);
-        while (active) {
-          const int leader = __ffsll((long long)active) - 1;
-          const int32_t cl = __shfl(ch, leader, 64);
-          const uint64_t m = __ballot(ch == cl);
-          if (ch == cl) {
-            const unsigned long long pos =
-                cbase[h] + (unsigned long long)crun[h] + (unsigned long long)__popcll(m & lanes_below(lane));
-            if ((int64_t)pos < a.ccap) {
-              a.cpid[pos] = ch;
-              a.crgraph[pos] = s_g[k];
-              a.cbound[pos] = bound;
-              uint32_t *dst = reinterpret_cast<uint32_t *>(a.cvt + pos * a.CS);
-              if (ROOT) {
-                dst[0] = (uint32_t)s_nvp[k] | ((uint32_t)w << 8);
-                for (int q = 1; q < CW; ++q) dst[q] = 0;
-              } else {
-                const int nvp = s_nvp[k];
-                for (int q = 0; q < CW; ++q) {
-                  uint32_t word = q < TW ? s_tup[k * TW + q] : 0u;
-                  if (w >= 0 && (nvp >> 2) == q) {
-                    const int sh = (nvp & 3) * 8;
-                    word = (word & ~(0xFFu << sh)) | ((uint32_t)w << sh);
+#pragma unroll
+        for (int u = 0; u < kWriteUnroll; ++u) {
+          chv[u] = -1;
+          if (jb0 + u * 64 + lane < T) {
+            int key, w;
+            cand_decode(ROOT, ent[u].x, key, w);
+            chv[u] = child_of((int64_t)s_p[kk[u]] * a.K + key);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kWriteUnroll; ++u) {
+          const int k = kk[u];
+          int key, w;
+          cand_decode(ROOT, ent[u].x, key, w);
+          int32_t ch = chv[u];
+          if (ch >= 0 && mod > 1 && ((uint32_t)ch & (mod - 1)) != res) ch = -1;
+          const uint32_t h = ch >= 0 ? lookup((uint32_t)ch) : 0u;
+          uint64_t active = __ballot(ch >= 0);
+          written += (unsigned long long)__popcll(active);
+          while (active) {
+            const int leader = __ffsll((long long)active) - 1;
+            const int32_t cl = __shfl(ch, leader, 64);
+            const uint64_t m = __ballot(ch == cl);
+            if (ch == cl) {
+              const unsigned long long pos =
+                  cbase[h] + (unsigned long long)crun[h] + (unsigned long long)__popcll(m & lanes_below(lane));
+              if ((int64_t)pos < a.ccap) {
+                a.cpid[pos] = ch;
+                a.crgraph[pos] = s_g[k];
+                a.cbound[pos] = ent[u].y;
+                uint32_t *dst = reinterpret_cast<uint32_t *>(a.cvt + pos * a.CS);
+                if (ROOT) {
+                  dst[0] = (uint32_t)s_nvp[k] | ((uint32_t)w << 8);
+                  for (int q = 1; q < CW; ++q) dst[q] = 0;
+                } else {
+                  const int nvp = s_nvp[k];
+                  for (int q = 0; q < CW; ++q) {
+                    uint32_t word = q < TW ? s_tup[k * TW + q] : 0u;
+                    if (w >= 0 && (nvp >> 2) == q) {
+                      const int sh = (nvp & 3) * 8;
+                      word = (word & ~(0xFFu << sh)) | ((uint32_t)w << sh);
+                    }
+                    dst[q] = word;
                   }
-                  dst[q] = word;
                 }
               }
             }
+            wave_sync();
+            if (lane == leader) crun[h] += (uint32_t)__popcll(m);
+            wave_sync();
+            active &= ~m;
           }
-          wave_sync();
-          if (lane == leader) crun[h] += (uint32_t)__popcll(m);
-          wave_sync();
-          active &= ~m;
         }
-        written += (unsigned long long)__popcll(__ballot(ch >= 0));
       }
       wave_sync();  // the next block reuses the staged rows
       lbase += btot;
@@ -961,6 +1037,8 @@ hipError_t level_tw(const StepArgs &a, hipStream_t s, size_t lds) {
 
 // ------------------------------------------------------------ launchers
 size_t level_lds_bytes(int table_bits) { return (size_t)20 << table_bits; }
+// the table admits keys up to its fill limit
+int level_chunk_keys(int table_bits) { return (1 << table_bits) - (1 << table_bits) / 8; }
 
 hipError_t launch_level(const StepArgs &a, bool root, hipStream_t s) {
   if (a.n_chunks <= 0) return hipSuccess;

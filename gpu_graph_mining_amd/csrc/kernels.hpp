@@ -33,7 +33,9 @@ struct alignas(16) ChunkInfo {
   int64_t start, end;
   unsigned long long list_base;  // first list entry of the chunk's rows
   uint32_t flags;                // kChunkFailed
-  uint32_t pad;
+  uint32_t nkeys;                // (key, embeddings) records the chunk left for k_write
+  int32_t p_lo;                  // their keys are (pattern - p_lo) * K + key
+  uint32_t pad[3];
 };
 constexpr uint32_t kChunkFailed = 1;  // the LDS count table overflowed; k_level_fix counted it
 
@@ -77,6 +79,11 @@ struct StepArgs {
   // packed counters acc[p * K + key] = support << 40 | embeddings
   unsigned long long *acc;
   int32_t table_bits;  // per-wave LDS count table of k_level
+  // per chunk, chunk_key_cap records {(pattern - p_lo) * K + key,
+  // embeddings}: the chunk's candidates per key, so that k_write reserves
+  // the children's rows without counting them again
+  uint2 *chunk_keys;
+  int32_t chunk_key_cap;
   uint32_t *fail_list, *fail_count;  // chunks whose count table overflowed
   unsigned long long *spread;
   int32_t *error_flag;
@@ -97,8 +104,10 @@ struct StepArgs {
   int64_t ccap;
 };
 
-// Dynamic LDS of k_level for table_bits.
+// Dynamic LDS of k_level for table_bits, and the per-chunk key records its
+// table leaves at most.
 size_t level_lds_bytes(int table_bits);
+int level_chunk_keys(int table_bits);
 hipError_t launch_level(const StepArgs &a, bool root, hipStream_t s);
 // Recount the chunks k_level listed as failed (persistent grid; cheap when
 // none failed).

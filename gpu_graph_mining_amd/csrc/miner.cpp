@@ -617,6 +617,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     grow_zero(w.acc, sizeof(unsigned long long) * std::max<int64_t>(nkeys, 1), s);
     grow(w.chunks, sizeof(ChunkInfo) * std::max<int64_t>(total_chunks, 1), s);
     grow(w.fail, sizeof(uint32_t) * std::max<int64_t>(total_chunks, 1), s);
+    grow(w.chunk_keys, sizeof(uint2) * level_chunk_keys(table_bits) * std::max<int64_t>(total_chunks, 1), s);
     grow(w.rccnt, sizeof(uint16_t) * std::max<int64_t>(rows_now, 1), s);
     grow(w.cand, sizeof(uint2) * std::max<uint64_t>(list_cap, 1), s);
     uint32_t fcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(nkeys, 1), std::max<int64_t>(1 << 20, P * 64));
@@ -629,7 +630,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // the stage; rerun with a longer frequent-key list when it overflowed
     for (int attempt = 0;; ++attempt) {
       const size_t freq_bytes = sizeof(Counters) + 3 * sizeof(unsigned long long) * fcap;
-      if (grow(w.freq, freq_bytes, s) != (const void *)dctr_last) counters_fresh = true;  // a new block
+      {
+        // a new block (the allocator may hand back the address it freed)
+        const size_t cap0 = w.freq.cap;
+        if (grow(w.freq, freq_bytes, s) != (const void *)dctr_last || w.freq.cap != cap0) counters_fresh = true;
+      }
       dctr = w.freq.as<Counters>();
       if (counters_fresh) {
         // written rows of an earlier level live in the old block: checked already
@@ -647,6 +652,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.fail_count = &dctr->nfail;
       a.acc = w.acc.as<unsigned long long>();
       a.table_bits = table_bits;
+      a.chunk_keys = w.chunk_keys.as<uint2>();
+      a.chunk_key_cap = level_chunk_keys(table_bits);
       a.spread = dctr->spread;
       a.error_flag = &dctr->err;
       a.sup = w.sup.as<int32_t>();
@@ -1058,7 +1065,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       us_overlap = t_over.sec() * 1e6;
       dev_sec += tdev2.sec();
       size_t live = db.bytes + w.sup.cap + w.acc.cap + w.freq.cap + w.rccnt.cap + w.cand.cap + w.chunks.cap +
-                    w.fail.cap + w.cursor.cap;
+                    w.fail.cap + w.cursor.cap + w.chunk_keys.cap;
       for (int i = 0; i < 2; ++i)
         live += w.pid[i].cap + w.rgraph[i].cap + w.vt[i].cap + w.rbound[i].cap + w.upl[i].cap;
       peak = std::max(peak, live);

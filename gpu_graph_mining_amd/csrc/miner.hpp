@@ -47,7 +47,7 @@ struct HostBuf {
 // Device buffers of the level steps, kept across mining calls on one
 // database so that repeated calls allocate nothing once warm.
 struct Workspace {
-  DevBuf minsup, sup, acc, freq, rccnt, cursor, fail, chunks, cand;
+  DevBuf minsup, sup, acc, freq, rccnt, cursor, fail, chunks, chunk_keys, cand;
   // double-buffered by level: rows (pid/rgraph/vt/rbound) and the pattern
   // upload (metadata, path masks)
   DevBuf pid[2], rgraph[2], vt[2], rbound[2], upl[2];
