@@ -173,9 +173,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--cpu-root-stride", type=int, default=0,
-                    help="CPU baseline sample: grow every Nth 1-edge root of the oracle's pattern tree "
-                         "(0: by workload -- the whole run for config 1, every 6th root for config 2)")
+    ap.add_argument("--cpu-root-stride", type=int, default=1,
+                    help="CPU baseline sample: grow every Nth 1-edge root of the oracle's pattern tree, each subtree "
+                         "in full (1: the whole workload, about 40 s on one EPYC core for config 2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--jobs", type=int, default=256,
                     help="batching: the first N jobs of the 256-job mix, batched vs one at a time (0: skip)")
@@ -269,7 +269,7 @@ def main() -> int:
     if not args.no_cpu:
         from oracle import py_oracle as po
         arrays = db.arrays()
-        stride = args.cpu_root_stride or (6 if args.config == 2 else 1)
+        stride = args.cpu_root_stride or 1
         if stride == 1:
             opats, ost = po.mine_arrays(arrays, minsup)
             sample = "the whole workload"

@@ -54,7 +54,7 @@ class gm_cfg(C.Structure):
         ("label_style", C.c_int32), ("max_edges", C.c_int32), ("canon_threads", C.c_int32),
         ("block_threads", C.c_int32), ("mem_budget_bytes", C.c_int64), ("collect_proj", C.c_int32),
         ("candidate_lists", C.c_int32), ("path_pruning", C.c_int32), ("speculate", C.c_int32), ("stage_events", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("chunk_rows", C.c_int32), ("table_bits", C.c_int32), ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -418,7 +418,8 @@ class graph_collecting_output(graph_output):
 
 # ------------------------------------------------------------------- mining
 def _make_cfg(label_style=GM_LABELS_FULL, max_edges=0, canon_threads=0, collect_proj=False, candidate_lists=True,
-              block_threads=0, path_pruning=True, speculate=0, stage_events=False) -> gm_cfg:
+              block_threads=0, path_pruning=True, speculate=0, stage_events=False, chunk_rows=0,
+              table_bits=0) -> gm_cfg:
     cfg = gm_cfg()
     load_library().gm_cfg_init(C.byref(cfg))
     cfg.label_style = label_style
@@ -430,6 +431,8 @@ def _make_cfg(label_style=GM_LABELS_FULL, max_edges=0, canon_threads=0, collect_
     cfg.path_pruning = 0 if path_pruning else 1
     cfg.speculate = speculate
     cfg.stage_events = 1 if stage_events else 0
+    cfg.chunk_rows = chunk_rows
+    cfg.table_bits = table_bits
     return cfg
 
 
@@ -437,7 +440,7 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
          collect_proj: bool = False, canon_threads: int = 0,
          on_pattern: Optional[Callable[[DFSCode, int, int], Optional[bool]]] = None, count_only: bool = False,
          candidate_lists: bool = True, block_threads: int = 0, path_pruning: bool = True,
-         speculate: int = 0, stage_events: bool = False):
+         speculate: int = 0, stage_events: bool = False, chunk_rows: int = 0, table_bits: int = 0):
     """Mine db on the GPU.  Returns (patterns, stats) where patterns is a
     list of (DFSCode, support, |Proj|) in emission order (empty when
     on_pattern is given; it then receives each pattern, and returning False
@@ -463,7 +466,7 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
 
     ccb = PATTERN_CB() if count_only else PATTERN_CB(cb)  # PATTERN_CB() is a NULL pointer
     cfg = _make_cfg(label_style, max_edges, canon_threads, collect_proj, candidate_lists, block_threads,
-                    path_pruning, speculate, stage_events)
+                    path_pruning, speculate, stage_events, chunk_rows, table_bits)
     st = gm_stats()
     _check(lib.gm_mine(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st)))
     return out, st.as_dict()

@@ -103,6 +103,16 @@ gm::MineOptions options(const gm_cfg *cfg) {
     if (cfg->speculate < 0 || cfg->speculate > 2) throw gm::Error(GM_E_ARG, "speculate must be 0, 1 or 2");
     o.speculate = cfg->speculate;
     o.stage_events = cfg->stage_events != 0;
+    if (cfg->chunk_rows != 0 &&
+        (cfg->chunk_rows < 64 || cfg->chunk_rows > 4096 || (cfg->chunk_rows & (cfg->chunk_rows - 1))))
+      throw gm::Error(GM_E_ARG, "chunk_rows must be 0 or a power of two in [64, 4096]");
+    if (cfg->table_bits != 0 && (cfg->table_bits < 7 || cfg->table_bits > 11))
+      throw gm::Error(GM_E_ARG, "table_bits must be 0 or in [7, 11]");
+    // (k_write takes a chunk's records in one table of 2^8 entries; larger
+    // k_level tables let chunks leave more, which k_write then counts in
+    // split passes)
+    o.chunk_rows = cfg->chunk_rows;
+    o.table_bits = cfg->table_bits;
     if (cfg->block_threads != 0 && cfg->block_threads != 64 && cfg->block_threads != 128 &&
         cfg->block_threads != 256)
       throw gm::Error(GM_E_ARG, "block_threads must be 0, 64, 128 or 256");

@@ -9,7 +9,7 @@
 // consecutive segments (a chunk) touches only a few patterns' keys.
 //
 // Per level:
-//   k_level      one wave per chunk of rows (about kChunkRows, moved forward
+//   k_level      one wave per chunk of rows (chunk_rows of them, moved forward
 //                to segment starts): every lane enumerates one row's
 //                extension candidates, appends them to the row's list and
 //                counts them in a per-wave LDS table keyed by (pattern,
@@ -17,15 +17,20 @@
 //                through a 64-bit mask of the block's segments per entry.
 //                The table is committed to the packed device counters once
 //                per chunk -- one device atomic per (chunk, key) instead of
-//                one per (graph, key).  A chunk whose table fills up is
+//                one per (graph, key) -- into replica (chunk mod R) of the
+//                counters, so that keys every chunk hits (the root's) are
+//                not one hot address.  A chunk whose table fills up is
 //                listed and recounted by k_level_fix from its lists.
-//   k_reduce     reads the counters (and clears them), lists the frequent
-//                keys;
-//   k_write      (after the host chose the children) one wave per chunk:
-//                counts the chunk's rows per child, reserves them with one
-//                atomic per (chunk, child) on the child's row cursor, and
-//                writes them in list order, so the child's rows stay
-//                pattern-major and graph-contiguous.
+//   k_reduce     sums the replicas of every key, lists the frequent keys
+//                (clearing the others);
+//   k_cursors    (after the host chose the children) points the chosen
+//                keys at their child ids and gives every (child, replica)
+//                its row and list sub-range from the replica's counts,
+//                clearing the frequent keys' replicas;
+//   k_write      one wave per chunk: reserves each child's rows of the chunk
+//                with one atomic on the (child, replica) cursors, and writes
+//                them in list order, so the child's rows stay pattern-major
+//                and graph-contiguous, each with its list offset.
 //
 // Extension rules (pattern side in PatMeta, graph side from the packed
 // adjacency):
@@ -66,8 +71,8 @@ namespace {
 
 constexpr uint32_t kEmpty32 = 0xFFFFFFFFu;
 constexpr int kSupShift = 40;  // packed counters: support << 40 | embeddings
-constexpr int kWriteTableBits = 9;  // k_write: children per chunk in one pass
-constexpr int kFixTableBits = 12;   // k_level_fix: (segment, key) pairs per pass
+constexpr int kWriteTableBits = 8;  // k_write: children per chunk in one pass (>= k_level's table)
+constexpr int kFixTableBits = 11;   // k_level_fix: (segment, key) pairs per pass
 
 // ----------------------------------------------------------- wave helpers
 __device__ __forceinline__ void wave_sync() {
@@ -161,18 +166,26 @@ __device__ __forceinline__ int64_t seg_at_or_after(const StepArgs &a, int64_t x)
 // Chunk ci of this launch: [start, end) moved forward to segment starts.
 template <bool ROOT>
 __device__ __forceinline__ void chunk_bounds(const StepArgs &a, int64_t ci, int64_t &start, int64_t &end) {
-  const int64_t nom0 = a.row_lo + ci * kChunkRows;
-  const int64_t nom1 = min(a.row_hi, nom0 + kChunkRows);
+  const int64_t nom0 = a.row_lo + ci * a.chunk_rows;
+  const int64_t nom1 = min(a.row_hi, nom0 + a.chunk_rows);
   start = seg_at_or_after<ROOT>(a, nom0);
   end = nom1 >= a.row_hi ? a.row_hi : seg_at_or_after<ROOT>(a, nom1);
 }
 
-// Items of row r (its list region): root, the vertex's degree; later
-// levels, the count the parent level recorded.
+// Items of row r (the size of its list region): root, the vertex's
+// degree; later levels, the count the parent level recorded.
 template <bool ROOT>
 __device__ __forceinline__ uint32_t row_items(const StepArgs &a, int64_t r) {
   if (ROOT) return (uint32_t)(a.vadj[r + 1] - a.vadj[r]);
   return a.rbound[r];
+}
+
+// First list entry of row r: root, the vertex's adjacency offset; later
+// levels, the offset the parent level's k_write gave the row.
+template <bool ROOT>
+__device__ __forceinline__ unsigned long long row_list(const StepArgs &a, int64_t r) {
+  if (ROOT) return (unsigned long long)a.vadj[r];
+  return a.rlist[r];
 }
 
 // Block of the chunk starting at row b (< end): up to 64 rows, cut at the
@@ -209,15 +222,17 @@ __device__ __forceinline__ int head_lane(uint64_t starts, int lane) {
 template <int TW>
 struct Tuple {
   uint32_t w[TW > 0 ? TW : 1];
-  uint32_t live[TW > 0 ? TW : 1];  // 0x80 in each byte below nv
-  __device__ __forceinline__ void load(const uint8_t *vt, int64_t r, int S, int nv) {
+  int nv;
+  __device__ __forceinline__ void load(const uint8_t *vt, int64_t r, int S, int nv_) {
     const uint32_t *p = reinterpret_cast<const uint32_t *>(vt + r * S);
+    nv = nv_;
 #pragma unroll
-    for (int q = 0; q < TW; ++q) {
-      w[q] = p[q];
-      const int valid = min(4, max(0, nv - 4 * q));
-      live[q] = valid >= 4 ? 0x80808080u : (0x80808080u & ((1u << (8 * valid)) - 1u));
-    }
+    for (int q = 0; q < TW; ++q) w[q] = p[q];
+  }
+  // 0x80 in each byte of word q below nv (computed, not kept: registers)
+  __device__ __forceinline__ uint32_t live(int q) const {
+    const int valid = nv - 4 * q;
+    return valid >= 4 ? 0x80808080u : valid <= 0 ? 0u : (0x80808080u >> (8 * (4 - valid)));
   }
   // byte i: masks instead of selects, so that the compiler does not turn
   // the words into an indexed (scratch) array
@@ -237,7 +252,7 @@ struct Tuple {
 #pragma unroll
     for (int q = TW - 1; q >= 0; --q) {
       const uint32_t x = w[q] ^ pat;
-      const uint32_t z = (x - 0x01010101u) & ~x & live[q];
+      const uint32_t z = (x - 0x01010101u) & ~x & live(q);
       if (z) f = 4 * q + (__builtin_ctz(z) >> 3);
     }
     return f;
@@ -249,7 +264,7 @@ struct Tuple {
 // p_lo) * K + extension key, support, embeddings, and the mask of the
 // block's segments (by head lane) that counted the key already.
 struct CountTable {
-  uint32_t *key, *sup, *emb;
+  uint32_t *key, *sup, *emb, *bsum;
   unsigned long long *mask;
   int *fill;
   int *failed;
@@ -265,14 +280,16 @@ __device__ __forceinline__ CountTable count_table(uint32_t *dyn, int bits, int *
   t.key = dyn;
   t.sup = dyn + t.cap;
   t.emb = dyn + 2 * t.cap;
-  t.mask = reinterpret_cast<unsigned long long *>(dyn + 3 * t.cap);  // 8-byte aligned: cap is even
+  t.bsum = dyn + 3 * t.cap;
+  t.mask = reinterpret_cast<unsigned long long *>(dyn + 4 * t.cap);  // 8-byte aligned
   t.fill = scalars;
   t.failed = scalars + 1;
   return t;
 }
 
-// Count one candidate of a row whose segment head lane is `head`.
-__device__ __forceinline__ void count_insert(const CountTable &t, uint32_t gk, int head) {
+// Count one candidate (child item count `bound`) of a row whose segment
+// head lane is `head`.
+__device__ __forceinline__ void count_insert(const CountTable &t, uint32_t gk, int head, uint32_t bound) {
   if (__hip_atomic_load(t.failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return;
   uint32_t h = hash_slot32(gk, t.bits);
   for (uint32_t probe = 0; probe < t.cap; ++probe) {
@@ -290,6 +307,7 @@ __device__ __forceinline__ void count_insert(const CountTable &t, uint32_t gk, i
       const unsigned long long old = atomicOr(&t.mask[h], bit);
       if (!(old & bit)) atomicAdd(&t.sup[h], 1u);
       atomicAdd(&t.emb[h], 1u);
+      atomicAdd(&t.bsum[h], bound);
       return;
     }
     h = (h + 1) & (t.cap - 1);
@@ -305,7 +323,7 @@ __device__ __forceinline__ void count_insert(const CountTable &t, uint32_t gk, i
 // applies the extension rules, appends each candidate to the row's list
 // region and counts it in the wave's table.
 template <bool ROOT, int TW>
-__global__ void __launch_bounds__(64) k_level(StepArgs a) {
+__global__ void __launch_bounds__(64, (TW <= 2 ? 6 : 5)) k_level(StepArgs a) {  // waves per SIMD the registers allow
   extern __shared__ uint32_t dyn[];
   __shared__ int s_scal[2];
   const int lane = threadIdx.x;
@@ -315,30 +333,17 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
   chunk_bounds<ROOT>(a, ci, start, end);
   ChunkInfo *info = a.chunks + a.chunk0 + ci;
   if (start >= end) {
-    if (lane == 0) *info = ChunkInfo{start, start, 0ull, 0u, 0u, 0, {0u, 0u, 0u}};
+    if (lane == 0) *info = ChunkInfo{start, start, 0u, 0u, 0, 0u};
     return;
   }
-  // the chunk's list region: one atomic per chunk
-  unsigned long long items = 0;
-  for (int64_t r = start + lane; r < end; r += 64) items += row_items<ROOT>(a, r);
-  items = wave_sum(items);
-  unsigned long long lbase = 0;
-  if (lane == 0) lbase = atomicAdd(a.list_cursor, items);
-  lbase = __shfl(lbase, 0, 64);
-  if (lbase + items > a.list_cap) {  // sized by the host from the parent level's bound sum
-    if (lane == 0) {
-      atomicOr(a.error_flag, 2);
-      *info = ChunkInfo{start, start, 0ull, 0u, 0u, 0, {0u, 0u, 0u}};
-    }
-    return;
-  }
-  if (lane == 0) *info = ChunkInfo{start, end, lbase, 0u, 0u, a.p_lo, {0u, 0u, 0u}};
+  if (lane == 0) *info = ChunkInfo{start, end, 0u, 0u, a.p_lo, 0u};
 
   const CountTable tab = count_table(dyn, a.table_bits, s_scal);
   for (uint32_t i = lane; i < tab.cap; i += 64) {
     tab.key[i] = kEmpty32;
     tab.sup[i] = 0;
     tab.emb[i] = 0;
+    tab.bsum[i] = 0;
     tab.mask[i] = 0;
   }
   if (lane == 0) {
@@ -360,9 +365,7 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
     const int64_t r = b + lane;
     const int head = head_lane(starts, lane);
     const uint32_t rb = live ? row_items<ROOT>(a, r) : 0u;
-    uint32_t btot;
-    const uint32_t rpre = wave_exclusive_scan<uint32_t>(rb, btot);
-    const unsigned long long lo = lbase + rpre;
+    const unsigned long long lo = live ? row_list<ROOT>(a, r) : 0ull;
     int cnt = 0;  // candidates listed for this row
     // Each step a lane takes up to 4 items of its current source, their
     // adjacency entries loaded together, then evaluates them in order.
@@ -386,7 +389,7 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
           ++cnt;
           ++n_cand;
           bsum += bound;
-          count_insert(tab, kb + (uint32_t)key, head);
+          count_insert(tab, kb + (uint32_t)key, head, bound);
         }
       };
       while (__ballot(ca < ce)) {
@@ -485,7 +488,7 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
             a.cand[lo + cnt] = make_uint2(cand_code(false, key, wout), bound);
             ++cnt;
             bsum += bound;
-            count_insert(tab, kb + (uint32_t)key, head);
+            count_insert(tab, kb + (uint32_t)key, head, bound);
           } else {
             err = 1;  // more candidates than items: the row's item count is wrong
           }
@@ -530,13 +533,15 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
     // the block's segments are done (but a carried one, at head lane 0)
     for (uint32_t i = lane; i < tab.cap; i += 64) tab.mask[i] = carry ? (tab.mask[i] & 1ull) : 0ull;
     wave_sync();
-    lbase += btot;
     b += nb;
   }
   const bool failed = s_scal[1] != 0;
   if (!failed) {
-    // commit the counts; keep the chunk's (key, embeddings) records
-    uint2 *rec = a.chunk_keys + (int64_t)(a.chunk0 + ci) * a.chunk_key_cap;
+    // commit the counts to the chunk's replica; keep the chunk's
+    // (key, embeddings, item sum) records
+    const int64_t gci = a.chunk0 + ci;
+    const int64_t roff = (gci & (a.rep - 1)) * a.nkeys + (int64_t)a.p_lo * a.K;
+    uint4 *rec = a.chunk_keys + gci * a.chunk_key_cap;
     uint32_t nrec = 0;
     for (uint32_t i0 = 0; i0 < tab.cap; i0 += 64) {
       const uint32_t i = i0 + lane;
@@ -544,9 +549,9 @@ __global__ void __launch_bounds__(64) k_level(StepArgs a) {
       const bool used = k != kEmpty32;
       const uint64_t m = __ballot(used);
       if (used) {
-        atomicAdd(&a.acc[(int64_t)a.p_lo * a.K + (int64_t)k],
-                  ((unsigned long long)tab.sup[i] << kSupShift) | (unsigned long long)tab.emb[i]);
-        rec[nrec + (uint32_t)__popcll(m & lanes_below(lane))] = make_uint2(k, tab.emb[i]);
+        atomicAdd(&a.acc[roff + (int64_t)k], ((unsigned long long)tab.sup[i] << kSupShift) | (unsigned long long)tab.emb[i]);
+        atomicAdd(&a.accb[roff + (int64_t)k], (unsigned long long)tab.bsum[i]);
+        rec[nrec + (uint32_t)__popcll(m & lanes_below(lane))] = make_uint4(k, tab.emb[i], tab.bsum[i], 0u);
       }
       nrec += (uint32_t)__popcll(m);
     }
@@ -581,13 +586,15 @@ template <bool ROOT>
 __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
   constexpr uint32_t cap = 1u << kFixTableBits, limit = cap - cap / 4;
   __shared__ unsigned long long tkey[cap];
-  __shared__ uint32_t tcnt[cap];
+  __shared__ uint32_t tcnt[cap], tbs[cap];
   __shared__ uint32_t stack_mod[32], stack_res[32];
   __shared__ int s_fill, s_over;
   const int lane = threadIdx.x;
   const uint32_t nfail = *a.fail_count;
   for (uint32_t fi = blockIdx.x; fi < nfail; fi += gridDim.x) {
-    const ChunkInfo c = a.chunks[a.fail_list[fi]];
+    const uint32_t gci = a.fail_list[fi];
+    const ChunkInfo c = a.chunks[gci];
+    const int64_t roff = (int64_t)(gci & (uint32_t)(a.rep - 1)) * a.nkeys;
     int top = 1;
     if (lane == 0) {
       stack_mod[0] = 1;
@@ -599,13 +606,13 @@ __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
       for (uint32_t i = lane; i < cap; i += 64) {
         tkey[i] = ~0ull;
         tcnt[i] = 0;
+        tbs[i] = 0;
       }
       if (lane == 0) {
         s_fill = 0;
         s_over = 0;
       }
       wave_sync();
-      unsigned long long lbase = c.list_base;
       int64_t seg_head = c.start;  // first row of the segment the block starts in
       for (int64_t b = c.start; b < c.end;) {
         const int nb = (int)min<int64_t>(64, c.end - b);
@@ -614,15 +621,14 @@ __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
         const uint64_t starts = __ballot(live && seg_start<ROOT>(a, r));
         const uint64_t upto = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
         const int64_t hrow = upto ? b + (63 - __clzll((long long)upto)) : seg_head;
-        const uint32_t rb = live ? row_items<ROOT>(a, r) : 0u;
-        uint32_t btot;
-        const unsigned long long lo = lbase + wave_exclusive_scan<uint32_t>(rb, btot);
+        const unsigned long long lo = live ? row_list<ROOT>(a, r) : 0ull;
         const int n = live ? (int)a.rccnt[r] : 0;
         const int maxn = wave_max(n);
         for (int j = 0; j < maxn; ++j) {
           if (j >= n) continue;
           int key, w;
-          cand_decode(ROOT, a.cand[lo + j].x, key, w);
+          const uint2 ce = a.cand[lo + j];
+          cand_decode(ROOT, ce.x, key, w);
           if (mod > 1 && ((uint32_t)key & (mod - 1)) != res) continue;
           if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
           // the segment (head row) fixes the pattern
@@ -633,6 +639,7 @@ __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
             const unsigned long long old = atomicCAS(&tkey[h], ~0ull, pk);
             if (old == ~0ull || old == pk) {
               atomicAdd(&tcnt[h], 1u);
+              atomicAdd(&tbs[h], ce.y);
               if (old == ~0ull && (uint32_t)atomicAdd(&s_fill, 1) >= limit) atomicOr(&s_over, 1);
               done = true;
             }
@@ -642,7 +649,6 @@ __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
         }
         const uint64_t last = starts ? 63 - __clzll((long long)starts) : 64;
         if (last < 64 && (int)last < nb) seg_head = b + (int64_t)last;
-        lbase += btot;
         b += nb;
         wave_sync();
       }
@@ -669,7 +675,9 @@ __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
         if (k == ~0ull) continue;
         const int64_t hr = c.start + (int64_t)(k >> 32);
         const int32_t p = ROOT ? a.gjob[a.vgraph[hr]] : a.pid[hr];
-        atomicAdd(&a.acc[(int64_t)p * a.K + (int64_t)(uint32_t)k], (1ull << kSupShift) | (unsigned long long)tcnt[i]);
+        const int64_t at = roff + (int64_t)p * a.K + (int64_t)(uint32_t)k;
+        atomicAdd(&a.acc[at], (1ull << kSupShift) | (unsigned long long)tcnt[i]);
+        atomicAdd(&a.accb[at], (unsigned long long)tbs[i]);
       }
       top -= 1;
       wave_sync();
@@ -679,14 +687,19 @@ __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
 }
 
 // ----------------------------------------------------------------- REDUCE
-// Read every key's counter (and clear it); keys reaching their job's min
-// support go to the frequent list as (key, support, embeddings).
+// Sum every key's replicas; keys reaching their job's min support go to the
+// frequent list as (key, support, embeddings, item sum) and keep their
+// replicas (k_cursors splits the children's rows by them, then clears
+// them); the other keys' replicas are cleared here.
 template <bool ROOT>
 __global__ void __launch_bounds__(256) k_reduce(StepArgs a) {
   const int lane = threadIdx.x & 63;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nkeys; i += (int64_t)gridDim.x * blockDim.x) {
-    const unsigned long long v = a.acc[i];
-    if (v) a.acc[i] = 0;
+    unsigned long long v = 0, bs = 0;
+    for (int r = 0; r < a.rep; ++r) {
+      v += a.acc[(int64_t)r * a.nkeys + i];
+      bs += a.accb[(int64_t)r * a.nkeys + i];
+    }
     const int32_t s = (int32_t)(v >> kSupShift);
     const unsigned long long c = v & ((1ull << kSupShift) - 1);
     a.sup[i] = s;
@@ -696,6 +709,11 @@ __global__ void __launch_bounds__(256) k_reduce(StepArgs a) {
       const int32_t owner = ROOT ? (int32_t)p : a.meta[p].job;
       hit = s >= a.minsup_job[owner];
     }
+    if (v && !hit)
+      for (int r = 0; r < a.rep; ++r) {
+        a.acc[(int64_t)r * a.nkeys + i] = 0;
+        a.accb[(int64_t)r * a.nkeys + i] = 0;
+      }
     // one atomic per wave for its frequent keys
     const unsigned long long mask = __ballot(hit);
     uint32_t first = 0;
@@ -705,24 +723,60 @@ __global__ void __launch_bounds__(256) k_reduce(StepArgs a) {
     if (hit) {
       const uint32_t slot = first + (uint32_t)__popcll(mask & lanes_below(lane));
       if (slot < a.freq_cap) {
-        unsigned long long *f = a.freq + 3 * (int64_t)slot;
+        unsigned long long *f = a.freq + 4 * (int64_t)slot;
         f[0] = (unsigned long long)i;
         f[1] = (unsigned long long)s;
         f[2] = c;
+        f[3] = bs;
       }
+    }
+  }
+}
+
+// After the host chose the children: frequent key i (fkeys[i]) becomes child
+// fchild[i] (>= 0; -1: dropped), its sup entry points at the child, and
+// every (child, replica) gets its first row and list entry -- the child's
+// bases plus the earlier replicas' embeddings and item sums -- and a zero
+// reservation cursor; the frequent keys' replicas are cleared.  Also copies
+// the children's upload.
+__global__ void k_cursors(StepArgs a, unsigned long long *dst, const unsigned long long *src, int64_t n,
+                          const unsigned long long *fkeys, const int32_t *fchild, const unsigned long long *row_base,
+                          const unsigned long long *list_base, int64_t F) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = t; i < n; i += stride) dst[i] = src[i];
+  for (int64_t i = t; i < F; i += stride) {
+    const int64_t key = (int64_t)fkeys[i];
+    const int32_t ch = fchild[i];
+    unsigned long long re = ch >= 0 ? row_base[ch] : 0ull, rl = ch >= 0 ? list_base[ch] : 0ull;
+    if (ch >= 0) a.sup[key] = -(ch + 1);
+    for (int r = 0; r < a.rep; ++r) {
+      const int64_t at = (int64_t)r * a.nkeys + key;
+      const unsigned long long v = a.acc[at], bs = a.accb[at];
+      if (ch >= 0) {
+        a.cur_rows[(int64_t)ch * a.rep + r] = re;
+        a.cur_list[(int64_t)ch * a.rep + r] = rl;
+        a.cursor[(int64_t)ch * a.rep + r] = 0;
+        re += v & ((1ull << kSupShift) - 1);
+        rl += bs;
+      }
+      a.acc[at] = 0;
+      a.accb[at] = 0;
     }
   }
 }
 
 // ------------------------------------------------------------------ WRITE
 // One wave per chunk (the chunks k_level recorded).  Each child's rows of
-// the chunk are reserved with one atomic on the child's cursor -- the
-// counts come from the chunk's (key, embeddings) records -- then written
-// with the candidates taken 64 at a time in list order (rows in order, each
-// row's list in order): a lane's position is its child's reserved base plus
-// the child's running count plus its rank among the window's lanes of the
-// same child.  The child's rows of one chunk are thus consecutive and in the
-// parent's row order, so every child's rows stay graph-contiguous.
+// the chunk, and their list regions, are reserved with one atomic each on
+// the child's cursors of the chunk's replica -- the counts come from the
+// chunk's (key, embeddings, item sum) records -- then written with the
+// candidates taken 64 at a time in list order (rows in order, each row's
+// list in order): a lane's position is its child's reserved base plus the
+// child's running count plus its rank among the window's lanes of the same
+// child, its list offset likewise with the item counts.  The child's rows
+// of one chunk are thus consecutive and in the parent's row order, so every
+// child's rows stay graph-contiguous.
 // A chunk that k_level_fix recounted has no records: it is counted here, a
 // pass over the children c with c mod `mod` == res at a time (lane per row),
 // and a pass whose table fills up is split in two by child residue before
@@ -733,9 +787,9 @@ template <bool ROOT, int TW>
 __global__ void __launch_bounds__(64) k_write(StepArgs a) {
   constexpr uint32_t cap = 1u << kWriteTableBits;
   __shared__ uint32_t ckey[cap];
-  __shared__ uint32_t ccnt[cap];
-  __shared__ uint32_t crun[cap];
-  __shared__ unsigned long long cbase[cap];
+  __shared__ uint32_t ccnt[cap], cbs[cap];
+  __shared__ uint32_t crun[cap], crunl[cap];
+  __shared__ unsigned long long cbase[cap], clbase[cap];
   __shared__ int s_pre[64];
   __shared__ unsigned long long s_lo[64];
   __shared__ int32_t s_p[64], s_g[64], s_nvp[64];
@@ -747,6 +801,7 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
   if (c.start >= c.end) return;
   bool counted = !(c.flags & kChunkFailed);
   const int CW = a.CS >> 2;
+  const int64_t rsel = (int64_t)(blockIdx.x & (unsigned)(a.rep - 1));  // the chunk's replica
   unsigned long long written = 0;
   int top = 1;
   if (lane == 0) {
@@ -767,14 +822,17 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
     }
     return 0;  // unreachable: every child of the pass was counted
   };
-  // add n rows of child ch to the table (false: the table is full)
-  auto table_add = [&](uint32_t ch, uint32_t n) -> bool {
+  // add n rows of child ch, with items bs, to the table (false: past the
+  // fill limit; a chunk's records, at most one per table entry of k_level,
+  // may fill it completely)
+  auto table_add = [&](uint32_t ch, uint32_t n, uint32_t bs, uint32_t limit) -> bool {
     uint32_t h = hash_slot32(ch, kWriteTableBits);
     for (uint32_t probe = 0; probe < cap; ++probe) {
       const uint32_t old = atomicCAS(&ckey[h], kEmpty32, ch);
       if (old == kEmpty32 || old == ch) {
         atomicAdd(&ccnt[h], n);
-        if (old == kEmpty32 && (uint32_t)atomicAdd(&s_fill, 1) >= cap - cap / 8) return false;
+        atomicAdd(&cbs[h], bs);
+        if (old == kEmpty32 && (uint32_t)atomicAdd(&s_fill, 1) >= limit) return false;
         return true;
       }
       h = (h + 1) & (cap - 1);
@@ -786,7 +844,9 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
     for (uint32_t i = lane; i < cap; i += 64) {
       ckey[i] = kEmpty32;
       ccnt[i] = 0;
+      cbs[i] = 0;
       crun[i] = 0;
+      crunl[i] = 0;
     }
     if (lane == 0) {
       s_fill = 0;
@@ -795,22 +855,19 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
     wave_sync();
     if (counted) {
       // the chunk's records: one per key, so one per child
-      const uint2 *rec = a.chunk_keys + (int64_t)blockIdx.x * a.chunk_key_cap;
+      const uint4 *rec = a.chunk_keys + (int64_t)blockIdx.x * a.chunk_key_cap;
       for (uint32_t i = lane; i < c.nkeys; i += 64) {
-        const uint2 kr = rec[i];
+        const uint4 kr = rec[i];
         const int32_t ch = child_of((int64_t)c.p_lo * a.K + (int64_t)kr.x);
-        if (ch >= 0 && !table_add((uint32_t)ch, kr.y)) atomicOr(&s_over, 1);
+        if (ch >= 0 && !table_add((uint32_t)ch, kr.y, kr.z, cap)) atomicOr(&s_over, 1);
       }
     } else {
       // count: lane per row
-      unsigned long long lbase = c.list_base;
       for (int64_t b = c.start; b < c.end;) {
         const int nb = (int)min<int64_t>(64, c.end - b);
         const int64_t r = b + lane;
         const bool live = lane < nb;
-        const uint32_t rb = live ? row_items<ROOT>(a, r) : 0u;
-        uint32_t btot;
-        const unsigned long long lo = lbase + wave_exclusive_scan<uint32_t>(rb, btot);
+        const unsigned long long lo = live ? row_list<ROOT>(a, r) : 0ull;
         int n = 0, p = 0;
         if (live) {
           n = a.rccnt[r];
@@ -820,13 +877,13 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
         for (int j = 0; j < maxn; ++j) {
           if (j >= n) continue;
           int key, w;
-          cand_decode(ROOT, a.cand[lo + j].x, key, w);
+          const uint2 ce = a.cand[lo + j];
+          cand_decode(ROOT, ce.x, key, w);
           const int32_t ch = child_of((int64_t)p * a.K + key);
           if (ch < 0 || (mod > 1 && ((uint32_t)ch & (mod - 1)) != res)) continue;
           if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
-          if (!table_add((uint32_t)ch, 1u)) atomicOr(&s_over, 1);
+          if (!table_add((uint32_t)ch, 1u, ce.y, cap - cap / 8)) atomicOr(&s_over, 1);
         }
-        lbase += btot;
         b += nb;
       }
     }
@@ -853,17 +910,21 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
     }
     // reserve
     for (uint32_t i = lane; i < cap; i += 64)
-      if (ckey[i] != kEmpty32) cbase[i] = atomicAdd(&a.cursor[ckey[i]], (unsigned long long)ccnt[i]);
+      if (ckey[i] != kEmpty32) {
+        // one atomic reserves both: rows << 32 | list entries
+        const int64_t at = (int64_t)ckey[i] * a.rep + rsel;
+        const unsigned long long v =
+            atomicAdd(&a.cursor[at], ((unsigned long long)ccnt[i] << 32) | (unsigned long long)cbs[i]);
+        cbase[i] = a.cur_rows[at] + (v >> 32);
+        clbase[i] = a.cur_list[at] + (v & 0xFFFFFFFFull);
+      }
     wave_sync();
     // write, candidates 64 at a time in list order
-    unsigned long long lbase = c.list_base;
     for (int64_t b = c.start; b < c.end;) {
       const int nb = (int)min<int64_t>(64, c.end - b);
       const int64_t r = b + lane;
       const bool live = lane < nb;
-      const uint32_t rb = live ? row_items<ROOT>(a, r) : 0u;
-      uint32_t btot;
-      const unsigned long long lo = lbase + wave_exclusive_scan<uint32_t>(rb, btot);
+      const unsigned long long lo = live ? row_list<ROOT>(a, r) : 0ull;
       int n = 0;
       int32_t p = 0, g = 0, nvp = 0;
       if (live) {
@@ -930,43 +991,47 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
           const uint32_t h = ch >= 0 ? lookup((uint32_t)ch) : 0u;
           uint64_t active = __ballot(ch >= 0);
           written += (unsigned long long)__popcll(active);
+          // list regions: any disjoint order within the child's reserved range
+          unsigned long long loff = 0;
+          if (ch >= 0) loff = clbase[h] + (unsigned long long)atomicAdd(&crunl[h], ent[u].y);
+          // row positions in list order: one round per distinct child of
+          // the window
+          unsigned long long pos = 0;
           while (active) {
             const int leader = __ffsll((long long)active) - 1;
             const int32_t cl = __shfl(ch, leader, 64);
             const uint64_t m = __ballot(ch == cl);
-            if (ch == cl) {
-              const unsigned long long pos =
-                  cbase[h] + (unsigned long long)crun[h] + (unsigned long long)__popcll(m & lanes_below(lane));
-              if ((int64_t)pos < a.ccap) {
-                a.cpid[pos] = ch;
-                a.crgraph[pos] = s_g[k];
-                a.cbound[pos] = ent[u].y;
-                uint32_t *dst = reinterpret_cast<uint32_t *>(a.cvt + pos * a.CS);
-                if (ROOT) {
-                  dst[0] = (uint32_t)s_nvp[k] | ((uint32_t)w << 8);
-                  for (int q = 1; q < CW; ++q) dst[q] = 0;
-                } else {
-                  const int nvp = s_nvp[k];
-                  for (int q = 0; q < CW; ++q) {
-                    uint32_t word = q < TW ? s_tup[k * TW + q] : 0u;
-                    if (w >= 0 && (nvp >> 2) == q) {
-                      const int sh = (nvp & 3) * 8;
-                      word = (word & ~(0xFFu << sh)) | ((uint32_t)w << sh);
-                    }
-                    dst[q] = word;
-                  }
-                }
-              }
-            }
+            if (ch == cl) pos = cbase[h] + (unsigned long long)crun[h] + (unsigned long long)__popcll(m & lanes_below(lane));
             wave_sync();
             if (lane == leader) crun[h] += (uint32_t)__popcll(m);
             wave_sync();
             active &= ~m;
           }
+          // then the rows, every selected lane at once
+          if (ch >= 0 && (int64_t)pos < a.ccap) {
+            a.cpid[pos] = ch;
+            a.crgraph[pos] = s_g[k];
+            a.cbound[pos] = ent[u].y;
+            a.crlist[pos] = loff;
+            uint32_t *dst = reinterpret_cast<uint32_t *>(a.cvt + pos * a.CS);
+            if (ROOT) {
+              dst[0] = (uint32_t)s_nvp[k] | ((uint32_t)w << 8);
+              for (int q = 1; q < CW; ++q) dst[q] = 0;
+            } else {
+              const int nvp = s_nvp[k];
+              for (int q = 0; q < CW; ++q) {
+                uint32_t word = q < TW ? s_tup[k * TW + q] : 0u;
+                if (w >= 0 && (nvp >> 2) == q) {
+                  const int sh = (nvp & 3) * 8;
+                  word = (word & ~(0xFFu << sh)) | ((uint32_t)w << sh);
+                }
+                dst[q] = word;
+              }
+            }
+          }
         }
       }
       wave_sync();  // the next block reuses the staged rows
-      lbase += btot;
       b += nb;
     }
     top -= 1;
@@ -985,26 +1050,11 @@ __global__ void k_copy8(unsigned long long *dst, const unsigned long long *src, 
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
-// The children's upload (copied to device memory), sup[key] = -(child + 1)
-// for the selected keys and each child's first row in its cursor.
-__global__ void k_upload_scatter(unsigned long long *dst, const unsigned long long *src, int64_t n, int32_t *sup,
-                                 const uint64_t *keys, unsigned long long *cursor, const unsigned long long *bases,
-                                 int64_t C) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (int64_t i = t; i < n; i += stride) dst[i] = src[i];
-  for (int64_t i = t; i < C; i += stride) {
-    sup[keys[i]] = -(int32_t)(i + 1);
-    cursor[i] = bases[i];
-  }
-}
-
 __global__ void __launch_bounds__(256) k_clear_counters(StepArgs a) {
   if (threadIdx.x == 0) {
     *a.freq_count = 0;
     *a.fail_count = 0;
     *a.error_flag = 0;
-    *a.list_cursor = 0;
   }
   // the written-row slots keep counting across levels (the host compares
   // their total with the rows it asked for)
@@ -1036,9 +1086,9 @@ hipError_t level_tw(const StepArgs &a, hipStream_t s, size_t lds) {
 }  // namespace
 
 // ------------------------------------------------------------ launchers
-size_t level_lds_bytes(int table_bits) { return (size_t)20 << table_bits; }
-// the table admits keys up to its fill limit
-int level_chunk_keys(int table_bits) { return (1 << table_bits) - (1 << table_bits) / 8; }
+size_t level_lds_bytes(int table_bits) { return (size_t)24 << table_bits; }
+// at most one record per table entry
+int level_chunk_keys(int table_bits) { return 1 << table_bits; }
 
 hipError_t launch_level(const StepArgs &a, bool root, hipStream_t s) {
   if (a.n_chunks <= 0) return hipSuccess;
@@ -1099,15 +1149,15 @@ hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_upload_scatter(void *dst, const void *src, size_t bytes, int32_t *sup, const uint64_t *keys,
-                                 unsigned long long *cursor, const unsigned long long *bases, int64_t C,
-                                 hipStream_t s) {
+hipError_t launch_cursors(const StepArgs &a, void *dst, const void *src, size_t bytes,
+                          const unsigned long long *fkeys, const int32_t *fchild, const unsigned long long *row_base,
+                          const unsigned long long *list_base, int64_t F, hipStream_t s) {
   if (bytes % 8) return hipErrorInvalidValue;
   const int64_t n = (int64_t)(bytes / 8);
-  if (n == 0 && C == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_upload_scatter, dim3(copy_grid(std::max(n, C))), dim3(256), 0, s,
-                     static_cast<unsigned long long *>(dst), static_cast<const unsigned long long *>(src), n, sup,
-                     keys, cursor, bases, C);
+  if (n == 0 && F == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cursors, dim3(copy_grid(std::max(n, F))), dim3(256), 0, s, a,
+                     static_cast<unsigned long long *>(dst), static_cast<const unsigned long long *>(src), n, fkeys,
+                     fchild, row_base, list_base, F);
   return hipGetLastError();
 }
 

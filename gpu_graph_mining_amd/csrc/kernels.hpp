@@ -9,8 +9,12 @@
 namespace gm {
 
 // Rows per chunk (nominal): one wave of k_level / k_write owns a chunk,
-// whose bounds are moved forward to the next segment start.
-constexpr int kChunkRows = 512;
+// whose bounds are moved forward to the next segment start.  Large levels
+// take kChunkRows; smaller ones shorter chunks (down to one block of 64
+// rows), so that a level has enough waves to fill the device.
+constexpr int kChunkRows = 256;
+constexpr int kMinChunkRows = 64;
+constexpr int64_t kLevelWaves = 16384;  // waves a level step aims for
 constexpr int kMaxTupleWords = kMaxPatV / 4;  // 8
 
 // Statistics counters that every wave bumps are spread over kSpread cache
@@ -28,14 +32,13 @@ constexpr int kSpreadFailed = 7;   // chunks recounted by k_level_fix
 
 // One chunk of a level's rows: [start, end) after alignment to segment
 // starts (a segment is the rows of one pattern in one graph; at the root,
-// the vertices of one graph), and the chunk's candidate-list region.
+// the vertices of one graph).
 struct alignas(16) ChunkInfo {
   int64_t start, end;
-  unsigned long long list_base;  // first list entry of the chunk's rows
-  uint32_t flags;                // kChunkFailed
-  uint32_t nkeys;                // (key, embeddings) records the chunk left for k_write
-  int32_t p_lo;                  // their keys are (pattern - p_lo) * K + key
-  uint32_t pad[3];
+  uint32_t flags;  // kChunkFailed
+  uint32_t nkeys;  // (key, embeddings, item sum) records the chunk left for k_write
+  int32_t p_lo;    // their keys are (pattern - p_lo) * K + key
+  uint32_t pad;
 };
 constexpr uint32_t kChunkFailed = 1;  // the LDS count table overflowed; k_level_fix counted it
 
@@ -49,13 +52,14 @@ struct StepArgs {
   const int32_t *gjob;
   const int32_t *vgraph;  // graph of each vertex
   // rows of this launch: [row_lo, row_hi); later levels: pattern-major rows
-  // (pid, rgraph, vt) whose list regions hold rbound entries each; root
-  // level: the database's vertices
+  // (pid, rgraph, vt) whose list regions hold rbound entries from rlist;
+  // root level: the database's vertices (lists at their adjacency offsets)
   int64_t row_lo, row_hi;
   const int32_t *pid;
   const int32_t *rgraph;
   const uint8_t *vt;
   const uint32_t *rbound;
+  const unsigned long long *rlist;
   int32_t S;  // tuple stride in bytes
   const PatMeta *meta;
   // path pruning masks (2 * mask_stride words per pattern; 0: none):
@@ -67,22 +71,25 @@ struct StepArgs {
   int64_t K;              // keys per pattern (root: keys per job)
   int32_t p_lo, p_hi;     // patterns (root: jobs) of this launch; (p - p_lo) * K < 2^32
   int32_t NEL, NVL;
-  // chunks of this launch: chunk i at chunks[chunk0 + i], n_chunks of them
+  // chunks of this launch: chunk i at chunks[chunk0 + i], n_chunks of them,
+  // chunk_rows nominal rows each
   ChunkInfo *chunks;
   int64_t chunk0, n_chunks;
-  // candidate lists: {code, child bound}; row r's list at its chunk's
-  // list_base + the prefix of its chunk's rbound, rccnt[r] entries
+  int32_t chunk_rows;
+  // candidate lists: {code, child item count}; row r's list at
+  // row_list(r), rccnt[r] entries
   uint2 *cand;
-  unsigned long long *list_cursor;
-  unsigned long long list_cap;
   uint16_t *rccnt;
-  // packed counters acc[p * K + key] = support << 40 | embeddings
-  unsigned long long *acc;
+  // rep replicas (a power of two) of the packed counters acc[p * K + key] =
+  // support << 40 | embeddings and of the item sums accb[p * K + key];
+  // chunk i counts into replica i mod rep, replica r at offset r * nkeys
+  unsigned long long *acc, *accb;
+  int32_t rep;
   int32_t table_bits;  // per-wave LDS count table of k_level
   // per chunk, chunk_key_cap records {(pattern - p_lo) * K + key,
-  // embeddings}: the chunk's candidates per key, so that k_write reserves
-  // the children's rows without counting them again
-  uint2 *chunk_keys;
+  // embeddings, item sum}: the chunk's candidates per key, so that k_write
+  // reserves the children's rows without counting them again
+  uint4 *chunk_keys;
   int32_t chunk_key_cap;
   uint32_t *fail_list, *fail_count;  // chunks whose count table overflowed
   unsigned long long *spread;
@@ -91,15 +98,18 @@ struct StepArgs {
   int32_t *sup;
   int64_t nkeys;
   const int32_t *minsup_job;
-  unsigned long long *freq;  // (key, support, embeddings) of frequent keys
+  unsigned long long *freq;  // (key, support, embeddings, item sum) of frequent keys
   uint32_t *freq_count;
   uint32_t freq_cap;
-  // k_write: child rows; cursor[c] is the next row of child c
-  unsigned long long *cursor;
+  // k_write: child rows; cur_rows / cur_list[c * rep + r] are the first row
+  // and the first list entry of child c for the chunks of replica r, and
+  // cursor[c * rep + r] = rows << 32 | list entries reserved so far
+  unsigned long long *cur_rows, *cur_list, *cursor;
   int32_t *cpid;
   int32_t *crgraph;
   uint8_t *cvt;
   uint32_t *cbound;
+  unsigned long long *crlist;
   int32_t CS;  // child tuple stride
   int64_t ccap;
 };
@@ -118,13 +128,15 @@ hipError_t launch_write(const StepArgs &a, bool root, hipStream_t s);
 // Copy `bytes` (a multiple of 8) between device memory and mapped pinned
 // host memory with a kernel on stream s.
 hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s);
-// Copy the children's upload (bytes, a multiple of 8), set
-// sup[keys[i]] = -(i + 1) and cursor[i] = bases[i] for i < C, in one kernel.
-hipError_t launch_upload_scatter(void *dst, const void *src, size_t bytes, int32_t *sup, const uint64_t *keys,
-                                 unsigned long long *cursor, const unsigned long long *bases, int64_t C,
-                                 hipStream_t s);
+// Copy the children's upload (bytes, a multiple of 8) and, for the F
+// frequent keys fkeys[i]: sup[key] = -(fchild[i] + 1) for the chosen ones
+// (fchild >= 0), their (child, replica) cursors from row_base / list_base
+// [child] and the replicas' counts, and clear the keys' replicas.
+hipError_t launch_cursors(const StepArgs &a, void *dst, const void *src, size_t bytes,
+                          const unsigned long long *fkeys, const int32_t *fchild, const unsigned long long *row_base,
+                          const unsigned long long *list_base, int64_t F, hipStream_t s);
 // Zero the per-level counters (frequent-key count, failure list, spread
-// statistics, list cursor).
+// statistics but the written-row slots).
 hipError_t launch_clear_counters(const StepArgs &a, hipStream_t s);
 
 }  // namespace gm

@@ -327,9 +327,10 @@ PatMeta make_meta(const DFS *code, int len, const uint8_t *vlab, int nv, int32_t
   return m;
 }
 
-// One frequent key as k_reduce appends it: key, support, embeddings.
+// One frequent key as k_reduce appends it: key, support, embeddings, and
+// the sum of the candidates' item counts (the children's list entries).
 struct Freq {
-  unsigned long long key, sup, proj;
+  unsigned long long key, sup, proj, items;
 };
 
 // The frequent keys in ascending key order (the order of the level's
@@ -480,15 +481,14 @@ class Pool {
 };
 
 // Device-side counters at the head of the frequent-key buffer: frequent
-// keys, failed chunks, error bits, the level's list cursor, then kSpread
+// keys, failed chunks, error bits, then kSpread
 // lines of spread statistics.  The level-start clear zeroes all but the
 // written-row slots (k_write of the previous level adds to them; the host
 // compares their running total with the rows it asked for).
 struct Counters {
   uint32_t fcount, nfail;
   int32_t err, pad0;
-  unsigned long long list_cursor;
-  unsigned long long pad[5];
+  unsigned long long pad[6];
   unsigned long long spread[kSpread * kSpreadStride];
 };
 static_assert(sizeof(Counters) == 64 + 8 * kSpread * kSpreadStride, "counter block layout");
@@ -523,7 +523,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   std::memset(&st, 0, sizeof(st));
   if (!db.ws) db.ws.reset(new Workspace());
   Workspace &w = *db.ws;
-  if (!w.acc_zero && w.acc.p) hip_check(hipMemsetAsync(w.acc.p, 0, w.acc.cap, s), "memset counters");
+  if (!w.acc_zero) {
+    if (w.acc.p) hip_check(hipMemsetAsync(w.acc.p, 0, w.acc.cap, s), "memset counters");
+    if (w.accb.p) hip_check(hipMemsetAsync(w.accb.p, 0, w.accb.cap, s), "memset counters");
+  }
   w.acc_zero = false;  // until this call ends normally
 
   grow(w.minsup, sizeof(int32_t) * opt.minsup.size(), s);
@@ -565,7 +568,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   unsigned long long written_expect = 0;  // child rows asked of every k_write so far
   uint64_t list_cap = (uint64_t)std::max<int64_t>(db.n_adj, 1);  // this level's list entries (root: 2E)
   uint32_t last_fcount = 0;
-  int table_bits = 9;
+  int table_bits = opt.table_bits > 0 ? opt.table_bits : 8;
+  const int max_chunk_rows = opt.chunk_rows > 0 ? opt.chunk_rows : kChunkRows;
   const Counters *dctr_last = nullptr;
   // Path pruning: t2[((job * NVL) + centre label) * AR + arm] is the mask of
   // the arms (elabel * NVL + neighbour label) that form a frequent 2-edge
@@ -610,17 +614,27 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     auto prow = [&](int64_t p) { return root ? job_row0[p] : L.row0[p]; };
     // launches over pattern ranges with range * K < 2^32 (32-bit table keys)
     const int64_t prange = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
+    // chunk length: kChunkRows, or shorter so that the level has about
+    // kLevelWaves chunks (a power of two, at least one block)
+    int chunk_rows = max_chunk_rows;
+    while (chunk_rows > kMinChunkRows && rows_now / chunk_rows < kLevelWaves) chunk_rows /= 2;
     int64_t total_chunks = 0;
     for (int64_t lo = 0; lo < P; lo += prange)
-      total_chunks += ceil_div(prow(std::min<int64_t>(P, lo + prange)) - prow(lo), kChunkRows);
+      total_chunks += ceil_div(prow(std::min<int64_t>(P, lo + prange)) - prow(lo), chunk_rows);
+    // counter replicas: chunk i counts into replica i mod rep, so that the
+    // keys every chunk hits (the root's) are not one hot address (atomics
+    // on one address serialise at ~100 ns each); up to 64 MB of them
+    int rep = 1;
+    while (rep < 64 && (int64_t)rep * 2 * nkeys * 16 <= (64ll << 20) && (int64_t)rep * 2 * 32 <= total_chunks) rep *= 2;
     grow(w.sup, sizeof(int32_t) * std::max<int64_t>(nkeys, 1), s);
-    grow_zero(w.acc, sizeof(unsigned long long) * std::max<int64_t>(nkeys, 1), s);
+    grow_zero(w.acc, sizeof(unsigned long long) * std::max<int64_t>(nkeys * rep, 1), s);
+    grow_zero(w.accb, sizeof(unsigned long long) * std::max<int64_t>(nkeys * rep, 1), s);
     grow(w.chunks, sizeof(ChunkInfo) * std::max<int64_t>(total_chunks, 1), s);
     grow(w.fail, sizeof(uint32_t) * std::max<int64_t>(total_chunks, 1), s);
-    grow(w.chunk_keys, sizeof(uint2) * level_chunk_keys(table_bits) * std::max<int64_t>(total_chunks, 1), s);
+    grow(w.chunk_keys, sizeof(uint4) * level_chunk_keys(table_bits) * std::max<int64_t>(total_chunks, 1), s);
     grow(w.rccnt, sizeof(uint16_t) * std::max<int64_t>(rows_now, 1), s);
     grow(w.cand, sizeof(uint2) * std::max<uint64_t>(list_cap, 1), s);
-    uint32_t fcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(nkeys, 1), std::max<int64_t>(1 << 20, P * 64));
+    uint32_t fcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(nkeys, 1), std::max<int64_t>(1 << 16, P * 64));
     Counters *dctr = nullptr;
     Counters cnt;
     const char *hbuf = nullptr;
@@ -629,7 +643,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     uint32_t guess = 0;  // frequent keys in the first download
     // the stage; rerun with a longer frequent-key list when it overflowed
     for (int attempt = 0;; ++attempt) {
-      const size_t freq_bytes = sizeof(Counters) + 3 * sizeof(unsigned long long) * fcap;
+      const size_t freq_bytes = sizeof(Counters) + sizeof(Freq) * fcap;
       {
         // a new block (the allocator may hand back the address it freed)
         const size_t cap0 = w.freq.cap;
@@ -645,14 +659,15 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       dctr_last = dctr;
       a.rccnt = w.rccnt.as<uint16_t>();
       a.cand = w.cand.as<uint2>();
-      a.list_cap = list_cap;
-      a.list_cursor = &dctr->list_cursor;
       a.chunks = w.chunks.as<ChunkInfo>();
       a.fail_list = w.fail.as<uint32_t>();
       a.fail_count = &dctr->nfail;
       a.acc = w.acc.as<unsigned long long>();
+      a.accb = w.accb.as<unsigned long long>();
+      a.rep = rep;
       a.table_bits = table_bits;
-      a.chunk_keys = w.chunk_keys.as<uint2>();
+      a.chunk_rows = chunk_rows;
+      a.chunk_keys = w.chunk_keys.as<uint4>();
       a.chunk_key_cap = level_chunk_keys(table_bits);
       a.spread = dctr->spread;
       a.error_flag = &dctr->err;
@@ -667,6 +682,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.rgraph = root ? nullptr : w.rgraph[cur].as<int32_t>();
       a.vt = root ? nullptr : w.vt[cur].as<uint8_t>();
       a.rbound = root ? nullptr : w.rbound[cur].as<uint32_t>();
+      a.rlist = root ? nullptr : w.rlist[cur].as<unsigned long long>();
       a.S = S;
       a.meta = root ? nullptr : reinterpret_cast<const PatMeta *>(w.upl[cur].as<char>());
       a.mask_stride = root ? 0 : mask_stride;
@@ -687,7 +703,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         a.row_lo = prow(lo);
         a.row_hi = prow(hi);
         a.chunk0 = chunk0;
-        a.n_chunks = ceil_div(a.row_hi - a.row_lo, kChunkRows);
+        a.n_chunks = ceil_div(a.row_hi - a.row_lo, chunk_rows);
         hip_check(launch_level(a, root, s), "k_level");
         chunk0 += a.n_chunks;
       }
@@ -701,8 +717,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       if (time_stage) hip_check(hipEventRecord(w.ev[1], s), "hipEventRecord");
       hip_check(launch_reduce(a, root, s), "k_reduce");
       // counters and (most often all of) the frequent keys in one copy
-      guess = std::min<uint32_t>(fcap, std::max<uint32_t>(4096, 2 * last_fcount));
-      const size_t head = sizeof(Counters) + 3 * sizeof(unsigned long long) * guess;
+      guess = std::min<uint32_t>(fcap, std::max<uint32_t>(1024, 2 * last_fcount));
+      const size_t head = sizeof(Counters) + sizeof(Freq) * guess;
       hbuf = static_cast<const char *>(w.down.ensure(head));
       hip_check(launch_copy8(w.down.dev, dctr, head, s), "download counters");
       hip_check(hipStreamSynchronize(s), "k_level");
@@ -746,7 +762,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     st.pruned += pruned;
     st.max_table_fill = std::max<uint64_t>(st.max_table_fill, max_fill);
     // a level with recounted chunks wants a larger table next time
-    if (failed * 64 > (unsigned long long)total_chunks && table_bits < 11) ++table_bits;
+    if (failed * 64 > (unsigned long long)total_chunks && table_bits < 11 && opt.table_bits == 0) ++table_bits;
     last_fcount = cnt.fcount;
     const int64_t F = cnt.fcount;
     if (F) {
@@ -879,61 +895,92 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     double us_host = 0, us_overlap = 0;
     int64_t alive = 0, alive_rows = 0;  // minimal children and their embeddings
     int CS = 4;
-    if (C > 0) {
-      // The embedding counts size the child level exactly: child c's rows
-      // start at the sum of the earlier children's counts, k_write reserves
-      // them per chunk from the child's cursor.  The selected keys point at
-      // their child ids first (read from the mapped staging buffer) and the
-      // write pass starts; the children's metadata (the next level's)
-      // follows in one upload.
-      Timer tdev2;
-      CS = (int)round4(max_nv);
-      const int nxt = cur ^ 1;
-      const size_t meta_bytes = sizeof(PatMeta) * C;
-      const size_t keys_off = (meta_bytes + 7) & ~size_t(7);
-      const size_t base_off = keys_off + sizeof(unsigned long long) * C;
-      // path masks of the children with >= 2 edges: a forward and a
-      // backward word per rightmost-path vertex (at most max_nv of them)
-      const bool masks = prune && new_len >= 2;
-      const int mstride = masks ? max_nv : 0;
-      const size_t masks_off = base_off + sizeof(unsigned long long) * C;
-      const size_t up_bytes = masks_off + sizeof(uint64_t) * 2 * mstride * C;
-      char *hu = static_cast<char *>(w.up.ensure(up_bytes));
-      PatMeta *hm = reinterpret_cast<PatMeta *>(hu);
-      unsigned long long *hk = reinterpret_cast<unsigned long long *>(hu + keys_off);
-      unsigned long long *hb = reinterpret_cast<unsigned long long *>(hu + base_off);
+    uint64_t child_items = 0;  // the children's list entries
+    // Upload (mapped staging, read by k_cursors and copied by it): the
+    // children's metadata, then per frequent key its key and child id (-1:
+    // dropped), per child its first row and first list entry, then the
+    // children's path masks.  Child c's rows follow the earlier children's
+    // (embedding counts), its list entries likewise (item sums).
+    const size_t meta_bytes = sizeof(PatMeta) * C;
+    const size_t fk_off = (meta_bytes + 7) & ~size_t(7);
+    const size_t fc_off = fk_off + sizeof(unsigned long long) * F;
+    const size_t rb_off = (fc_off + sizeof(int32_t) * F + 7) & ~size_t(7);
+    const size_t lb_off = rb_off + sizeof(unsigned long long) * C;
+    // path masks of the children with >= 2 edges: a forward and a
+    // backward word per rightmost-path vertex (at most max_nv of them)
+    const bool masks = prune && new_len >= 2 && C > 0;
+    const int mstride = masks ? max_nv : 0;
+    const size_t masks_off = lb_off + sizeof(unsigned long long) * C;
+    const size_t up_bytes = masks_off + sizeof(uint64_t) * 2 * mstride * C;
+    char *hu = F > 0 ? static_cast<char *>(w.up.ensure(up_bytes)) : nullptr;
+    if (F > 0) {
+      unsigned long long *hfk = reinterpret_cast<unsigned long long *>(hu + fk_off);
+      int32_t *hfc = reinterpret_cast<int32_t *>(hu + fc_off);
+      unsigned long long *hrb = reinterpret_cast<unsigned long long *>(hu + rb_off);
+      unsigned long long *hlb = reinterpret_cast<unsigned long long *>(hu + lb_off);
       int64_t run = 0;
       for (int64_t i = 0; i < F; ++i) {
         const int64_t c = kidx[i];
+        hfk[i] = fr[i].key;
+        hfc[i] = (int32_t)c;
         if (c < 0) continue;
-        hk[c] = fr[i].key;
-        hb[c] = (unsigned long long)run;
+        hrb[c] = (unsigned long long)run;
+        hlb[c] = (unsigned long long)child_items;
         N.row0[c] = run;
         run += (int64_t)fr[i].proj;
+        child_items += fr[i].items;
+        // k_write reserves rows << 32 | list entries in one 64-bit atomic
+        if (fr[i].proj >= (1ull << 32) || fr[i].items >= (1ull << 32))
+          throw Error(GM_E_UNSUPPORTED, "a pattern with 2^32 or more embeddings or candidate-list entries");
       }
       N.row0[C] = run;
+    }
+    if (F > 0 && C == 0) {  // nothing to write: the frequent keys' replicas are cleared
+      const char *su = static_cast<const char *>(w.up.dev);
+      hip_check(launch_cursors(a, nullptr, nullptr, 0, reinterpret_cast<const unsigned long long *>(su + fk_off),
+                               reinterpret_cast<const int32_t *>(su + fc_off), nullptr, nullptr, F, s),
+                "k_cursors");
+    }
+    if (C > 0) {
+      // The embedding counts size the child level exactly; k_write reserves
+      // each child's rows per chunk from the cursors of the chunk's replica.
+      // The chosen keys point at their child ids and the cursors are set
+      // first (read from the mapped staging buffer) and the write pass
+      // starts; the children's metadata (the next level's) follows in one
+      // upload.
+      Timer tdev2;
+      CS = (int)round4(max_nv);
+      const int nxt = cur ^ 1;
+      PatMeta *hm = reinterpret_cast<PatMeta *>(hu);
       char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
       const char *su = static_cast<const char *>(w.up.dev);
       grow(w.pid[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.rgraph[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
       grow(w.vt[nxt], (size_t)CS * std::max<int64_t>(child_rows, 1), s);
       grow(w.rbound[nxt], sizeof(uint32_t) * std::max<int64_t>(child_rows, 1), s);
-      grow(w.cursor, sizeof(unsigned long long) * C, s);
+      grow(w.rlist[nxt], sizeof(unsigned long long) * std::max<int64_t>(child_rows, 1), s);
+      grow(w.cur_rows, sizeof(unsigned long long) * C * rep, s);
+      grow(w.cur_list, sizeof(unsigned long long) * C * rep, s);
+      grow(w.cursor, sizeof(unsigned long long) * C * rep, s);
+      a.cur_rows = w.cur_rows.as<unsigned long long>();
+      a.cur_list = w.cur_list.as<unsigned long long>();
       a.cursor = w.cursor.as<unsigned long long>();
       a.ccap = child_rows;
       a.cpid = w.pid[nxt].as<int32_t>();
       a.crgraph = w.rgraph[nxt].as<int32_t>();
       a.cvt = w.vt[nxt].as<uint8_t>();
       a.cbound = w.rbound[nxt].as<uint32_t>();
+      a.crlist = w.rlist[nxt].as<unsigned long long>();
       a.CS = CS;
       a.chunk0 = 0;
       a.n_chunks = total_chunks;
-      // the selected keys point at their child ids and the children's first
-      // rows go to their cursors, then the write pass (launched back to
-      // back: the device does not wait on the host between)
-      hip_check(launch_upload_scatter(du, su, 0, w.sup.as<int32_t>(), reinterpret_cast<const uint64_t *>(su + keys_off),
-                                      a.cursor, reinterpret_cast<const unsigned long long *>(su + base_off), C, s),
-                "scatter child ids");
+      // child ids and cursors, then the write pass (launched back to back:
+      // the device does not wait on the host between)
+      hip_check(launch_cursors(a, du, su, 0, reinterpret_cast<const unsigned long long *>(su + fk_off),
+                               reinterpret_cast<const int32_t *>(su + fc_off),
+                               reinterpret_cast<const unsigned long long *>(su + rb_off),
+                               reinterpret_cast<const unsigned long long *>(su + lb_off), F, s),
+                "k_cursors");
       hip_check(launch_write(a, root, s), "k_write");
       written_expect += (unsigned long long)child_rows;
       us_host = t_host.sec() * 1e6;
@@ -1064,10 +1111,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       hip_check(launch_copy8(du, su, up_bytes, s), "upload children");
       us_overlap = t_over.sec() * 1e6;
       dev_sec += tdev2.sec();
-      size_t live = db.bytes + w.sup.cap + w.acc.cap + w.freq.cap + w.rccnt.cap + w.cand.cap + w.chunks.cap +
-                    w.fail.cap + w.cursor.cap + w.chunk_keys.cap;
+      size_t live = db.bytes + w.sup.cap + w.acc.cap + w.accb.cap + w.freq.cap + w.rccnt.cap + w.cand.cap +
+                    w.chunks.cap + w.fail.cap + w.cur_rows.cap + w.cur_list.cap + w.cursor.cap + w.chunk_keys.cap;
       for (int i = 0; i < 2; ++i)
-        live += w.pid[i].cap + w.rgraph[i].cap + w.vt[i].cap + w.rbound[i].cap + w.upl[i].cap;
+        live += w.pid[i].cap + w.rgraph[i].cap + w.vt[i].cap + w.rbound[i].cap + w.rlist[i].cap + w.upl[i].cap;
       peak = std::max(peak, live);
       cur = nxt;
       mask_off = masks_off;
@@ -1106,9 +1153,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     std::swap(L, N);
     S = CS;
     n_rows = child_rows;
-    // the children's lists: at most the item counts of every listed
-    // candidate of this level (the selected ones need less)
-    list_cap = std::max<uint64_t>(cand_bound, 1);
+    // the children's lists: exactly their item counts
+    list_cap = std::max<uint64_t>(child_items, 1);
     ++level;
   }
   // the last write's row count

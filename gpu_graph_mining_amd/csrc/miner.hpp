@@ -47,10 +47,10 @@ struct HostBuf {
 // Device buffers of the level steps, kept across mining calls on one
 // database so that repeated calls allocate nothing once warm.
 struct Workspace {
-  DevBuf minsup, sup, acc, freq, rccnt, cursor, fail, chunks, chunk_keys, cand;
-  // double-buffered by level: rows (pid/rgraph/vt/rbound) and the pattern
-  // upload (metadata, path masks)
-  DevBuf pid[2], rgraph[2], vt[2], rbound[2], upl[2];
+  DevBuf minsup, sup, acc, accb, freq, rccnt, cur_rows, cur_list, cursor, fail, chunks, chunk_keys, cand;
+  // double-buffered by level: rows (pid/rgraph/vt/rbound/rlist) and the
+  // pattern upload (metadata, path masks)
+  DevBuf pid[2], rgraph[2], vt[2], rbound[2], rlist[2], upl[2];
   HostBuf up, down;
   hipEvent_t ev[2] = {nullptr, nullptr};  // stage timing (MineOptions::stage_events)
   // the packed counters hold zeros outside a level step (k_reduce clears
@@ -121,6 +121,8 @@ struct MineOptions {
   bool path_pruning = true;     // k_level drops candidates with an infrequent 2-edge sub-path
   int32_t speculate = 0;        // 0 adaptive, 1 never, 2 always (gm_cfg::speculate)
   bool stage_events = false;    // time the extension + support stages with HIP events
+  int32_t chunk_rows = 0;       // launch shape (gm_cfg::chunk_rows); 0 = default
+  int32_t table_bits = 0;       // launch shape (gm_cfg::table_bits); 0 = default
   // Query mode: instead of mining, follow the codes of `queries` (every
   // prefix of a minimum code is one) and store each query's support in
   // query_support; the report callback is not called.

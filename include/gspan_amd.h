@@ -66,7 +66,11 @@ typedef struct gm_cfg {
                                last level's non-minimal share was small (default), 1 = never, 2 = always */
   int32_t stage_events;     /* 1 = time each level's extension + support stage with HIP events on the
                                mining stream (gm_stats.seconds_stage_events); 0 = off (default) */
-  int32_t reserved[3];
+  int32_t chunk_rows;       /* launch shape: rows per wave of the level kernels (64..4096, a power of
+                               two; smaller levels use shorter chunks); 0 = default (256) */
+  int32_t table_bits;       /* launch shape: log2 entries of a wave's LDS count table (7..11); a chunk
+                               that fills it is recounted; 0 = default (8) */
+  int32_t reserved[1];
 } gm_cfg;
 
 /* Per-run statistics (filled when a non-NULL pointer is passed). */
