@@ -46,6 +46,7 @@ EXPORTED_SYMBOLS = (
     "gm_db_write_txt", "gm_db_counts", "gm_db_export", "gm_db_upload", "gm_db_destroy", "gm_mine",
     "gm_mine_batch", "gm_dfs_code_is_min", "gm_gen_random", "gm_gen_planted", "gm_gen_molecule", "gm_db_subset",
     "gm_batch_create", "gm_batch_mine", "gm_batch_destroy", "gm_support_queries", "gm_support_query",
+    "gm_db_write", "gm_db_load_part",
 )
 
 
@@ -98,6 +99,8 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "gm_db_create": (C.c_int, [C.c_int32, i32p, i32p, i32p, i32p, i32p, i32p, P(dbp)]),
         "gm_db_load": (C.c_int, [C.c_char_p, C.c_char_p, P(dbp)]),
         "gm_db_write_txt": (C.c_int, [dbp, C.c_char_p]),
+        "gm_db_write": (C.c_int, [dbp, C.c_char_p, C.c_char_p]),
+        "gm_db_load_part": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, P(dbp)]),
         "gm_db_counts": (C.c_int, [dbp, P(C.c_int64)]),
         "gm_db_export": (C.c_int, [dbp, i32p, i32p, i32p, i32p, i32p, i32p]),
         "gm_db_upload": (C.c_int, [dbp]),
@@ -276,6 +279,14 @@ class GraphDB:
         return cls(h.value)
 
     @classmethod
+    def load_part(cls, path: os.PathLike, total_parts: int, part: int) -> "GraphDB":
+        """Part `part` of `total_parts` of a BIN file (dbio::read_database_bin
+        with parts, src/globals/dbio.cpp:275-329)."""
+        h = C.c_void_p()
+        _check(load_library().gm_db_load_part(str(path).encode(), int(total_parts), int(part), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
     def random(cls, seed: int, n_graphs: int, vmin: int, vmax: int, edge_factor: float = 1.2, n_vlabels: int = 5,
                n_elabels: int = 3) -> "GraphDB":
         h = C.c_void_p()
@@ -339,6 +350,11 @@ class GraphDB:
 
     def write_txt(self, path: os.PathLike) -> None:
         _check(load_library().gm_db_write_txt(self._h, str(path).encode()))
+
+    def write(self, path: os.PathLike, filetype: str = "txt") -> None:
+        """Write as txt, bin (dbio::write_database_bin) or cudabin
+        (dbio::write_database_cudabin)."""
+        _check(load_library().gm_db_write(self._h, str(path).encode(), filetype.encode()))
 
     def upload(self) -> None:
         _check(load_library().gm_db_upload(self._h))

@@ -190,6 +190,33 @@ int gm_db_write_txt(const gm_db *db, const char *path) {
   });
 }
 
+int gm_db_write(const gm_db *db, const char *path, const char *filetype) {
+  return guard([&]() {
+    if (!db || !path || !filetype) throw gm::Error(GM_E_ARG, "null argument");
+    std::string ft(filetype);
+    if (!ft.empty() && ft[0] == '-') ft = ft.substr(1);
+    if (ft == "txt")
+      gm::write_txt(db->host, path);
+    else if (ft == "bin")
+      gm::write_bin(db->host, path);
+    else if (ft == "cudabin")
+      gm::write_cudabin(db->host, path);
+    else
+      throw gm::Error(GM_E_ARG, "unknown filetype");
+  });
+}
+
+int gm_db_load_part(const char *path, int32_t total_parts, int32_t part, gm_db **out) {
+  return guard([&]() {
+    if (!path || !out) throw gm::Error(GM_E_ARG, "null argument");
+    *out = nullptr;
+    gm::HostDB h;
+    gm::read_bin_part(path, total_parts, part, h);
+    h.check_consistent();
+    *out = new_db(std::move(h));
+  });
+}
+
 int gm_db_counts(const gm_db *db, int64_t counts[6]) {
   return guard([&]() {
     if (!db || !counts) throw gm::Error(GM_E_ARG, "null argument");

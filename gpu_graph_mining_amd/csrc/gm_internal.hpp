@@ -46,8 +46,11 @@ struct HostDB {
 // Readers / writers (host_db.cpp).
 void read_txt(const std::string &path, bool fsg, HostDB &db);
 void read_bin(const std::string &path, HostDB &db);
+void read_bin_part(const std::string &path, int32_t parts, int32_t part, HostDB &db);
 void read_cudabin(const std::string &path, HostDB &db);
 void write_txt(const HostDB &db, const std::string &path);
+void write_bin(const HostDB &db, const std::string &path);
+void write_cudabin(const HostDB &db, const std::string &path);
 
 // Synthetic generators (generators.cpp).
 void gen_random(uint64_t seed, int32_t n_graphs, int32_t vmin, int32_t vmax, double edge_factor,

@@ -20,6 +20,7 @@
 #include <fstream>
 #include <map>
 #include <sstream>
+#include <tuple>
 
 #include "gm_internal.hpp"
 #include "gspan_amd.h"
@@ -142,6 +143,43 @@ std::vector<char> slurp(const std::string &path) {
 }
 }  // namespace
 
+// One graph record of a BIN file (Graph::serialize, graph_types.cpp:566-614)
+// at c.pos, appended to db.  Each adjacency entry appears once per
+// direction; an edge is kept from its lower endpoint, a self-loop (both of
+// its entries sit at the vertex) once per pair of entries.
+static void read_graph_record(Cursor &c, const std::string &path, HostDB &db, std::vector<int32_t> &labels,
+                              std::vector<int32_t> &ef, std::vector<int32_t> &et, std::vector<int32_t> &el) {
+  const uint64_t stored = c.get<uint64_t>();
+  const size_t start = c.pos;
+  const int32_t total = c.get<int32_t>();
+  (void)c.get<int32_t>();  // edge_size_
+  (void)c.get<uint8_t>();  // directed flag (bool)
+  const int32_t nv = c.get<int32_t>();
+  if ((uint64_t)total != stored || nv < 0) throw Error(GM_E_ARG, "corrupt graph record in " + path);
+  labels.assign(nv, 0);
+  ef.clear();
+  et.clear();
+  el.clear();
+  for (int32_t v = 0; v < nv; ++v) {
+    (void)c.get<int32_t>();  // serialized vertex size
+    labels[v] = c.get<int32_t>();
+    const int32_t ne = c.get<int32_t>();
+    bool loop_open = false;
+    for (int32_t k = 0; k < ne; ++k) {
+      const int32_t from = c.get<int32_t>(), to = c.get<int32_t>(), lab = c.get<int32_t>();
+      (void)c.get<int32_t>();  // edge id
+      if (from != v || to < 0 || to >= nv) throw Error(GM_E_ARG, "corrupt adjacency in " + path);
+      if (from < to || (from == to && (loop_open = !loop_open))) {
+        ef.push_back(from);
+        et.push_back(to);
+        el.push_back(lab);
+      }
+    }
+  }
+  if (c.pos - start != stored) throw Error(GM_E_ARG, "graph record length mismatch in " + path);
+  db.add_graph(labels, ef, et, el);
+}
+
 void read_bin(const std::string &path, HostDB &db) {
   std::vector<char> buf = slurp(path);
   db = HostDB();
@@ -153,35 +191,146 @@ void read_bin(const std::string &path, HostDB &db) {
   std::vector<int32_t> labels, ef, et, el;
   for (uint64_t i = 0; i < n; ++i) {
     if (c.pos != offs[i]) throw Error(GM_E_ARG, "graph stored at an unexpected offset in " + path);
-    uint64_t stored = c.get<uint64_t>();
-    size_t start = c.pos;
-    int32_t total = c.get<int32_t>();
-    (void)c.get<int32_t>();  // edge_size_
-    (void)c.get<uint8_t>();  // directed flag (bool)
-    int32_t nv = c.get<int32_t>();
-    if ((uint64_t)total != stored || nv < 0) throw Error(GM_E_ARG, "corrupt graph record in " + path);
-    labels.assign(nv, 0);
-    ef.clear();
-    et.clear();
-    el.clear();
+    read_graph_record(c, path, db, labels, ef, et, el);
+  }
+  if (c.pos != buf.size()) throw Error(GM_E_ARG, "trailing bytes after the last graph in " + path);
+}
+
+// Part `part` of `parts` of a BIN file: transactions [n / parts * part,
+// n / parts * (part + 1)), the last part up to n, read from their stored
+// offset (dbio.cpp:275-329).
+void read_bin_part(const std::string &path, int32_t parts, int32_t part, HostDB &db) {
+  if (parts < 1 || part < 0 || part >= parts)
+    throw Error(GM_E_ARG, "incorrect parameter value: part should be from [0, total_parts-1]");
+  std::vector<char> buf = slurp(path);
+  db = HostDB();
+  Cursor c{buf, 0};
+  const uint64_t n = c.get<uint64_t>();
+  if (n > (1ull << 31)) throw Error(GM_E_ARG, "implausible graph count in " + path);
+  const int64_t first = (int64_t)n / parts * part;
+  const int64_t last = part == parts - 1 ? (int64_t)n : (int64_t)n / parts * (part + 1);
+  if (first >= last) return;
+  c.pos = sizeof(uint64_t) * (1 + (size_t)first);
+  c.pos = (size_t)c.get<uint64_t>();
+  std::vector<int32_t> labels, ef, et, el;
+  for (int64_t i = first; i < last; ++i) read_graph_record(c, path, db, labels, ef, et, el);
+}
+
+namespace {
+// Adjacency of graph g as Graph::read builds it (graph_types.cpp:66-113):
+// every edge in input order pushed at its from vertex, then at its to
+// vertex; entry = (to, elabel, id) with ids as Graph::buildEdge assigns
+// them (graph_types.cpp:34-60): in order of first appearance scanning the
+// vertices, one id per (low, high, elabel).
+struct AdjEntry {
+  int32_t to, elabel, id;
+};
+int32_t build_adjacency(const HostDB &db, int32_t g, std::vector<std::vector<AdjEntry>> &adj) {
+  const int32_t nv = db.vbegin[g + 1] - db.vbegin[g];
+  adj.assign(nv, {});
+  for (int32_t e = db.ebegin[g]; e < db.ebegin[g + 1]; ++e) {
+    adj[db.efrom[e]].push_back(AdjEntry{db.eto[e], db.elabel[e], -1});
+    adj[db.eto[e]].push_back(AdjEntry{db.efrom[e], db.elabel[e], -1});
+  }
+  std::map<std::tuple<int32_t, int32_t, int32_t>, int32_t> ids;
+  for (int32_t v = 0; v < nv; ++v)
+    for (AdjEntry &a : adj[v]) {
+      const auto key = std::make_tuple(std::min(v, a.to), std::max(v, a.to), a.elabel);
+      auto it = ids.find(key);
+      if (it == ids.end()) it = ids.emplace(key, (int32_t)ids.size()).first;
+      a.id = it->second;
+    }
+  return (int32_t)ids.size();
+}
+
+template <class T>
+void put(std::vector<char> &out, T v) {
+  const char *p = reinterpret_cast<const char *>(&v);
+  out.insert(out.end(), p, p + sizeof(T));
+}
+
+void write_file(const std::string &path, const std::vector<char> &bytes) {
+  std::ofstream os(path.c_str(), std::ios::binary | std::ios::trunc);
+  if (!os) throw Error(GM_E_IO, "cannot write " + path);
+  os.write(bytes.data(), (std::streamsize)bytes.size());
+  if (!os) throw Error(GM_E_IO, "cannot write " + path);
+}
+}  // namespace
+
+// BIN (dbio.cpp:209-268): size_t graph count, size_t offset of each graph
+// record, then per graph a size_t length and the Graph::serialize bytes --
+// int total size, int edge count, bool directed (false), int vertex count,
+// per vertex int size, int label, int entry count and (from, to, elabel,
+// id) per adjacency entry (graph_types.cpp:446-614).
+void write_bin(const HostDB &db, const std::string &path) {
+  if (db.n_graphs == 0) throw Error(GM_E_ARG, "Cannot save empty database.");
+  std::vector<char> body;
+  std::vector<uint64_t> offs;
+  std::vector<std::vector<AdjEntry>> adj;
+  const uint64_t head = sizeof(uint64_t) * (1 + (uint64_t)db.n_graphs);
+  for (int32_t g = 0; g < db.n_graphs; ++g) {
+    offs.push_back(head + body.size());
+    const int32_t edges = build_adjacency(db, g, adj);
+    const int32_t nv = (int32_t)adj.size();
+    int32_t size = 3 * (int32_t)sizeof(int32_t) + (int32_t)sizeof(bool);
+    for (int32_t v = 0; v < nv; ++v) size += 3 * (int32_t)sizeof(int32_t) + 16 * (int32_t)adj[v].size();
+    put<uint64_t>(body, (uint64_t)size);
+    put<int32_t>(body, size);
+    put<int32_t>(body, edges);
+    put<uint8_t>(body, 0);  // directed = false
+    put<int32_t>(body, nv);
     for (int32_t v = 0; v < nv; ++v) {
-      (void)c.get<int32_t>();  // serialized vertex size
-      labels[v] = c.get<int32_t>();
-      int32_t ne = c.get<int32_t>();
-      for (int32_t k = 0; k < ne; ++k) {
-        int32_t from = c.get<int32_t>(), to = c.get<int32_t>(), lab = c.get<int32_t>();
-        (void)c.get<int32_t>();  // edge id
-        if (from != v || to < 0 || to >= nv) throw Error(GM_E_ARG, "corrupt adjacency in " + path);
-        if (from < to) {  // undirected: keep one direction of each pair
-          ef.push_back(from);
-          et.push_back(to);
-          el.push_back(lab);
-        }
+      put<int32_t>(body, 3 * (int32_t)sizeof(int32_t) + 16 * (int32_t)adj[v].size());
+      put<int32_t>(body, db.vlabel[db.vbegin[g] + v]);
+      put<int32_t>(body, (int32_t)adj[v].size());
+      for (const AdjEntry &a : adj[v]) {
+        put<int32_t>(body, v);
+        put<int32_t>(body, a.to);
+        put<int32_t>(body, a.elabel);
+        put<int32_t>(body, a.id);
       }
     }
-    if (c.pos - start != stored) throw Error(GM_E_ARG, "graph record length mismatch in " + path);
-    db.add_graph(labels, ef, et, el);
   }
+  std::vector<char> bytes;
+  put<uint64_t>(bytes, (uint64_t)db.n_graphs);
+  for (uint64_t o : offs) put<uint64_t>(bytes, o);
+  bytes.insert(bytes.end(), body.begin(), body.end());
+  write_file(path, bytes);
+}
+
+// cudabin (dbio.cpp:346-370): int graph count, int adjacency entries, int
+// largest vertex count, int vertex count, then the padded-CSR arrays of
+// graph_database_cuda::create_from_host_representation
+// (cuda_graph_types.cpp:65-122): entries (neighbour slot g * m + to), their
+// edge labels, vertex labels per slot (-1 on padding), vertex offsets per
+// slot (-1 on padding) and the total at slot n * m.
+void write_cudabin(const HostDB &db, const std::string &path) {
+  const int32_t m = db.max_vertices();
+  std::vector<std::vector<AdjEntry>> adj;
+  std::vector<int32_t> edges, elabs, vlabs((size_t)db.n_graphs * m, -1), voffs((size_t)db.n_graphs * m + 1, -1);
+  for (int32_t g = 0; g < db.n_graphs; ++g) {
+    build_adjacency(db, g, adj);
+    const int64_t base = (int64_t)g * m;
+    for (int32_t v = 0; v < (int32_t)adj.size(); ++v) {
+      voffs[base + v] = (int32_t)edges.size();
+      vlabs[base + v] = db.vlabel[db.vbegin[g] + v];
+      for (const AdjEntry &a : adj[v]) {
+        edges.push_back((int32_t)(a.to + base));
+        elabs.push_back(a.elabel);
+      }
+    }
+  }
+  voffs[(size_t)db.n_graphs * m] = (int32_t)edges.size();
+  std::vector<char> bytes;
+  put<int32_t>(bytes, db.n_graphs);
+  put<int32_t>(bytes, (int32_t)edges.size());
+  put<int32_t>(bytes, m);
+  put<int32_t>(bytes, (int32_t)db.n_vertices());
+  for (int32_t x : edges) put<int32_t>(bytes, x);
+  for (int32_t x : elabs) put<int32_t>(bytes, x);
+  for (int32_t x : vlabs) put<int32_t>(bytes, x);
+  for (int32_t x : voffs) put<int32_t>(bytes, x);
+  write_file(path, bytes);
 }
 
 void read_cudabin(const std::string &path, HostDB &db) {
@@ -212,10 +361,11 @@ void read_cudabin(const std::string &path, HostDB &db) {
       while (nx < slots && voffs[nx] == -1) ++nx;
       int64_t e = voffs[nx];
       if (s < 0 || e < s || e > ne) throw Error(GM_E_ARG, "corrupt cudabin offsets in " + path);
+      bool loop_open = false;
       for (int64_t k = s; k < e; ++k) {
         int32_t to = (int32_t)(edges[k] - base);
         if (to < 0 || to >= nv) throw Error(GM_E_ARG, "corrupt cudabin adjacency in " + path);
-        if (v < to) {
+        if (v < to || (v == to && (loop_open = !loop_open))) {
           ef.push_back(v);
           et.push_back(to);
           el.push_back(elabs[k]);

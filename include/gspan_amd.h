@@ -141,6 +141,18 @@ int gm_db_load(const char *path, const char *filetype, gm_db **out);
  * the format Graph::read parses (src/globals/graph_types.cpp:66-113). */
 int gm_db_write_txt(const gm_db *db, const char *path);
 
+/* Write the database in `filetype` "txt", "bin" or "cudabin" (optionally
+ * '-'-prefixed).  Replaces dbio::write_database_bin /
+ * write_database_cudabin (src/globals/dbio.hpp; dbio.cpp:209-268,
+ * 346-370); bin rejects an empty database as the reference does. */
+int gm_db_write(const gm_db *db, const char *path, const char *filetype);
+
+/* Read part `part` of `total_parts` of a BIN file: transactions
+ * [n / total_parts * part, n / total_parts * (part + 1)), the last part up
+ * to n.  Replaces dbio::read_database_bin(filename, gdb, total_parts, part)
+ * (src/globals/dbio.cpp:275-329). */
+int gm_db_load_part(const char *path, int32_t total_parts, int32_t part, gm_db **out);
+
 /* Sizes of the database: counts[0..5] = n_graphs, n_vertices, n_edges,
  * max vertices per graph, distinct vertex labels, distinct edge labels. */
 int gm_db_counts(const gm_db *db, int64_t counts[6]);
