@@ -46,7 +46,7 @@ EXPORTED_SYMBOLS = (
     "gm_db_write_txt", "gm_db_counts", "gm_db_export", "gm_db_upload", "gm_db_destroy", "gm_mine",
     "gm_mine_batch", "gm_dfs_code_is_min", "gm_gen_random", "gm_gen_planted", "gm_gen_molecule", "gm_db_subset",
     "gm_batch_create", "gm_batch_mine", "gm_batch_destroy", "gm_support_queries", "gm_support_query",
-    "gm_db_write", "gm_db_load_part",
+    "gm_db_write", "gm_db_load_part", "gm_mine_noalloc",
 )
 
 
@@ -106,6 +106,7 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "gm_db_upload": (C.c_int, [dbp]),
         "gm_db_destroy": (None, [dbp]),
         "gm_mine": (C.c_int, [dbp, C.c_int32, P(gm_cfg), PATTERN_CB, C.c_void_p, P(gm_stats)]),
+        "gm_mine_noalloc": (C.c_int, [dbp, C.c_int32, P(gm_cfg), PATTERN_CB, C.c_void_p, P(gm_stats)]),
         "gm_mine_batch": (C.c_int, [C.c_int32, P(dbp), i32p, P(gm_cfg), JOB_PATTERN_CB, C.c_void_p, P(gm_stats)]),
         "gm_dfs_code_is_min": (C.c_int, [i32p, C.c_int32]),
         "gm_gen_random": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_int32,
@@ -485,6 +486,26 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
                     path_pruning, speculate, stage_events, chunk_rows, table_bits)
     st = gm_stats()
     _check(lib.gm_mine(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st)))
+    return out, st.as_dict()
+
+
+def mine_noalloc(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, collect_proj: bool = False,
+                 count_only: bool = False):
+    """mine() through gm_mine_noalloc: no device allocation, every buffer of
+    the level steps from the library's static 64 MiB device array (the
+    database must be uploaded first).  Returns (patterns, stats) like
+    mine()."""
+    lib = load_library()
+    out: list = []
+
+    def cb(_user, dfs, n, sup, nproj):
+        out.append((DFSCode(tuple(dfs[5 * i + j] for j in range(5)) for i in range(n)), int(sup), int(nproj)))
+        return 0
+
+    ccb = PATTERN_CB() if count_only else PATTERN_CB(cb)
+    cfg = _make_cfg(label_style, 0, 0, collect_proj)
+    st = gm_stats()
+    _check(lib.gm_mine_noalloc(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st)))
     return out, st.as_dict()
 
 

@@ -9,12 +9,14 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 
 #include "gspan_amd.h"
 #include "gm_internal.hpp"
 #include "miner.hpp"
+#include "kernels.hpp"
 
 struct gm_db {
   gm::HostDB host;
@@ -292,6 +294,46 @@ int gm_mine(gm_db *db, int32_t abs_minsup, const gm_cfg *cfg, gm_pattern_cb cb, 
         return cb(user, flat.data(), n, sup, nproj) == 0;
       };
     if (!gm::mine(*db->dev, o, rep, stats, db->stream)) throw gm::Error(GM_E_CALLBACK, "callback stopped mining");
+  });
+}
+
+int gm_mine_noalloc(gm_db *db, int32_t abs_minsup, const gm_cfg *cfg, gm_pattern_cb cb, void *user,
+                    gm_stats *stats) {
+  return guard([&]() {
+    if (!db) throw gm::Error(GM_E_ARG, "null database");
+    if (abs_minsup < 1) throw gm::Error(GM_E_ARG, "absolute min support < 1");
+    if (!db->dev) throw gm::Error(GM_E_ARG, "gm_mine_noalloc needs a database already on the device (gm_db_upload)");
+    void *base = nullptr;
+    size_t bytes = 0;
+    gm::hip_check(gm::static_scratch(&base, &bytes), "static workspace");
+    gm::MineOptions o = options(cfg);
+    o.minsup = {abs_minsup};
+    o.stage_events = false;  // events are allocations
+    const int style = cfg ? cfg->label_style : GM_LABELS_FULL;
+    std::vector<int32_t> flat;
+    gm::ReportFn rep;
+    if (cb)
+      rep = [&](int32_t, const gm::DFS *code, int32_t n, uint32_t sup, uint64_t nproj) {
+        flatten(code, n, style, flat);
+        return cb(user, flat.data(), n, sup, nproj) == 0;
+      };
+    // one call at a time per device: every call shares the device's array
+    static std::mutex mu[64];
+    int dev = 0;
+    gm::hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(mu[dev & 63]);
+    gm::Arena arena;
+    arena.base = static_cast<char *>(base);
+    arena.cap = bytes;
+    arena.hi = bytes;
+    gm::Workspace ws;
+    ws.use_arena(&arena);
+    // the call's permanent buffer (the min supports) at the bottom of side 0
+    ws.minsup.arena = &arena;
+    ws.minsup.side = 0;
+    ws.minsup.ensure(sizeof(int32_t));
+    arena.lo0 = arena.lo;
+    if (!gm::mine(*db->dev, o, rep, stats, db->stream, &ws)) throw gm::Error(GM_E_CALLBACK, "callback stopped mining");
   });
 }
 

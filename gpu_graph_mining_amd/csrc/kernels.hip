@@ -61,6 +61,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "device.hpp"
 #include "kernels.hpp"
@@ -1050,6 +1051,11 @@ __global__ void k_copy8(unsigned long long *dst, const unsigned long long *src, 
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+__global__ void k_fill8(unsigned long long *dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = 0ull;
+}
+
 __global__ void __launch_bounds__(256) k_clear_counters(StepArgs a) {
   if (threadIdx.x == 0) {
     *a.freq_count = 0;
@@ -1161,9 +1167,45 @@ hipError_t launch_cursors(const StepArgs &a, void *dst, const void *src, size_t 
   return hipGetLastError();
 }
 
+hipError_t launch_fill8(void *dst, size_t bytes, hipStream_t s) {
+  if (bytes % 8) return hipErrorInvalidValue;
+  const int64_t n = (int64_t)(bytes / 8);
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill8, dim3(copy_grid(n)), dim3(256), 0, s, static_cast<unsigned long long *>(dst), n);
+  return hipGetLastError();
+}
+
 hipError_t launch_clear_counters(const StepArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_counters, dim3(1), dim3(256), 0, s, a);
   return hipGetLastError();
+}
+
+// ------------------------------------------------------ static workspace
+// The entry point that allocates nothing (gm_mine_noalloc) takes every
+// buffer of its level steps from this array.  Each GPU the library runs on
+// holds its own copy until the process ends.  In host code `scratch` only
+// names the symbol: its device address is looked up once per device with
+// hipGetSymbolAddress, with that device current, and only that address is
+// handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+hipError_t static_scratch(void **ptr, size_t *bytes) {
+  static std::mutex mu;
+  static void *addr[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = p;
+  }
+  *ptr = addr[dev];
+  *bytes = kStaticScratchBytes;
+  return hipSuccess;
 }
 
 }  // namespace gm

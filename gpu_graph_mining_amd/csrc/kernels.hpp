@@ -135,6 +135,13 @@ hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s)
 hipError_t launch_cursors(const StepArgs &a, void *dst, const void *src, size_t bytes,
                           const unsigned long long *fkeys, const int32_t *fchild, const unsigned long long *row_base,
                           const unsigned long long *list_base, int64_t F, hipStream_t s);
+// Zero `bytes` (a multiple of 8) of device memory with a kernel.
+hipError_t launch_fill8(void *dst, size_t bytes, hipStream_t s);
+// The static workspace of the entry point that allocates nothing: a
+// file-scope device array of kStaticScratchBytes in kernels.hip; *ptr gets
+// its address on the current device (looked up once per device).
+constexpr size_t kStaticScratchBytes = 64ull << 20;
+hipError_t static_scratch(void **ptr, size_t *bytes);
 // Zero the per-level counters (frequent-key count, failure list, spread
 // statistics but the written-row slots).
 hipError_t launch_clear_counters(const StepArgs &a, hipStream_t s);

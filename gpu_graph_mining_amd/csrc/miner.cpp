@@ -51,15 +51,41 @@ void hip_check(hipError_t e, const char *what) {
   throw Error(GM_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+void *Arena::take(int side, size_t bytes) {
+  if (side < 0) side = scratch_side;
+  size_t at;
+  if (side == 0) {
+    at = (lo + 255) & ~size_t(255);
+    if (at + bytes > hi) at = cap;  // meets side 1
+    else lo = at + bytes;
+  } else {
+    at = bytes <= hi ? (hi - bytes) & ~size_t(255) : cap;
+    if (at < lo || at >= cap) at = cap;
+    else hi = at;
+  }
+  if (at >= cap)
+    throw Error(GM_E_OOM, "the level steps need more than the " + std::to_string(cap >> 20) +
+                              " MiB static workspace (" + std::to_string((lo + (cap - hi) + bytes) >> 20) +
+                              " MiB asked)");
+  peak = std::max(peak, lo + (cap - hi));
+  return base + at;
+}
+
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p && !arena) (void)hipFree(p);
   p = nullptr;
   cap = 0;
 }
 
 void *DevBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return p;
-  const size_t want = std::max<size_t>({bytes, cap + cap / 2, 256});
+  if (arena) {  // exact sizes (whole 256-byte blocks); the stacks are emptied level by level
+    release();
+    cap = (std::max<size_t>(bytes, 256) + 255) & ~size_t(255);
+    p = arena->take(side, cap);
+    return p;
+  }
+  const size_t want = (std::max<size_t>({bytes, cap + cap / 2, 256}) + 255) & ~size_t(255);  // fills take 8-byte words
   release();
   hip_check(hipMalloc(&p, want), "hipMalloc");
   cap = want;
@@ -67,11 +93,18 @@ void *DevBuf::ensure(size_t bytes) {
 }
 
 HostBuf::~HostBuf() {
-  if (p) (void)hipHostFree(p);
+  if (p && !arena) (void)hipHostFree(p);
 }
 
 void *HostBuf::ensure(size_t bytes) {
   if (bytes <= cap && p) return p;
+  if (arena) {
+    host.resize(std::max<size_t>(bytes, 256));
+    p = host.data();
+    dev = arena->take(-1, host.size());
+    cap = host.size();
+    return p;
+  }
   const size_t want = std::max<size_t>({bytes, cap + cap / 2, 4096});
   if (p) (void)hipHostFree(p);
   p = nullptr;
@@ -497,23 +530,46 @@ static_assert(sizeof(Counters) == 64 + 8 * kSpread * kSpreadStride, "counter blo
 // allocation may still be in use by queued kernels.
 void *grow(DevBuf &b, size_t bytes, hipStream_t s) {
   if (bytes <= b.cap && b.p) return b.p;
-  hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  // (arena regions are only reused by work queued later on the stream)
+  if (!b.arena) hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   return b.ensure(bytes);
 }
 
 // Grow the packed counters and clear them (their zero invariant covers all
-// of the buffer).
+// of the buffer); a fill kernel, not hipMemsetAsync (which graph capture
+// does not replay reliably).
 void grow_zero(DevBuf &b, size_t bytes, hipStream_t s) {
   if (bytes <= b.cap && b.p) return;
   grow(b, bytes, s);
-  hip_check(hipMemsetAsync(b.p, 0, b.cap, s), "memset counters");
+  hip_check(launch_fill8(b.p, b.cap, s), "clear counters");
+}
+
+// Device -> host staging: the copy kernel into mapped memory, or (arena
+// workspace) an explicit copy into the host side.
+void stage_down(Workspace &w, HostBuf &hb, const void *src, size_t bytes, hipStream_t s) {
+  if (w.arena)
+    hip_check(hipMemcpyAsync(hb.p, src, bytes, hipMemcpyDeviceToHost, s), "download");
+  else
+    hip_check(launch_copy8(hb.dev, src, bytes, s), "download");
+}
+
+// Host -> device staging for kernels that read hb.dev: nothing to do for
+// mapped memory; the arena workspace copies the bytes [off, off + bytes).
+void stage_up(Workspace &w, HostBuf &hb, size_t off, size_t bytes, hipStream_t s) {
+  if (w.arena && bytes) {
+    hip_check(hipMemcpyAsync(static_cast<char *>(hb.dev) + off, static_cast<const char *>(hb.p) + off, bytes,
+                             hipMemcpyHostToDevice, s),
+              "upload");
+    hip_check(hipStreamSynchronize(s), "upload");  // the host side is rewritten next level
+  }
 }
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 }  // namespace
 
-bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats *stats, hipStream_t s) {
+bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats *stats, hipStream_t s,
+          Workspace *ws) {
   std::lock_guard<std::mutex> lock(db.mu);
   Timer total;
   const int32_t NEL = db.NEL, NVL = db.NVL;
@@ -521,11 +577,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
                                       : (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
   gm_stats st;
   std::memset(&st, 0, sizeof(st));
-  if (!db.ws) db.ws.reset(new Workspace());
-  Workspace &w = *db.ws;
+  if (!ws && !db.ws) db.ws.reset(new Workspace());
+  Workspace &w = ws ? *ws : *db.ws;
   if (!w.acc_zero) {
-    if (w.acc.p) hip_check(hipMemsetAsync(w.acc.p, 0, w.acc.cap, s), "memset counters");
-    if (w.accb.p) hip_check(hipMemsetAsync(w.accb.p, 0, w.accb.cap, s), "memset counters");
+    if (w.acc.p) hip_check(launch_fill8(w.acc.p, w.acc.cap, s), "clear counters");
+    if (w.accb.p) hip_check(launch_fill8(w.accb.p, w.accb.cap, s), "clear counters");
   }
   w.acc_zero = false;  // until this call ends normally
 
@@ -606,6 +662,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   for (;;) {
     const bool root = level == 0;
     if (!root && L.n == 0) break;
+    if (w.arena) {
+      w.arena_level(cur);
+      counters_fresh = true;  // the counter block is taken again
+    }
     Timer tdev;
     const int64_t P = root ? db.n_jobs : L.n;
     const int64_t K = root ? (int64_t)NVL * NEL * NVL : (int64_t)S * NEL * (NVL + 1);
@@ -624,8 +684,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // counter replicas: chunk i counts into replica i mod rep, so that the
     // keys every chunk hits (the root's) are not one hot address (atomics
     // on one address serialise at ~100 ns each); up to 64 MB of them
+    // (one replica in the static workspace of the entry point that
+    // allocates nothing)
     int rep = 1;
-    while (rep < 64 && (int64_t)rep * 2 * nkeys * 16 <= (64ll << 20) && (int64_t)rep * 2 * 32 <= total_chunks) rep *= 2;
+    while (!w.arena && rep < 64 && (int64_t)rep * 2 * nkeys * 16 <= (64ll << 20) && (int64_t)rep * 2 * 32 <= total_chunks)
+      rep *= 2;
     grow(w.sup, sizeof(int32_t) * std::max<int64_t>(nkeys, 1), s);
     grow_zero(w.acc, sizeof(unsigned long long) * std::max<int64_t>(nkeys * rep, 1), s);
     grow_zero(w.accb, sizeof(unsigned long long) * std::max<int64_t>(nkeys * rep, 1), s);
@@ -652,7 +715,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       dctr = w.freq.as<Counters>();
       if (counters_fresh) {
         // written rows of an earlier level live in the old block: checked already
-        hip_check(hipMemsetAsync(dctr, 0, sizeof(Counters), s), "clear counters");
+        hip_check(launch_fill8(dctr, sizeof(Counters), s), "clear counters");
         written_expect = 0;
         counters_fresh = false;
       }
@@ -689,7 +752,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.pmask = a.mask_stride ? reinterpret_cast<const unsigned long long *>(w.upl[cur].as<char>() + mask_off)
                               : nullptr;
       hip_check(launch_clear_counters(a, s), "clear counters");
-      const bool time_stage = opt.stage_events && !root;
+      const bool time_stage = opt.stage_events && !root && !w.arena;
       if (time_stage) {
         for (hipEvent_t &e : w.ev)
           if (!e) hip_check(hipEventCreate(&e), "hipEventCreate");
@@ -720,7 +783,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       guess = std::min<uint32_t>(fcap, std::max<uint32_t>(1024, 2 * last_fcount));
       const size_t head = sizeof(Counters) + sizeof(Freq) * guess;
       hbuf = static_cast<const char *>(w.down.ensure(head));
-      hip_check(launch_copy8(w.down.dev, dctr, head, s), "download counters");
+      stage_down(w, w.down, dctr, head, s);
       hip_check(hipStreamSynchronize(s), "k_level");
       std::memcpy(&cnt, hbuf, sizeof(cnt));
       if (cnt.err) throw Error(GM_E_DEVICE, "internal: level step error " + std::to_string(cnt.err));
@@ -772,8 +835,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         std::memcpy(frall.data(), src, sizeof(Freq) * guess);
         const size_t rest = sizeof(Freq) * (F - guess);
         char *h2 = static_cast<char *>(w.down.ensure(rest));
-        hip_check(launch_copy8(w.down.dev, reinterpret_cast<const char *>(a.freq) + sizeof(Freq) * guess, rest, s),
-                  "download frequent keys");
+        stage_down(w, w.down, reinterpret_cast<const char *>(a.freq) + sizeof(Freq) * guess, rest, s);
         hip_check(hipStreamSynchronize(s), "download frequent keys");
         std::memcpy(frall.data() + guess, h2, rest);
         src = frall.data();
@@ -935,6 +997,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       }
       N.row0[C] = run;
     }
+    if (F > 0) stage_up(w, w.up, fk_off, masks_off - fk_off, s);  // keys, child ids, bases
     if (F > 0 && C == 0) {  // nothing to write: the frequent keys' replicas are cleared
       const char *su = static_cast<const char *>(w.up.dev);
       hip_check(launch_cursors(a, nullptr, nullptr, 0, reinterpret_cast<const unsigned long long *>(su + fk_off),
@@ -1108,7 +1171,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         for (int64_t i = 0; i < F; ++i)
           if (kidx[i] < 0) nonmin_rows += (int64_t)fr[i].proj;
       dead_share = (double)nonmin_rows / (double)std::max<int64_t>(1, alive_rows + nonmin_rows);
-      hip_check(launch_copy8(du, su, up_bytes, s), "upload children");
+      if (w.arena) {
+        hip_check(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, s), "upload children");
+        hip_check(hipStreamSynchronize(s), "upload children");
+      }
+      else
+        hip_check(launch_copy8(du, su, up_bytes, s), "upload children");
       us_overlap = t_over.sec() * 1e6;
       dev_sec += tdev2.sec();
       size_t live = db.bytes + w.sup.cap + w.acc.cap + w.accb.cap + w.freq.cap + w.rccnt.cap + w.cand.cap +
@@ -1176,7 +1244,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   st.seconds_ext_kernel = ext_ms * 1e-3;
   st.seconds_stage_events = ev_stage_ms * 1e-3;
   st.ext_kernel_launches = ext_launches;
-  st.peak_device_bytes = peak;
+  st.peak_device_bytes = w.arena ? w.arena->peak : peak;
   if (stats) *stats = st;
   return !stopped;
 }

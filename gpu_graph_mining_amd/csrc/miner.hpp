@@ -15,10 +15,37 @@ namespace gm {
 
 void hip_check(hipError_t e, const char *what);
 
-// Growable device allocation (grows only; freed with the owner).
+// A fixed device region (the static workspace of the entry point that
+// allocates nothing) used as two stacks: side 0 grows up from the bottom,
+// side 1 down from the top.  The rows of a level live on side (level id
+// parity, the Workspace's double-buffer index); while a level step runs,
+// its other buffers (scratch) are stacked on the side that holds its rows,
+// and the other side is emptied for the children's rows.  Everything on one
+// stream, so a region is only reused by work ordered after its last use.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0;
+  size_t lo = 0, hi = 0;   // side 0 used [0, lo), side 1 used [hi, cap)
+  size_t lo0 = 0;          // side 0's origin (after the call's permanent buffers)
+  int scratch_side = 0;    // where scratch goes during the current level
+  size_t peak = 0;         // most bytes in use at once
+  void *take(int side, size_t bytes);  // Error(GM_E_OOM) when the stacks meet
+  void reset(int side) {
+    if (side == 0)
+      lo = lo0;
+    else
+      hi = cap;
+  }
+};
+
+// Growable device allocation (grows only; freed with the owner).  With an
+// arena, the memory is taken from it instead of hipMalloc: from side
+// `side` (0 / 1), or from the current scratch side (side < 0).
 struct DevBuf {
   void *p = nullptr;
   size_t cap = 0;
+  Arena *arena = nullptr;
+  int side = -1;
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
@@ -31,10 +58,14 @@ struct DevBuf {
 
 // Growable pinned host allocation, mapped into the device's address space:
 // the level step's copy kernels read and write it on the compute queue.
+// With an arena: ordinary host memory `p` and a device copy `dev` taken from
+// the arena, moved by explicit copies.
 struct HostBuf {
   void *p = nullptr;
-  void *dev = nullptr;  // the same memory as the device addresses it (mapped)
+  void *dev = nullptr;  // the same memory as the device addresses it (mapped), or its device copy
   size_t cap = 0;
+  Arena *arena = nullptr;
+  std::vector<char> host;  // arena mode: the host side
   HostBuf() = default;
   HostBuf(const HostBuf &) = delete;
   HostBuf &operator=(const HostBuf &) = delete;
@@ -56,6 +87,44 @@ struct Workspace {
   // the packed counters hold zeros outside a level step (k_reduce clears
   // what it reads); false after a call that ended early
   bool acc_zero = false;
+  // every buffer from this region (no device allocation, no pinned host
+  // memory, no events); null: ordinary allocations
+  Arena *arena = nullptr;
+  void use_arena(Arena *ar) {
+    arena = ar;
+    for (DevBuf *b : scratch_bufs()) b->arena = ar;
+    for (int i = 0; i < 2; ++i)
+      for (DevBuf *b : row_bufs(i)) {
+        b->arena = ar;
+        b->side = i;
+      }
+    up.arena = ar;
+    down.arena = ar;
+  }
+  // Arena workspace, at the start of a level step whose rows are on side
+  // `cur`: the scratch buffers are taken again on that side, the other side
+  // is emptied for the children's rows.
+  void arena_level(int cur) {
+    arena->reset(cur ^ 1);
+    arena->scratch_side = cur;
+    for (DevBuf *b : scratch_bufs()) {
+      b->p = nullptr;
+      b->cap = 0;
+    }
+    for (DevBuf *b : row_bufs(cur ^ 1)) {
+      b->p = nullptr;
+      b->cap = 0;
+    }
+    for (HostBuf *h : {&up, &down}) {
+      h->dev = nullptr;
+      h->p = nullptr;
+      h->cap = 0;
+    }
+  }
+  std::vector<DevBuf *> scratch_bufs() {
+    return {&sup, &acc, &accb, &freq, &rccnt, &cur_rows, &cur_list, &cursor, &fail, &chunks, &chunk_keys, &cand};
+  }
+  std::vector<DevBuf *> row_bufs(int i) { return {&pid[i], &rgraph[i], &vt[i], &rbound[i], &rlist[i], &upl[i]}; }
   ~Workspace() {
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -131,7 +200,9 @@ struct MineOptions {
 };
 
 // Runs the level-synchronous miner; returns false when the report callback
-// asked to stop.
-bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats *stats, hipStream_t s);
+// asked to stop.  `ws`: the workspace to use instead of the database's own
+// (the arena-backed one of the entry point that allocates nothing).
+bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats *stats, hipStream_t s,
+          Workspace *ws = nullptr);
 
 }  // namespace gm

@@ -182,6 +182,21 @@ void gm_db_destroy(gm_db *db);
 int gm_mine(gm_db *db, int32_t abs_minsup, const gm_cfg *cfg, gm_pattern_cb cb, void *user,
             gm_stats *stats);
 
+/* gm_mine without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  Every buffer of the
+ * level steps comes from a static device array of 64 MiB per GPU (held
+ * until the process ends; host staging uses ordinary host memory and
+ * explicit copies instead of pinned memory; no events).  The database must
+ * already be on the device (gm_db_upload; GM_E_ARG otherwise).  A run whose
+ * level steps need more than the 64 MiB fails with GM_E_OOM.  Calls on one
+ * GPU share its array, which keeps the last call's contents: they take
+ * turns (a lock per device), and calls that overlap on one GPU from
+ * different processes each have their own array.  Same results as gm_mine;
+ * cfg.stage_events is ignored. */
+int gm_mine_noalloc(gm_db *db, int32_t abs_minsup, const gm_cfg *cfg, gm_pattern_cb cb, void *user,
+                    gm_stats *stats);
+
 /* Mine n_jobs independent databases in one batched schedule: the level
  * steps of every job run in the same kernel launches.  Each job has its own
  * database and absolute min support; results are identical to mining each
