@@ -137,6 +137,29 @@ class GMError(RuntimeError):
         self.code = code
 
 
+class _Guard:
+    """Wraps a pattern callback so that no Python exception is lost in
+    ctypes (which would print it and return 0): the exception is kept, the
+    callback returns 1 (stop mining) and raise_pending() re-raises it after
+    the library call returned."""
+
+    def __init__(self, fn):
+        self.fn = fn
+        self.error: Optional[BaseException] = None
+
+    def __call__(self, *args):
+        try:
+            return self.fn(*args)
+        except BaseException as e:  # noqa: BLE001 -- re-raised by raise_pending
+            if self.error is None:
+                self.error = e
+            return 1
+
+    def raise_pending(self) -> None:
+        if self.error is not None:
+            raise self.error
+
+
 def _check(rc: int) -> None:
     if rc != GM_OK:
         raise GMError(rc, load_library().gm_last_error().decode(errors="replace"))
@@ -481,11 +504,14 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
         out.append((code, int(sup), int(nproj)))
         return 0
 
-    ccb = PATTERN_CB() if count_only else PATTERN_CB(cb)  # PATTERN_CB() is a NULL pointer
+    guard = _Guard(cb)
+    ccb = PATTERN_CB() if count_only else PATTERN_CB(guard)  # PATTERN_CB() is a NULL pointer
     cfg = _make_cfg(label_style, max_edges, canon_threads, collect_proj, candidate_lists, block_threads,
                     path_pruning, speculate, stage_events, chunk_rows, table_bits)
     st = gm_stats()
-    _check(lib.gm_mine(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st)))
+    rc = lib.gm_mine(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st))
+    guard.raise_pending()
+    _check(rc)
     return out, st.as_dict()
 
 
@@ -502,10 +528,13 @@ def mine_noalloc(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL,
         out.append((DFSCode(tuple(dfs[5 * i + j] for j in range(5)) for i in range(n)), int(sup), int(nproj)))
         return 0
 
-    ccb = PATTERN_CB() if count_only else PATTERN_CB(cb)
+    guard = _Guard(cb)
+    ccb = PATTERN_CB() if count_only else PATTERN_CB(guard)
     cfg = _make_cfg(label_style, 0, 0, collect_proj)
     st = gm_stats()
-    _check(lib.gm_mine_noalloc(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st)))
+    rc = lib.gm_mine_noalloc(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st))
+    guard.raise_pending()
+    _check(rc)
     return out, st.as_dict()
 
 
@@ -520,12 +549,15 @@ def mine_batch(dbs: Sequence[GraphDB], minsups: Sequence[int], *, label_style: i
         res[job].append((DFSCode(tuple(dfs[5 * i + j] for j in range(5)) for i in range(n)), int(sup), int(nproj)))
         return 0
 
-    ccb = JOB_PATTERN_CB() if count_only else JOB_PATTERN_CB(cb)
+    guard = _Guard(cb)
+    ccb = JOB_PATTERN_CB() if count_only else JOB_PATTERN_CB(guard)
     handles = (C.c_void_p * len(dbs))(*[d._h.value for d in dbs])
     ms = _i32(minsups)
     cfg = _make_cfg(label_style, 0, canon_threads, collect_proj)
     st = gm_stats()
-    _check(lib.gm_mine_batch(len(dbs), handles, _ptr(ms), C.byref(cfg), ccb, None, C.byref(st)))
+    rc = lib.gm_mine_batch(len(dbs), handles, _ptr(ms), C.byref(cfg), ccb, None, C.byref(st))
+    guard.raise_pending()
+    _check(rc)
     return res, st.as_dict()
 
 
@@ -571,11 +603,14 @@ class Batch:
                              int(nproj)))
             return 0
 
-        ccb = JOB_PATTERN_CB() if count_only else JOB_PATTERN_CB(cb)
+        guard = _Guard(cb)
+        ccb = JOB_PATTERN_CB() if count_only else JOB_PATTERN_CB(guard)
         ms = _i32(minsups)
         cfg = _make_cfg(label_style, 0, canon_threads, collect_proj, path_pruning=path_pruning, speculate=speculate)
         st = gm_stats()
-        _check(lib.gm_batch_mine(self._h, _ptr(ms), C.byref(cfg), ccb, None, C.byref(st)))
+        rc = lib.gm_batch_mine(self._h, _ptr(ms), C.byref(cfg), ccb, None, C.byref(st))
+        guard.raise_pending()
+        _check(rc)
         return res, st.as_dict()
 
     def close(self) -> None:

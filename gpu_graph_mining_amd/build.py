@@ -92,7 +92,23 @@ def build(verbose: bool = False, out: Path = LIB, build_dir: Path = BUILD, defin
     tmp = out.with_suffix(".so.tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, out)
+    if out == LIB:
+        build_cli()
     return out
+
+
+CLI_SRC = PKG / "cli" / "gspan_amd_main.cpp"
+CLI = LIB_DIR / "gspan_amd"
+
+
+def build_cli() -> Path:
+    """The command-line driver (cli/gspan_amd_main.cpp), plain C++ against
+    include/gspan_amd.hpp, linked to the library next to it."""
+    tmp = CLI.with_suffix(".tmp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{PKG.parent / 'include'}", str(CLI_SRC), "-o", str(tmp),
+          f"-L{LIB_DIR}", "-lgspan_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, CLI)
+    return CLI
 
 
 def build_oracle() -> None:

@@ -1,0 +1,235 @@
+/*
+ * gspan_amd.hpp -- C++ driver surface of the MI355X-native gSpan miner,
+ * header-only over the C ABI of gspan_amd.h.
+ *
+ * It keeps the method names and call order of the reference's mining
+ * driver so that a reference-style driver (src/gspan_cuda/common_main.cpp:
+ * 85-117) compiles against it with only its includes and namespaces
+ * changed:
+ *
+ *   reference (file:line)                           here
+ *   gspan_cuda::gspan_cuda (gspan_cuda.hpp:281-292)  gspan_amd::gspan_cuda
+ *   get_gspan_cuda_instance() (common_main.cpp:28)   gspan_amd::get_gspan_cuda_instance()
+ *   graph_output / graph_counting_output /
+ *     graph_file_output (graph_output.hpp:9-52)      gspan_amd::graph_output / ...
+ *   types::DFS / types::DFSCode to_string
+ *     (dfs_code.cpp:109-118, 199-214)                gspan_amd::DFS / DFSCode
+ *   dbio::read_database / ftype2str
+ *     (dbio.hpp:8-37; dbio.cpp:33-41)                gspan_amd::graph_database::load
+ *   cudapp::cuda_configurator
+ *     (cuda_configurator.hpp)                        gspan_amd::cuda_configurator
+ *
+ * Nothing here talks to the GPU itself: every call goes through gm_*.  Errors
+ * become std::runtime_error, as the reference's CUDA_EXEC checks throw
+ * (src/cuda_pp/cuda_tools.hpp:10-18).
+ */
+#ifndef GSPAN_AMD_HPP
+#define GSPAN_AMD_HPP
+
+#include <cstdint>
+#include <exception>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gspan_amd.h"
+
+namespace gspan_amd {
+
+inline void check(int rc, const char *what) {
+  if (rc != GM_OK) throw std::runtime_error(std::string(what) + ": [" + std::to_string(rc) + "] " + gm_last_error());
+}
+
+// One DFS tuple (from, to, fromlabel, elabel, tolabel).
+struct DFS {
+  int from = 0, to = 0, fromlabel = 0, elabel = 0, tolabel = 0;
+  bool is_forward() const { return from < to; }
+  bool is_backward() const { return from > to; }
+  // "F(0 1 2 0 3)" / "B(...)" with the edge type, "(0 1 2 0 3)" without.
+  std::string to_string(bool print_edge_type = true) const {
+    std::stringstream ss;
+    if (print_edge_type) ss << (is_forward() ? "F" : "B");
+    ss << "(" << from << " " << to << " " << fromlabel << " " << elabel << " " << tolabel << ")";
+    return ss.str();
+  }
+};
+
+// A DFS code; to_string joins the tuples with ';'.
+struct DFSCode : public std::vector<DFS> {
+  std::string to_string(bool print_edge_type = true) const {
+    std::string s;
+    for (size_t i = 0; i < size(); ++i) {
+      if (i) s += ";";
+      s += (*this)[i].to_string(print_edge_type);
+    }
+    return s;
+  }
+};
+
+// Pattern sinks (graph_output.hpp:9-52).
+class graph_output {
+ public:
+  virtual void output_graph(const DFSCode &code, size_t support = 0) = 0;
+  virtual double get_count() = 0;
+  virtual std::string to_string() = 0;
+  virtual ~graph_output() {}
+};
+
+class graph_counting_output : public graph_output {
+  double counter = 0;
+
+ public:
+  void output_graph(const DFSCode &, size_t = 0) override { counter += 1.0; }
+  double get_count() override { return counter; }
+  std::string to_string() override {
+    std::stringstream ss;
+    ss << counter;
+    return ss.str();
+  }
+};
+
+// One code.to_string(false) line per pattern (graph_output.cpp:90-95).
+class graph_file_output : public graph_output {
+  std::ofstream out;
+  std::string filename;
+  double count = 0;
+
+ public:
+  explicit graph_file_output(const std::string &fname) : out(fname.c_str()), filename(fname) {
+    if (!out) throw std::runtime_error("cannot open " + fname);
+  }
+  void output_graph(const DFSCode &code, size_t = 0) override {
+    out << code.to_string(false) << "\n";
+    if (!out) throw std::runtime_error("cannot write " + filename);
+    count += 1.0;
+  }
+  double get_count() override { return count; }
+  std::string to_string() override { return "graph_file_output, filename: " + filename; }
+};
+
+// A graph database (types::graph_database_t with the dbio readers).
+class graph_database {
+  gm_db *db_ = nullptr;
+
+ public:
+  graph_database() = default;
+  graph_database(const graph_database &) = delete;
+  graph_database &operator=(const graph_database &) = delete;
+  ~graph_database() { gm_db_destroy(db_); }
+  // filetype "txt", "fsg", "bin" or "cudabin", optionally '-'-prefixed
+  void load(const std::string &filetype, const std::string &filename) {
+    gm_db *d = nullptr;
+    check(gm_db_load(filename.c_str(), filetype.c_str(), &d), "read_database");
+    gm_db_destroy(db_);
+    db_ = d;
+  }
+  size_t size() const {
+    if (!db_) return 0;
+    int64_t c[6];
+    check(gm_db_counts(db_, c), "gm_db_counts");
+    return (size_t)c[0];
+  }
+  gm_db *handle() const { return db_; }
+};
+
+// Launch parameters by name, as cudapp::cuda_configurator holds the
+// reference's per-kernel block sizes.  The names that mean something here
+// are the gm_cfg fields (label_style, max_edges, canon_threads,
+// path_pruning, speculate, chunk_rows, table_bits, collect_proj); the
+// reference's own kernel names are accepted and have no effect (their
+// kernels do not exist here).
+class cuda_configurator {
+  std::map<std::string, int> p_;
+
+ public:
+  void set_parameter(const std::string &name, int value) { p_[name] = value; }
+  int get_parameter(const std::string &name) const {
+    auto it = p_.find(name);
+    if (it == p_.end()) throw std::runtime_error("unknown parameter " + name);
+    return it->second;
+  }
+  void fill(gm_cfg &cfg) const {
+    gm_cfg_init(&cfg);
+    auto get = [&](const char *k, int32_t &field) {
+      auto it = p_.find(k);
+      if (it != p_.end()) field = it->second;
+    };
+    get("label_style", cfg.label_style);
+    get("max_edges", cfg.max_edges);
+    get("canon_threads", cfg.canon_threads);
+    get("path_pruning", cfg.path_pruning);
+    get("speculate", cfg.speculate);
+    get("chunk_rows", cfg.chunk_rows);
+    get("table_bits", cfg.table_bits);
+    get("collect_proj", cfg.collect_proj);
+  }
+};
+
+// The mining driver: set_database, set_min_support, set_graph_output,
+// set_exec_configurator, run, delete_database_from_device
+// (gspan_cuda.hpp:281-292).  Patterns reach the graph_output fully
+// labelled, as gspan_cuda reports them (gspan_cuda_no_sort.cpp:409-416).
+class gspan_cuda {
+ protected:
+  graph_database *db_ = nullptr;
+  int minsup_ = 0;
+  graph_output *out_ = nullptr;
+  cuda_configurator *cc_ = nullptr;
+
+ public:
+  gspan_cuda() = default;
+  virtual ~gspan_cuda() {}
+  // convert_to_cuda: copy the device layout now (else the first run does)
+  void set_database(graph_database &graph_database, bool convert_to_cuda = true) {
+    db_ = &graph_database;
+    if (convert_to_cuda) check(gm_db_upload(db_->handle()), "set_database");
+  }
+  void delete_database_from_device() { db_ = nullptr; }
+  void set_min_support(int minsup) { minsup_ = minsup; }
+  void set_graph_output(graph_output *gout) { out_ = gout; }
+  void set_exec_configurator(cuda_configurator *cc) { cc_ = cc; }
+  virtual void run() {
+    if (!cc_) throw std::runtime_error("exec configurator is not set");  // gspan_cuda.cpp:127
+    if (!db_ || !db_->handle()) throw std::runtime_error("database is not set");
+    gm_cfg cfg;
+    cc_->fill(cfg);
+    Sink sink{out_, nullptr};
+    const int rc = gm_mine(db_->handle(), minsup_, &cfg, out_ ? &gspan_cuda::report : nullptr, &sink, nullptr);
+    if (sink.error) std::rethrow_exception(sink.error);  // the sink's own exception
+    check(rc, "run");
+  }
+
+ private:
+  struct Sink {
+    graph_output *out;
+    std::exception_ptr error;
+  };
+  // no exception crosses the C ABI: a throwing sink stops mining and its
+  // exception is rethrown by run()
+  static int report(void *user, const int32_t *dfs5, int32_t n_edges, uint32_t support, uint64_t) {
+    Sink *sink = static_cast<Sink *>(user);
+    try {
+      DFSCode code;
+      code.resize((size_t)n_edges);
+      for (int32_t i = 0; i < n_edges; ++i) {
+        const int32_t *d = dfs5 + 5 * i;
+        code[(size_t)i] = DFS{d[0], d[1], d[2], d[3], d[4]};
+      }
+      sink->out->output_graph(code, support);
+      return 0;
+    } catch (...) {
+      sink->error = std::current_exception();
+      return 1;
+    }
+  }
+};
+
+// The factory the reference CLI links against (common_main.cpp:28).
+inline gspan_cuda *get_gspan_cuda_instance() { return new gspan_cuda(); }
+
+}  // namespace gspan_amd
+
+#endif /* GSPAN_AMD_HPP */

@@ -218,3 +218,20 @@ def test_gpu_bad_block_threads_rejected(golden):
     with pytest.raises(gm.GMError) as e:
         gm.mine(db, 5, block_threads=96)
     assert e.value.code == gm.GM_E_ARG
+
+
+def test_gpu_callback_exception_propagates(golden):
+    """An exception raised by a pattern sink stops mining and reaches the
+    caller (ctypes would otherwise print it and carry on)."""
+    case = golden["chem_68"]
+    db = case_db(case)
+    seen = []
+
+    def sink(code, sup, nproj):
+        seen.append(code)
+        if len(seen) == 3:
+            raise KeyError("sink failed")
+
+    with pytest.raises(KeyError):
+        gm.mine(db, case["minsup"], on_pattern=sink)
+    assert len(seen) == 3

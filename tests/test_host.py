@@ -255,3 +255,14 @@ def test_serving_request_mix_is_seeded_and_connected():
                 seen |= {f, t}
         else:
             assert r.minsup >= 1 and len(r.graphs) == len(set(r.graphs))
+
+
+def test_callback_guard_keeps_the_exception():
+    def boom(*args):
+        raise ValueError("x")
+
+    g = gm._Guard(boom)
+    assert g(1, 2) == 1  # stop mining
+    with pytest.raises(ValueError):
+        g.raise_pending()
+    assert gm._Guard(lambda *a: 0)(1) == 0
