@@ -54,11 +54,12 @@ typedef struct gm_cfg {
   int32_t label_style;      /* GM_LABELS_FULL (default) or GM_LABELS_WILDCARD */
   int32_t max_edges;        /* stop growing patterns beyond this many edges; 0 = unbounded */
   int32_t canon_threads;    /* host threads for the canonical check; 0 = auto */
-  int32_t block_threads;    /* threads per graph workgroup: 0 = by rows per graph, or 64/128/256 */
+  int32_t block_threads;    /* accepted for compatibility (0, 64, 128 or 256) and otherwise unused: the
+                               level kernels run one 64-lane wave per chunk of rows */
   int64_t mem_budget_bytes; /* device bytes the embedding store may use; 0 = auto */
   int32_t collect_proj;     /* 1 = also report |Proj(P)| per pattern (no extra pass) */
-  int32_t candidate_lists;  /* 0 = the support pass keeps each embedding's extension candidates for
-                               the write pass (default); 1 = the write pass enumerates again */
+  int32_t candidate_lists;  /* accepted for compatibility and unused: the count pass always keeps each
+                               embedding's candidates for the write pass */
   int32_t path_pruning;     /* 0 = drop extension candidates that have an infrequent 2-edge sub-path
                                (exact: such a child is infrequent; default); 1 = keep every candidate */
   int32_t speculate;        /* children written before their canonical check (the check overlaps the
@@ -87,9 +88,9 @@ typedef struct gm_stats {
   double seconds_ext_kernel; /* time inside the extension/support kernel  */
   uint64_t ext_kernel_launches;
   uint64_t peak_device_bytes;
-  uint64_t split_passes;   /* per-graph dedup passes redone in smaller parts */
-  uint64_t max_graph_rows; /* most embeddings one graph held at one level   */
-  uint64_t max_table_fill; /* most distinct extensions one graph produced in one pass */
+  uint64_t split_passes;   /* chunks recounted because their LDS count table filled */
+  uint64_t max_graph_rows; /* unused (0) */
+  uint64_t max_table_fill; /* most distinct keys one chunk's count table held */
   uint64_t pruned;         /* candidates dropped by the 2-edge path rule (counted in e_cand) */
   double seconds_stage_events; /* extension + support stages timed by HIP events (cfg.stage_events) */
   uint64_t reserved[1];

@@ -126,6 +126,20 @@ class graph_database {
     gm_db_destroy(db_);
     db_ = d;
   }
+  // from vertex and edge arrays (gm_db_create): the hand-over point for a
+  // database already held in memory, e.g. a types::graph_database_t
+  void create(const std::vector<int32_t> &graph_vbegin, const std::vector<int32_t> &vlabels,
+              const std::vector<int32_t> &graph_ebegin, const std::vector<int32_t> &efrom,
+              const std::vector<int32_t> &eto, const std::vector<int32_t> &elabel) {
+    if (graph_vbegin.empty() || graph_ebegin.size() != graph_vbegin.size())
+      throw std::runtime_error("create: graph_vbegin / graph_ebegin need one entry more than there are graphs");
+    gm_db *d = nullptr;
+    check(gm_db_create((int32_t)graph_vbegin.size() - 1, graph_vbegin.data(), vlabels.data(), graph_ebegin.data(),
+                       efrom.data(), eto.data(), elabel.data(), &d),
+          "create");
+    gm_db_destroy(db_);
+    db_ = d;
+  }
   size_t size() const {
     if (!db_) return 0;
     int64_t c[6];

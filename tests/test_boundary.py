@@ -66,6 +66,54 @@ int main(int argc, char **argv) {
                    check=True)
 
 
+def test_cpp_in_memory_database_hand_over(tmp_path):
+    """The in-memory hand-over a maintainer's binding uses: a database held
+    as the reference's types::graph_database_t (graph_types.hpp:21-64:
+    vertices with labels, each undirected edge in both endpoints' lists),
+    flattened into gm_db_create's arrays with one entry per edge, reads back
+    with the same counts and labels (host side only: no device copy is made
+    before mining)."""
+    src = tmp_path / "handover.cpp"
+    src.write_text('''#include <cstdio>
+#include "gspan_amd.hpp"
+struct Edge { int from, to, elabel; };
+struct Vertex { int label; std::vector<Edge> edge; };
+using Graph = std::vector<Vertex>;
+int main() {
+  std::vector<Graph> db(2);
+  db[0] = {{1, {{0, 1, 5}, {0, 2, 6}}}, {2, {{1, 0, 5}}}, {3, {{2, 0, 6}}}};
+  db[1] = {{4, {{0, 1, 7}}}, {4, {{1, 0, 7}}}};
+  std::vector<int32_t> vb{0}, vl, eb{0}, ef, et, el;
+  for (const Graph &g : db) {
+    for (const Vertex &v : g) {
+      vl.push_back(v.label);
+      for (const Edge &e : v.edge)
+        if (e.from < e.to) { ef.push_back(e.from); et.push_back(e.to); el.push_back(e.elabel); }
+    }
+    vb.push_back((int32_t)vl.size());
+    eb.push_back((int32_t)ef.size());
+  }
+  gspan_amd::graph_database gdb;
+  gdb.create(vb, vl, eb, ef, et, el);
+  int64_t c[6];
+  gspan_amd::check(gm_db_counts(gdb.handle(), c), "counts");
+  std::printf("%zu %lld %lld %lld\\n", gdb.size(), (long long)c[1], (long long)c[2], (long long)c[3]);
+  try {
+    gdb.create({0, 2}, {1, 1}, {0}, {}, {}, {});
+  } catch (const std::runtime_error &) {
+    std::printf("rejected\\n");
+  }
+  return 0;
+}
+''')
+    exe = tmp_path / "handover"
+    subprocess.run(["g++", "-std=c++17", f"-I{ROOT / 'include'}", str(src), f"-L{gbuild.LIB_DIR}", "-lgspan_amd",
+                    f"-Wl,-rpath,{gbuild.LIB_DIR}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True, timeout=60).stdout.split("\n")
+    assert tuple(map(int, out[0].split())) == (2, 5, 3, 3)  # graphs, vertices, edges, largest graph
+    assert out[1] == "rejected"
+
+
 def _cli(*args):
     exe = gbuild.CLI
     if not exe.exists():
