@@ -143,59 +143,30 @@ def _serving_section(n_requests: int, rate: float, out: dict) -> None:
                       **{k: v for k, v in res.items()}}
 
 
-def _copy_calibration(mib: int) -> None:
-    """One device-to-device hipMemcpy of mib MiB, which the HIP runtime runs
-    as a blit kernel (__amd_rocclr_copyBuffer) that reads and writes exactly
-    mib MiB: its FETCH_SIZE and WRITE_SIZE readings give the counters' scale
-    on this device."""
+def _calibration(kinds: str, mib: int) -> None:
+    """Kernels of known traffic (gpu_graph_mining_amd/csrc/calibrate.hip),
+    each touching every byte of an mib-MiB buffer once: stream16 (16-byte
+    loads and stores, reads and writes mib MiB), gather4 (scattered 4-byte
+    loads, reads mib MiB), atomic8 (one 64-bit atomic add per word, in a
+    scattered order).  Their FETCH_SIZE / WRITE_SIZE readings scale the
+    counters to bytes for the access widths the level kernels use."""
     import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    n = mib << 20
-    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
-    for p in (src, dst):
-        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
-            raise RuntimeError("hipMalloc failed")
-    try:
-        if hip.hipMemset(src, 1, ctypes.c_size_t(n)) != 0 or hip.hipDeviceSynchronize() != 0:
-            raise RuntimeError("hipMemset failed")
-        if hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3) != 0 or hip.hipDeviceSynchronize() != 0:  # 3: D2D
-            raise RuntimeError("hipMemcpy failed")
-    finally:
-        hip.hipFree(src)
-        hip.hipFree(dst)
-    print(f"# copy calibration: {mib} MiB read + {mib} MiB written by one device-to-device copy", file=sys.stderr,
-          flush=True)
+    from gpu_graph_mining_amd import build as gbuild
+    lib = ctypes.CDLL(str(gbuild.CALIB))
+    lib.gmc_run.argtypes = [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+    for kind in kinds.split(","):
+        ms = ctypes.c_double(0)
+        rc = lib.gmc_run(kind.encode(), ctypes.c_uint64(mib << 20), ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError(f"calibration kernel {kind} failed ({rc})")
+        print(f"# calibration {kind}: {mib} MiB in {ms.value:.3f} ms ({(mib << 20) / ms.value / 1e6:.1f} GB/s)",
+              file=sys.stderr, flush=True)
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--cpu-root-stride", type=int, default=1,
-                    help="CPU baseline sample: grow every Nth 1-edge root of the oracle's pattern tree, each subtree "
-                         "in full (1: the whole workload, about 40 s on one EPYC core for config 2)")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--jobs", type=int, default=256,
-                    help="batching: the first N jobs of the 256-job mix, batched vs one at a time (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=400,
-                    help="serving: requests against the resident 50,000-graph molecule database (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=200.0, help="serving: Poisson arrival rate, requests/s")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="after the run, make one device-to-device copy of MIB MiB (a blit kernel that reads and "
-                         "writes MIB MiB): the reference that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single GPU; "
-              f"multi-GPU sharding by graph id is a design note only)", file=sys.stderr)
-        return 2
-
+def _main_section(args) -> dict:
+    """The headline workload: timed mining runs, parity against the
+    reference listing and the oracle, the CPU baseline and the roofline."""
     import gpu_graph_mining_amd as gm
-    if gm.device_count() < 1:
-        print("no HIP device visible", file=sys.stderr)
-        return 3
-
     db, minsup, config = _workload(args.config)
     t0 = time.perf_counter()
     db.upload()
@@ -256,7 +227,8 @@ def main() -> int:
 
     roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
             "traffic": _traffic("stage", config["workload"]),
-            "kernel": "k_expand + k_support (extension + support stage of a level step)",
+            "kernel": "k_level (+ k_level_fix): extension enumeration and support counting of one level step",
+            "traffic_unit": "HBM bytes per launch (one level step's k_level + k_level_fix)",
             "launches_per_step": launches / args.steps,
             "avg_launch_ms": (ev_s / launches * 1e3) if launches else None}
     # parity: the reference's listing (sha256 + count), the oracle's counts
@@ -298,12 +270,53 @@ def main() -> int:
     if cpu is not None:
         line["cpu_baseline"] = cpu
     line["roofline"] = roof
-    if args.jobs:
+    return line
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--cpu-root-stride", type=int, default=1,
+                    help="CPU baseline sample: grow every Nth 1-edge root of the oracle's pattern tree, each subtree "
+                         "in full (1: the whole workload, about 40 s on one EPYC core for config 2)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--jobs", type=int, default=256,
+                    help="batching: the first N jobs of the 256-job mix, batched vs one at a time (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=400,
+                    help="serving: requests against the resident 50,000-graph molecule database (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="serving: Poisson arrival rate, requests/s")
+    ap.add_argument("--calibrate", default="", metavar="KINDS",
+                    help="after the run, the counter-calibration kernels (comma-separated: stream16, gather4, "
+                         "atomic8) over --calibrate-mib MiB each, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--calibrate-mib", type=int, default=256)
+    ap.add_argument("--sections", default="main,batch,serving",
+                    help="comma-separated parts to run: main (the headline workload), batch, serving")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single GPU; "
+              f"multi-GPU sharding by graph id is a design note only)", file=sys.stderr)
+        return 2
+
+    import gpu_graph_mining_amd as gm
+    if gm.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 3
+
+    sections = set(args.sections.split(","))
+    if "main" in sections:
+        line = _main_section(args)
+    else:
+        line = {"metric": "extended embeddings/s", "value": None, "unit": "embeddings/s", "n_gpus": 1,
+                "sections": sorted(sections)}
+    if args.jobs and "batch" in sections:
         _batch_section(args.jobs, line)
-    if args.serve_requests:
+    if args.serve_requests and "serving" in sections:
         _serving_section(args.serve_requests, args.serve_rate, line)
-    if args.calibrate_copy:
-        _copy_calibration(args.calibrate_copy)
+    if args.calibrate:
+        _calibration(args.calibrate, args.calibrate_mib)
     print(json.dumps(line), flush=True)
     return 0
 

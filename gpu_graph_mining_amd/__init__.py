@@ -46,7 +46,7 @@ EXPORTED_SYMBOLS = (
     "gm_db_write_txt", "gm_db_counts", "gm_db_export", "gm_db_upload", "gm_db_destroy", "gm_mine",
     "gm_mine_batch", "gm_dfs_code_is_min", "gm_gen_random", "gm_gen_planted", "gm_gen_molecule", "gm_db_subset",
     "gm_batch_create", "gm_batch_mine", "gm_batch_destroy", "gm_support_queries", "gm_support_query",
-    "gm_db_write", "gm_db_load_part", "gm_mine_noalloc",
+    "gm_db_write", "gm_db_load_part", "gm_mine_noalloc", "gm_dfs_codes_are_min",
 )
 
 
@@ -55,7 +55,7 @@ class gm_cfg(C.Structure):
         ("label_style", C.c_int32), ("max_edges", C.c_int32), ("canon_threads", C.c_int32),
         ("block_threads", C.c_int32), ("mem_budget_bytes", C.c_int64), ("collect_proj", C.c_int32),
         ("candidate_lists", C.c_int32), ("path_pruning", C.c_int32), ("speculate", C.c_int32), ("stage_events", C.c_int32),
-        ("chunk_rows", C.c_int32), ("table_bits", C.c_int32), ("reserved", C.c_int32 * 1),
+        ("chunk_rows", C.c_int32), ("table_bits", C.c_int32), ("canon_host", C.c_int32),
     ]
 
 
@@ -66,7 +66,7 @@ class gm_stats(C.Structure):
         ("seconds_device", C.c_double), ("seconds_canon", C.c_double), ("seconds_ext_kernel", C.c_double),
         ("ext_kernel_launches", C.c_uint64), ("peak_device_bytes", C.c_uint64), ("split_passes", C.c_uint64),
         ("max_graph_rows", C.c_uint64), ("max_table_fill", C.c_uint64),
-        ("pruned", C.c_uint64), ("seconds_stage_events", C.c_double), ("reserved", C.c_uint64 * 1),
+        ("pruned", C.c_uint64), ("seconds_stage_events", C.c_double), ("canon_host_keys", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -109,6 +109,7 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
         "gm_mine_noalloc": (C.c_int, [dbp, C.c_int32, P(gm_cfg), PATTERN_CB, C.c_void_p, P(gm_stats)]),
         "gm_mine_batch": (C.c_int, [C.c_int32, P(dbp), i32p, P(gm_cfg), JOB_PATTERN_CB, C.c_void_p, P(gm_stats)]),
         "gm_dfs_code_is_min": (C.c_int, [i32p, C.c_int32]),
+        "gm_dfs_codes_are_min": (C.c_int, [C.c_int32, C.c_int32, i32p, P(C.c_uint8)]),
         "gm_gen_random": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_int32,
                                     P(dbp)]),
         "gm_gen_planted": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_int32,
@@ -244,6 +245,27 @@ class DFSCode(tuple):
         if rc < 0:
             _check(rc)
         return rc == 1
+
+
+def codes_are_min(codes: Sequence[Sequence[Sequence[int]]]) -> List[bool]:
+    """Minimum-DFS-code check of many codes on the device (one wave per
+    code; gm_dfs_codes_are_min), grouped by length; the same answers as
+    DFSCode.is_min."""
+    codes = [tuple(tuple(int(x) for x in t) for t in c) for c in codes]
+    out = [False] * len(codes)
+    by_len: dict = {}
+    for i, c in enumerate(codes):
+        by_len.setdefault(len(c), []).append(i)
+    lib = load_library()
+    for n, idx in by_len.items():
+        if n == 0:
+            raise GMError(GM_E_ARG, "empty code")
+        flat = _i32(np.array([codes[i] for i in idx], dtype=np.int32).reshape(-1))
+        res = np.zeros(len(idx), dtype=np.uint8)
+        _check(lib.gm_dfs_codes_are_min(len(idx), n, _ptr(flat), res.ctypes.data_as(C.POINTER(C.c_uint8))))
+        for k, i in enumerate(idx):
+            out[i] = bool(res[k])
+    return out
 
 
 # ------------------------------------------------------------------ GraphDB
@@ -459,7 +481,7 @@ class graph_collecting_output(graph_output):
 # ------------------------------------------------------------------- mining
 def _make_cfg(label_style=GM_LABELS_FULL, max_edges=0, canon_threads=0, collect_proj=False, candidate_lists=True,
               block_threads=0, path_pruning=True, speculate=0, stage_events=False, chunk_rows=0,
-              table_bits=0) -> gm_cfg:
+              table_bits=0, canon_host=0) -> gm_cfg:
     cfg = gm_cfg()
     load_library().gm_cfg_init(C.byref(cfg))
     cfg.label_style = label_style
@@ -473,6 +495,7 @@ def _make_cfg(label_style=GM_LABELS_FULL, max_edges=0, canon_threads=0, collect_
     cfg.stage_events = 1 if stage_events else 0
     cfg.chunk_rows = chunk_rows
     cfg.table_bits = table_bits
+    cfg.canon_host = int(canon_host)
     return cfg
 
 
@@ -480,7 +503,8 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
          collect_proj: bool = False, canon_threads: int = 0,
          on_pattern: Optional[Callable[[DFSCode, int, int], Optional[bool]]] = None, count_only: bool = False,
          candidate_lists: bool = True, block_threads: int = 0, path_pruning: bool = True,
-         speculate: int = 0, stage_events: bool = False, chunk_rows: int = 0, table_bits: int = 0):
+         speculate: int = 0, stage_events: bool = False, chunk_rows: int = 0, table_bits: int = 0,
+         canon_host: int = 0):
     """Mine db on the GPU.  Returns (patterns, stats) where patterns is a
     list of (DFSCode, support, |Proj|) in emission order (empty when
     on_pattern is given; it then receives each pattern, and returning False
@@ -491,7 +515,10 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
     path_pruning=False keeps the candidates whose new edge forms an
     infrequent 2-edge path; speculate (0 adaptive, 1 never, 2 always)
     controls whether a level's children are written before their canonical
-    check (the results are the same either way); stage_events=True times
+    check, which runs on the host while the device writes them or, on the
+    levels written after the check, on the device (canon_host=0); 1 keeps it
+    on the host, 2 on the device; the results are the same either way;
+    stage_events=True times
     every level's extension + support stage with HIP events on the mining
     stream (stats["seconds_stage_events"])."""
     lib = load_library()
@@ -507,7 +534,7 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
     guard = _Guard(cb)
     ccb = PATTERN_CB() if count_only else PATTERN_CB(guard)  # PATTERN_CB() is a NULL pointer
     cfg = _make_cfg(label_style, max_edges, canon_threads, collect_proj, candidate_lists, block_threads,
-                    path_pruning, speculate, stage_events, chunk_rows, table_bits)
+                    path_pruning, speculate, stage_events, chunk_rows, table_bits, canon_host)
     st = gm_stats()
     rc = lib.gm_mine(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st))
     guard.raise_pending()
@@ -594,7 +621,8 @@ class Batch:
         self.n_jobs = len(dbs)
 
     def mine(self, minsups: Sequence[int], *, label_style: int = GM_LABELS_FULL, collect_proj: bool = False,
-             canon_threads: int = 0, count_only: bool = False, path_pruning: bool = True, speculate: int = 0):
+             canon_threads: int = 0, count_only: bool = False, path_pruning: bool = True, speculate: int = 0,
+             canon_host: int = 0):
         lib = load_library()
         res: List[list] = [[] for _ in range(self.n_jobs)]
 
@@ -606,7 +634,8 @@ class Batch:
         guard = _Guard(cb)
         ccb = JOB_PATTERN_CB() if count_only else JOB_PATTERN_CB(guard)
         ms = _i32(minsups)
-        cfg = _make_cfg(label_style, 0, canon_threads, collect_proj, path_pruning=path_pruning, speculate=speculate)
+        cfg = _make_cfg(label_style, 0, canon_threads, collect_proj, path_pruning=path_pruning, speculate=speculate,
+                        canon_host=canon_host)
         st = gm_stats()
         rc = lib.gm_batch_mine(self._h, _ptr(ms), C.byref(cfg), ccb, None, C.byref(st))
         guard.raise_pending()
@@ -677,5 +706,5 @@ class gspan_cuda:
 __all__ = [
     "GraphDB", "Batch", "DFS", "DFSCode", "graph_output", "graph_counting_output", "graph_file_output",
     "graph_collecting_output", "gspan_cuda", "mine", "mine_batch", "load_library", "device_count", "version",
-    "GMError", "GM_LABELS_FULL", "GM_LABELS_WILDCARD", "EXPORTED_SYMBOLS",
+    "GMError", "codes_are_min", "GM_LABELS_FULL", "GM_LABELS_WILDCARD", "EXPORTED_SYMBOLS",
 ]

@@ -26,7 +26,7 @@ ARCH = os.environ.get("GM_OFFLOAD_ARCH", "gfx950")
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-parameter", f"-I{PKG.parent / 'include'}"]
 
-DEVICE_SOURCES = ["kernels.hip"]
+DEVICE_SOURCES = ["kernels.hip", "canon.hip"]
 HOST_SOURCES = ["capi.cpp", "miner.cpp", "canonical.cpp", "host_db.cpp", "generators.cpp"]
 
 
@@ -94,7 +94,21 @@ def build(verbose: bool = False, out: Path = LIB, build_dir: Path = BUILD, defin
     os.replace(tmp, out)
     if out == LIB:
         build_cli()
+        build_calibration()
     return out
+
+
+CALIB = LIB_DIR / "libgm_calib.so"
+
+
+def build_calibration() -> Path:
+    """The benchmark's counter-calibration kernels (csrc/calibrate.hip) in a
+    library of their own: measurement support, not part of the miner."""
+    tmp = CALIB.with_suffix(".so.tmp")
+    _run([_hipcc(), f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, "-shared", str(CSRC / "calibrate.hip"), "-o",
+          str(tmp)])
+    os.replace(tmp, CALIB)
+    return CALIB
 
 
 CLI_SRC = PKG / "cli" / "gspan_amd_main.cpp"

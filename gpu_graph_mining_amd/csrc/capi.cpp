@@ -115,6 +115,8 @@ gm::MineOptions options(const gm_cfg *cfg) {
     // split passes)
     o.chunk_rows = cfg->chunk_rows;
     o.table_bits = cfg->table_bits;
+    if (cfg->canon_host < 0 || cfg->canon_host > 2) throw gm::Error(GM_E_ARG, "canon_host must be 0, 1 or 2");
+    o.canon_mode = cfg->canon_host;
     if (cfg->block_threads != 0 && cfg->block_threads != 64 && cfg->block_threads != 128 &&
         cfg->block_threads != 256)
       throw gm::Error(GM_E_ARG, "block_threads must be 0, 64, 128 or 256");
@@ -445,27 +447,82 @@ int gm_mine_batch(int32_t n_jobs, gm_db *const *dbs, const int32_t *abs_minsup, 
   return rc;
 }
 
+namespace {
+
+// A code as the check takes it: every tuple carries the labels its vertices
+// get anywhere in the code (a wildcard code completed from its labelled
+// tuples).
+void complete_code(const int32_t *dfs5, int32_t n_edges, gm::DFS *c) {
+  std::vector<int32_t> vl(n_edges + 2, -1);
+  for (int32_t i = 0; i < n_edges; ++i) {
+    const int32_t *d = dfs5 + 5 * i;
+    c[i] = gm::DFS{d[0], d[1], d[2], d[3], d[4]};
+    int32_t m = std::max(d[0], d[1]);
+    if (d[0] < 0 || d[1] < 0 || m >= (int32_t)vl.size()) throw gm::Error(GM_E_ARG, "malformed code");
+    if (d[2] != -1) vl[d[0]] = d[2];
+    if (d[4] != -1) vl[d[1]] = d[4];
+  }
+  for (int32_t i = 0; i < n_edges; ++i) {
+    c[i].fromlabel = vl[c[i].from];
+    c[i].tolabel = vl[c[i].to];
+  }
+}
+
+}  // namespace
+
 int gm_dfs_code_is_min(const int32_t *dfs5, int32_t n_edges) {
   int result = 0;
   int rc = guard([&]() {
     if (!dfs5 || n_edges < 1) throw gm::Error(GM_E_ARG, "empty code");
     std::vector<gm::DFS> c(n_edges);
-    std::vector<int32_t> vl(n_edges + 2, -1);
-    for (int32_t i = 0; i < n_edges; ++i) {
-      const int32_t *d = dfs5 + 5 * i;
-      c[i] = gm::DFS{d[0], d[1], d[2], d[3], d[4]};
-      int32_t m = std::max(d[0], d[1]);
-      if (d[0] < 0 || d[1] < 0 || m >= (int32_t)vl.size()) throw gm::Error(GM_E_ARG, "malformed code");
-      if (d[2] != -1) vl[d[0]] = d[2];
-      if (d[4] != -1) vl[d[1]] = d[4];
-    }
-    for (auto &d : c) {  // complete a wildcard code from the labelled tuples
-      d.fromlabel = vl[d.from];
-      d.tolabel = vl[d.to];
-    }
+    complete_code(dfs5, n_edges, c.data());
     result = gm::dfs_code_is_min(c.data(), n_edges) ? 1 : 0;
   });
   return rc == GM_OK ? result : rc;
+}
+
+int gm_dfs_codes_are_min(int32_t n_codes, int32_t n_edges, const int32_t *dfs5, uint8_t *out) {
+  return guard([&]() {
+    if (n_codes < 0 || n_edges < 1 || (n_codes > 0 && (!dfs5 || !out))) throw gm::Error(GM_E_ARG, "bad arguments");
+    if (n_codes == 0) return;
+    static_assert(sizeof(gm::DFS) == 5 * sizeof(int32_t), "DFS layout");
+    const size_t tuples = (size_t)n_codes * n_edges;
+    std::vector<gm::DFS> codes(tuples);
+    for (int32_t q = 0; q < n_codes; ++q) complete_code(dfs5 + (size_t)5 * n_edges * q, n_edges, &codes[(size_t)q * n_edges]);
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count < 1) throw gm::Error(GM_E_DEVICE, "no HIP device");
+    hipStream_t s = nullptr;
+    void *dcodes = nullptr, *dout = nullptr;
+    auto release = [&]() {
+      if (dcodes) (void)hipFree(dcodes);
+      if (dout) (void)hipFree(dout);
+      if (s) (void)hipStreamDestroy(s);
+    };
+    try {
+      gm::hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+      gm::hip_check(hipMalloc(&dcodes, sizeof(gm::DFS) * tuples), "hipMalloc");
+      gm::hip_check(hipMalloc(&dout, (size_t)n_codes), "hipMalloc");
+      gm::hip_check(hipMemcpyAsync(dcodes, codes.data(), sizeof(gm::DFS) * tuples, hipMemcpyHostToDevice, s), "upload codes");
+      gm::CanonArgs c;
+      std::memset(&c, 0, sizeof(c));
+      c.codes = static_cast<const int32_t *>(dcodes);
+      c.n = n_codes;
+      c.len = n_edges;
+      c.out = static_cast<uint8_t *>(dout);
+      gm::hip_check(gm::launch_canon_list(c, s), "k_canon_list");
+      gm::hip_check(hipMemcpyAsync(out, dout, (size_t)n_codes, hipMemcpyDeviceToHost, s), "download answers");
+      gm::hip_check(hipStreamSynchronize(s), "k_canon_list");
+    } catch (...) {
+      release();
+      throw;
+    }
+    release();
+    // codes the device left undecided are checked here
+    for (int32_t q = 0; q < n_codes; ++q) {
+      if (out[q] == gm::kCanonBad) throw gm::Error(GM_E_ARG, "malformed DFS code");
+      if (out[q] == gm::kCanonHost) out[q] = gm::dfs_code_is_min(&codes[(size_t)q * n_edges], n_edges) ? 1 : 0;
+    }
+  });
 }
 
 int gm_gen_random(uint64_t seed, int32_t n_graphs, int32_t vmin, int32_t vmax, double edge_factor, int32_t n_vlabels,

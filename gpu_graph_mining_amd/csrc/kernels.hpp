@@ -142,6 +142,36 @@ hipError_t launch_fill8(void *dst, size_t bytes, hipStream_t s);
 // its address on the current device (looked up once per device).
 constexpr size_t kStaticScratchBytes = 64ull << 20;
 hipError_t static_scratch(void **ptr, size_t *bytes);
+// Minimum DFS code check on the device (canon.hip): one wave per code, the
+// pattern graph and the embedding lists in LDS.  Codes beyond these sizes are
+// left to the host check (kCanonHost).
+constexpr int kCanonV = 40;     // pattern vertices (+1 for the grown code's next vertex)
+constexpr int kCanonE = 64;     // pattern edges (64-bit used-edge masks)
+constexpr int kCanonCap = 64;   // embeddings of the grown code held at once
+constexpr int kCanonNotMin = 0, kCanonMin = 1, kCanonHost = 2, kCanonBad = 3;
+struct CanonArgs {
+  // a later level's frequent records (key, support, embeddings, item sum):
+  // the answer + 1 for record i goes to bits 32-39 of its support word
+  unsigned long long *freq;
+  const uint32_t *freq_count;
+  uint32_t freq_cap;
+  // the level's patterns: nv from meta, len tuples each packed from | to << 8
+  // | elabel << 16 | tolabel << 24 (dense labels), kMaxPatV vertex labels each
+  const PatMeta *meta;
+  const uint32_t *pcode;
+  const uint8_t *pvlab;
+  int32_t len;
+  int64_t K;
+  int32_t S, NEL, NVL;
+  // explicit codes: n codes of len (from, to, fromlabel, elabel, tolabel)
+  // tuples; out[i] gets the answer (kCanonBad: a malformed tuple)
+  const int32_t *codes;
+  int64_t n;
+  uint8_t *out;
+};
+// grid: waves over the frequent records (any count covers them)
+hipError_t launch_canon_level(const CanonArgs &c, int grid, hipStream_t s);
+hipError_t launch_canon_list(const CanonArgs &c, hipStream_t s);
 // Zero the per-level counters (frequent-key count, failure list, spread
 // statistics but the written-row slots).
 hipError_t launch_clear_counters(const StepArgs &a, hipStream_t s);

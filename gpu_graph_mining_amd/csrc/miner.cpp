@@ -642,6 +642,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   bool t2_ready = false;
   size_t mask_off = 0;  // this level's masks in its pattern upload (stride 0: none)
   int mask_stride = 0;
+  // the level's packed codes and vertex labels in its pattern upload, read
+  // by the device's minimum-code check of the frequent keys
+  size_t code_off = 0, vlab_off = 0;
+  bool codes_ready = false;
+  const bool want_codes = opt.canon_mode != 1 && !opt.queries;
   double dead_share = 0;  // the last level's rows of non-minimal children / its child rows
   // GM_LEVEL_TRACE=1: one line per level step on stderr (sizes, timings)
   const char *trace_env = getenv("GM_LEVEL_TRACE");
@@ -698,6 +703,14 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     grow(w.rccnt, sizeof(uint16_t) * std::max<int64_t>(rows_now, 1), s);
     grow(w.cand, sizeof(uint2) * std::max<uint64_t>(list_cap, 1), s);
     uint32_t fcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(nkeys, 1), std::max<int64_t>(1 << 16, P * 64));
+    // the children would pass max_edges: none is kept
+    const bool past_max = opt.max_edges > 0 && (root ? 1 : L.len + 1) > opt.max_edges;
+    // the minimum-code check of this level's frequent keys runs on the
+    // device when asked to, or when the host would check them before the
+    // write (no speculation: the check is then on the critical path)
+    const bool spec_wanted = opt.speculate == 2 || (opt.speculate == 0 && dead_share <= 0.3);
+    const bool canon_dev = !root && want_codes && codes_ready && !past_max &&
+                           (opt.canon_mode == 2 || (opt.canon_mode == 0 && !spec_wanted));
     Counters *dctr = nullptr;
     Counters cnt;
     const char *hbuf = nullptr;
@@ -779,6 +792,24 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       hip_check(launch_level_fix(a, root, s), "k_level_fix");
       if (time_stage) hip_check(hipEventRecord(w.ev[1], s), "hipEventRecord");
       hip_check(launch_reduce(a, root, s), "k_reduce");
+      if (canon_dev) {
+        CanonArgs c;
+        std::memset(&c, 0, sizeof(c));
+        c.freq = a.freq;
+        c.freq_count = a.freq_count;
+        c.freq_cap = fcap;
+        c.meta = a.meta;
+        c.pcode = reinterpret_cast<const uint32_t *>(w.upl[cur].as<char>() + code_off);
+        c.pvlab = reinterpret_cast<const uint8_t *>(w.upl[cur].as<char>() + vlab_off);
+        c.len = L.len;
+        c.K = K;
+        c.S = S;
+        c.NEL = NEL;
+        c.NVL = NVL;
+        // one wave per record; the grid loops over however many there are
+        const int64_t grid = std::min<int64_t>(fcap, std::max<int64_t>(256, std::min<int64_t>(8192, 2 * (int64_t)last_fcount)));
+        hip_check(launch_canon_level(c, (int)grid, s), "k_canon_level");
+      }
       // counters and (most often all of) the frequent keys in one copy
       guess = std::min<uint32_t>(fcap, std::max<uint32_t>(1024, 2 * last_fcount));
       const size_t head = sizeof(Counters) + sizeof(Freq) * guess;
@@ -896,11 +927,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // their metadata and the next level skips their rows.  Not at the root
     // (every 1-edge code is minimal), in query mode (the trie selects), past
     // max_edges, or when a child would have more than kMaxPatV vertices.
-    const bool past_max = opt.max_edges > 0 && new_len > opt.max_edges;
     // adaptive: speculate while the last level's non-minimal children held
     // at most 30 % of its rows (each costs a written, unused row; molecule-
     // like databases, rich in symmetric patterns, hold about half)
-    const bool want_spec = opt.speculate == 2 || (opt.speculate == 0 && dead_share <= 0.3);
+    const bool want_spec = spec_wanted;
     // short lists are decoded while the device writes (a serial pass on the
     // critical path otherwise); long ones in parallel before the write
     const bool early_decode = F >= 4096 || max_nv_bound > kMaxPatV;
@@ -908,9 +938,23 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     int max_nv_exact = 0;
     if (early_decode)
       for (int64_t i = 0; i < F; ++i) max_nv_exact = std::max(max_nv_exact, (int)knv[i]);
-    const bool spec = want_spec && !root && !opt.queries && !past_max &&
+    const bool spec = want_spec && !canon_dev && !root && !opt.queries && !past_max &&
                       (early_decode ? max_nv_exact : max_nv_bound) <= kMaxPatV;
     const bool late_decode = spec && !early_decode;
+    // the device's answer for frequent key i (written by k_canon_level as
+    // answer + 1 in bits 32-39 of the support word); undecided keys are
+    // checked here
+    std::atomic<uint64_t> host_keys(0);
+    auto dev_min = [&](int64_t i) {
+      const unsigned code = (unsigned)((fr[i].sup >> 32) & 0xFF);
+      if (code == 0 || code > kCanonHost + 1)
+        throw Error(GM_E_DEVICE, "internal: frequent key without a minimum-code answer");
+      if (code - 1 == kCanonHost) {
+        host_keys.fetch_add(1, std::memory_order_relaxed);
+        return is_min_child(i);
+      }
+      return code - 1 == kCanonMin;
+    };
     if (!spec) {
       if (!early_decode) Pool::get().run(F, threads, decode, 4096);
       std::atomic<bool> too_big(false);
@@ -925,7 +969,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
           keep = node >= 0 && query_reached(node, sup) && cnv <= kMaxPatV;
         } else if (root) {
           keep = true;
-        } else if (!past_max && is_min_child(i)) {
+        } else if (!past_max && (canon_dev ? dev_min(i) : is_min_child(i))) {
           if (cnv > kMaxPatV)
             too_big = true;
           else
@@ -940,6 +984,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     }
     canon_sec += tcan.sec();
     double us_canon = tcan.sec() * 1e6;
+    st.canon_host_keys += host_keys.load();
     st.n_candidates += root ? 0 : F;
 
     // children in key order; child c's rows are [row0[c], row0[c + 1])
@@ -973,7 +1018,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const bool masks = prune && new_len >= 2 && C > 0;
     const int mstride = masks ? max_nv : 0;
     const size_t masks_off = lb_off + sizeof(unsigned long long) * C;
-    const size_t up_bytes = masks_off + sizeof(uint64_t) * 2 * mstride * C;
+    // then per child its packed code (from | to << 8 | elabel << 16 |
+    // tolabel << 24 per tuple) and its vertex labels, for the next level's
+    // minimum-code check on the device
+    const size_t codes_off = masks_off + sizeof(uint64_t) * 2 * mstride * C;
+    const size_t cvl_off = (codes_off + (want_codes ? sizeof(uint32_t) * new_len * C : 0) + 7) & ~size_t(7);
+    const size_t up_bytes = cvl_off + (want_codes ? (size_t)kMaxPatV * C : 0);
     char *hu = F > 0 ? static_cast<char *>(w.up.ensure(up_bytes)) : nullptr;
     if (F > 0) {
       unsigned long long *hfk = reinterpret_cast<unsigned long long *>(hu + fk_off);
@@ -1080,6 +1130,13 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         nproj[c] = fr[i].proj;
         hm[c] = make_meta(cc, new_len, cv, cnv, N.job[c]);
         if (!kalive[i]) hm[c].flags |= kPatDead;
+        if (want_codes) {
+          uint32_t *pc = reinterpret_cast<uint32_t *>(hu + codes_off) + (size_t)c * new_len;
+          for (int k = 0; k < new_len; ++k)
+            pc[k] = (uint32_t)cc[k].from | (uint32_t)cc[k].to << 8 | (uint32_t)cc[k].elabel << 16 |
+                    (uint32_t)cc[k].tolabel << 24;
+          std::memcpy(hu + cvl_off + (size_t)c * kMaxPatV, cv, kMaxPatV);
+        }
       };
       auto mask_child = [&](int64_t c) {
         if (!N.alive[c]) return;  // dead patterns' rows are never extended
@@ -1186,6 +1243,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       peak = std::max(peak, live);
       cur = nxt;
       mask_off = masks_off;
+      code_off = codes_off;
+      vlab_off = cvl_off;
+      codes_ready = want_codes;
       mask_stride = masks && t2_ready ? mstride : 0;
     }
     if (root)
@@ -1212,10 +1272,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       fprintf(stderr,
               "[gm level %d] patterns %lld rows %lld K %lld chunks %lld table_bits %d stage_us %.1f "
               "candidates %llu pruned %llu frequent %u children %lld child_rows %lld minimal %lld spec %d "
-              "failed %llu max_fill %llu list_cap %llu host_us canon %.1f host %.1f overlap %.1f\n",
+              "failed %llu max_fill %llu list_cap %llu canon_dev %d host_us canon %.1f host %.1f overlap %.1f\n",
               level, (long long)P, (long long)rows_now, (long long)K, (long long)total_chunks, table_bits, ms * 1e3,
               ecand, pruned, cnt.fcount, (long long)C, (long long)child_rows, (long long)alive, spec ? 1 : 0, failed,
-              max_fill, (unsigned long long)list_cap, us_canon, us_host, us_overlap);
+              max_fill, (unsigned long long)list_cap, canon_dev ? 1 : 0, us_canon, us_host, us_overlap);
     st.n_levels++;
     if (stopped) break;
     std::swap(L, N);

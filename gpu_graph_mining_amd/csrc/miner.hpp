@@ -192,6 +192,11 @@ struct MineOptions {
   bool stage_events = false;    // time the extension + support stages with HIP events
   int32_t chunk_rows = 0;       // launch shape (gm_cfg::chunk_rows); 0 = default
   int32_t table_bits = 0;       // launch shape (gm_cfg::table_bits); 0 = default
+  // minimum-code check of a level's frequent keys: 0 on the device for the
+  // levels whose children the host would check before writing them (no
+  // speculation), else on the host pool overlapping the write; 1 always on
+  // the host; 2 always on the device (k_canon_level)
+  int32_t canon_mode = 0;
   // Query mode: instead of mining, follow the codes of `queries` (every
   // prefix of a minimum code is one) and store each query's support in
   // query_support; the report callback is not called.

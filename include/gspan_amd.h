@@ -71,7 +71,11 @@ typedef struct gm_cfg {
                                two; smaller levels use shorter chunks); 0 = default (256) */
   int32_t table_bits;       /* launch shape: log2 entries of a wave's LDS count table (7..11); a chunk
                                that fills it is recounted; 0 = default (8) */
-  int32_t reserved[1];
+  int32_t canon_host;       /* where the minimum-code check of a level's frequent keys runs: 0 = on
+                               the device before their download for the levels whose children are
+                               not written speculatively, else on the host thread pool while the
+                               device writes them (default); 1 = always on the host; 2 = always on
+                               the device */
 } gm_cfg;
 
 /* Per-run statistics (filled when a non-NULL pointer is passed). */
@@ -93,7 +97,7 @@ typedef struct gm_stats {
   uint64_t max_table_fill; /* most distinct keys one chunk's count table held */
   uint64_t pruned;         /* candidates dropped by the 2-edge path rule (counted in e_cand) */
   double seconds_stage_events; /* extension + support stages timed by HIP events (cfg.stage_events) */
-  uint64_t reserved[1];
+  uint64_t canon_host_keys;    /* frequent keys the device check left to the host (beyond its capacity) */
 } gm_stats;
 
 /* Pattern callback: n_edges rows of (from, to, fromlabel, elabel, tolabel),
@@ -241,6 +245,16 @@ int gm_support_query(gm_db *db, const int32_t *dfs5, int32_t n_edges, uint32_t *
  * negative error.  Replaces types::DFSCode::dfs_code_is_min
  * (src/globals/dfs_ismin.cpp:23-160). */
 int gm_dfs_code_is_min(const int32_t *dfs5, int32_t n_edges);
+
+/* The same check for n_codes codes of n_edges tuples each (dfs5 holds
+ * n_codes * n_edges rows), on the device: one wave per code.  out[i] = 1
+ * minimal, 0 not; each answer equals gm_dfs_code_is_min's.  Codes beyond
+ * the device check's capacity (more than 64 edges or 39 vertices, or more
+ * than 64 embeddings of a prefix in the code's own graph) are checked on
+ * the host.  The batch form of the check gspan_cuda runs per candidate
+ * (src/gspan_cuda/gspan_cuda_no_sort.cpp:408-414 calling
+ * dfs_ismin.cpp:23-160); it allocates device memory for the codes. */
+int gm_dfs_codes_are_min(int32_t n_codes, int32_t n_edges, const int32_t *dfs5, uint8_t *out);
 
 /* ---- synthetic databases (seeded, deterministic) ---------------------- */
 /* Random connected graphs: n_graphs, vertex count uniform in [vmin,vmax],
