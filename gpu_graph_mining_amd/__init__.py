@@ -481,7 +481,7 @@ class graph_collecting_output(graph_output):
 # ------------------------------------------------------------------- mining
 def _make_cfg(label_style=GM_LABELS_FULL, max_edges=0, canon_threads=0, collect_proj=False, candidate_lists=True,
               block_threads=0, path_pruning=True, speculate=0, stage_events=False, chunk_rows=0,
-              table_bits=0, canon_host=0) -> gm_cfg:
+              table_bits=0, canon_host=0, mem_budget=0) -> gm_cfg:
     cfg = gm_cfg()
     load_library().gm_cfg_init(C.byref(cfg))
     cfg.label_style = label_style
@@ -496,6 +496,7 @@ def _make_cfg(label_style=GM_LABELS_FULL, max_edges=0, canon_threads=0, collect_
     cfg.chunk_rows = chunk_rows
     cfg.table_bits = table_bits
     cfg.canon_host = int(canon_host)
+    cfg.mem_budget_bytes = int(mem_budget)
     return cfg
 
 
@@ -504,7 +505,7 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
          on_pattern: Optional[Callable[[DFSCode, int, int], Optional[bool]]] = None, count_only: bool = False,
          candidate_lists: bool = True, block_threads: int = 0, path_pruning: bool = True,
          speculate: int = 0, stage_events: bool = False, chunk_rows: int = 0, table_bits: int = 0,
-         canon_host: int = 0):
+         canon_host: int = 0, mem_budget: int = 0):
     """Mine db on the GPU.  Returns (patterns, stats) where patterns is a
     list of (DFSCode, support, |Proj|) in emission order (empty when
     on_pattern is given; it then receives each pattern, and returning False
@@ -518,6 +519,8 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
     check, which runs on the host while the device writes them or, on the
     levels written after the check, on the device (canon_host=0); 1 keeps it
     on the host, 2 on the device; the results are the same either way;
+    mem_budget bounds the device bytes of the embedding rows and candidate
+    lists held at once (levels are then mined in pieces; 0 = unlimited);
     stage_events=True times
     every level's extension + support stage with HIP events on the mining
     stream (stats["seconds_stage_events"])."""
@@ -534,7 +537,7 @@ def mine(db: GraphDB, minsup: int, *, label_style: int = GM_LABELS_FULL, max_edg
     guard = _Guard(cb)
     ccb = PATTERN_CB() if count_only else PATTERN_CB(guard)  # PATTERN_CB() is a NULL pointer
     cfg = _make_cfg(label_style, max_edges, canon_threads, collect_proj, candidate_lists, block_threads,
-                    path_pruning, speculate, stage_events, chunk_rows, table_bits, canon_host)
+                    path_pruning, speculate, stage_events, chunk_rows, table_bits, canon_host, mem_budget)
     st = gm_stats()
     rc = lib.gm_mine(db._h, int(minsup), C.byref(cfg), ccb, None, C.byref(st))
     guard.raise_pending()
@@ -622,7 +625,7 @@ class Batch:
 
     def mine(self, minsups: Sequence[int], *, label_style: int = GM_LABELS_FULL, collect_proj: bool = False,
              canon_threads: int = 0, count_only: bool = False, path_pruning: bool = True, speculate: int = 0,
-             canon_host: int = 0):
+             canon_host: int = 0, mem_budget: int = 0):
         lib = load_library()
         res: List[list] = [[] for _ in range(self.n_jobs)]
 
@@ -635,7 +638,7 @@ class Batch:
         ccb = JOB_PATTERN_CB() if count_only else JOB_PATTERN_CB(guard)
         ms = _i32(minsups)
         cfg = _make_cfg(label_style, 0, canon_threads, collect_proj, path_pruning=path_pruning, speculate=speculate,
-                        canon_host=canon_host)
+                        canon_host=canon_host, mem_budget=mem_budget)
         st = gm_stats()
         rc = lib.gm_batch_mine(self._h, _ptr(ms), C.byref(cfg), ccb, None, C.byref(st))
         guard.raise_pending()

@@ -113,6 +113,8 @@ gm::MineOptions options(const gm_cfg *cfg) {
     // (k_write takes a chunk's records in one table of 2^8 entries; larger
     // k_level tables let chunks leave more, which k_write then counts in
     // split passes)
+    if (cfg->mem_budget_bytes < 0) throw gm::Error(GM_E_ARG, "mem_budget_bytes must be >= 0");
+    o.mem_budget = cfg->mem_budget_bytes;
     o.chunk_rows = cfg->chunk_rows;
     o.table_bits = cfg->table_bits;
     if (cfg->canon_host < 0 || cfg->canon_host > 2) throw gm::Error(GM_E_ARG, "canon_host must be 0, 1 or 2");

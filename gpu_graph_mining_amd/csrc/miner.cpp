@@ -608,7 +608,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   hip_check(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute");
   if (wall_khz <= 0) wall_khz = 100000;
 
-  Level L, N;             // patterns of the current and of the next level
+  Level N;                // patterns of the next level
   Arr<Freq> frall, fr, frtmp;
   Arr<int64_t> kidx;      // per frequent key: child index, -1 if dropped
   Arr<DFS> ktup;          // per frequent key: the tuple it adds
@@ -617,12 +617,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   Arr<uint8_t> kalive;    // per frequent key: the child's code is minimal
   Arr<uint64_t> nproj;    // per child: embeddings reported
   bool stopped = false;
-  int level = 0;          // 0 = root stage
-  int S = 0;              // tuple stride of the current level
-  int64_t n_rows = 0;
-  int cur = 0;                   // which of the double buffers holds the current level
   unsigned long long written_expect = 0;  // child rows asked of every k_write so far
-  uint64_t list_cap = (uint64_t)std::max<int64_t>(db.n_adj, 1);  // this level's list entries (root: 2E)
   uint32_t last_fcount = 0;
   int table_bits = opt.table_bits > 0 ? opt.table_bits : 8;
   const int max_chunk_rows = opt.chunk_rows > 0 ? opt.chunk_rows : kChunkRows;
@@ -640,12 +635,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   const bool prune = opt.path_pruning && !opt.queries && AR <= 64;
   std::vector<uint64_t> t2;
   bool t2_ready = false;
-  size_t mask_off = 0;  // this level's masks in its pattern upload (stride 0: none)
-  int mask_stride = 0;
-  // the level's packed codes and vertex labels in its pattern upload, read
-  // by the device's minimum-code check of the frequent keys
-  size_t code_off = 0, vlab_off = 0;
-  bool codes_ready = false;
+  bool t2_partial = false;  // the 2-edge level was written in pieces: no path table
   const bool want_codes = opt.canon_mode != 1 && !opt.queries;
   double dead_share = 0;  // the last level's rows of non-minimal children / its child rows
   // GM_LEVEL_TRACE=1: one line per level step on stderr (sizes, timings)
@@ -664,9 +654,52 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   std::vector<DFS> orig;  // a reported code in the database's own labels
   bool counters_fresh = true;  // the counter block must be zeroed (first level, or it moved)
 
+  // The levels being mined, the current one last.  A level is normally
+  // expanded whole: its children become the next level and it is dropped.
+  // Under a memory budget, a level whose children would not fit is expanded
+  // in pieces by parent pattern (or by key within one pattern), and each
+  // piece's children are mined to the end before the next piece: the stack
+  // then holds one level per depth, each in its own row set.
+  struct Frame {
+    Level L;                 // patterns (the root's "patterns" are the jobs)
+    int level = 0;           // 0 = root stage
+    int buf = 0;             // row set of its rows and pattern upload
+    int S = 0;               // tuple stride
+    int64_t n_rows = 0;
+    uint64_t list_cap = 1;   // list entries of its rows
+    size_t mask_off = 0;     // path masks in its pattern upload (stride 0: none)
+    int mask_stride = 0;
+    size_t code_off = 0, vlab_off = 0;  // packed codes and vertex labels for the device check
+    bool codes_ready = false;
+    int64_t p_next = 0;      // first pattern whose children are not all written
+    uint64_t key_next = 0;   // first key of p_next not written (a pattern split over pieces)
+    int64_t p_counted = 0;   // patterns whose candidates are in e_cand
+    int64_t p_end_next = 0;  // > 0: the next piece ends there (its children were measured)
+  };
+  std::deque<Frame> frames(1);
+  frames.back().list_cap = (uint64_t)std::max<int64_t>(db.n_adj, 1);  // root lists: 2E
+  const int64_t budget = w.arena ? 0 : opt.mem_budget;
+  double piece_bytes_per_row = 0;  // child bytes per parent row of the last piece (estimate)
+  // device bytes a level's rows take (rows with their counts) and a list entry
+  auto rows_bytes = [](int64_t rows, int S_) { return (int64_t)rows * (22 + S_); };
+
   for (;;) {
+    if (frames.empty()) break;
+    Frame &Fr = frames.back();
+    const int level = Fr.level;
     const bool root = level == 0;
-    if (!root && L.n == 0) break;
+    Level &L = Fr.L;
+    if (Fr.p_next >= (root ? db.n_jobs : L.n)) {
+      frames.pop_back();
+      continue;
+    }
+    const int cur = Fr.buf;
+    const int S = Fr.S;
+    const int64_t n_rows = Fr.n_rows;
+    const uint64_t list_cap = Fr.list_cap;
+    const size_t mask_off = Fr.mask_off, code_off = Fr.code_off, vlab_off = Fr.vlab_off;
+    const int mask_stride = Fr.mask_stride;
+    const bool codes_ready = Fr.codes_ready;
     if (w.arena) {
       w.arena_level(cur);
       counters_fresh = true;  // the counter block is taken again
@@ -679,13 +712,42 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     auto prow = [&](int64_t p) { return root ? job_row0[p] : L.row0[p]; };
     // launches over pattern ranges with range * K < 2^32 (32-bit table keys)
     const int64_t prange = std::max<int64_t>(1, (int64_t)0xFFFFFFFEll / K);
-    // chunk length: kChunkRows, or shorter so that the level has about
+    // the piece: patterns [p_begin, p_end); under a budget, as many as the
+    // last piece's child bytes per parent row say will fit
+    const int64_t p_begin = Fr.p_next;
+    int64_t p_end = P;
+    const int64_t p_end_measured = Fr.p_end_next;
+    Fr.p_end_next = 0;
+    int64_t held = 0;  // rows of the levels on the stack, and this level's lists
+    for (const Frame &f : frames) held += rows_bytes(f.n_rows, f.S);
+    held += (int64_t)list_cap * 8;
+    const int64_t avail = budget - held;
+    if (Fr.key_next > 0) {
+      // a pattern split by key stays a piece of its own until it is done
+      // (its candidates are counted once, with its first piece)
+      p_end = p_begin + 1;
+    } else if (p_end_measured > 0) {
+      p_end = p_end_measured;
+    } else if (budget > 0 && piece_bytes_per_row > 0) {
+      const double rows_fit = std::max(0.0, (double)avail / piece_bytes_per_row);
+      int64_t lo = p_begin + 1, hi = P;  // the last p_end whose rows fit, at least p_begin + 1
+      while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if ((double)(prow(mid) - prow(p_begin)) <= rows_fit)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      p_end = lo;
+    }
+    const int64_t prow_lo = prow(p_begin), prow_hi = prow(p_end);
+    // chunk length: kChunkRows, or shorter so that the piece has about
     // kLevelWaves chunks (a power of two, at least one block)
     int chunk_rows = max_chunk_rows;
-    while (chunk_rows > kMinChunkRows && rows_now / chunk_rows < kLevelWaves) chunk_rows /= 2;
+    while (chunk_rows > kMinChunkRows && (prow_hi - prow_lo) / chunk_rows < kLevelWaves) chunk_rows /= 2;
     int64_t total_chunks = 0;
-    for (int64_t lo = 0; lo < P; lo += prange)
-      total_chunks += ceil_div(prow(std::min<int64_t>(P, lo + prange)) - prow(lo), chunk_rows);
+    for (int64_t lo = p_begin; lo < p_end; lo += prange)
+      total_chunks += ceil_div(prow(std::min<int64_t>(p_end, lo + prange)) - prow(lo), chunk_rows);
     // counter replicas: chunk i counts into replica i mod rep, so that the
     // keys every chunk hits (the root's) are not one hot address (atomics
     // on one address serialise at ~100 ns each); up to 64 MB of them
@@ -754,15 +816,15 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.freq = reinterpret_cast<unsigned long long *>(dctr + 1);
       a.freq_cap = fcap;
       a.freq_count = &dctr->fcount;
-      a.pid = root ? nullptr : w.pid[cur].as<int32_t>();
-      a.rgraph = root ? nullptr : w.rgraph[cur].as<int32_t>();
-      a.vt = root ? nullptr : w.vt[cur].as<uint8_t>();
-      a.rbound = root ? nullptr : w.rbound[cur].as<uint32_t>();
-      a.rlist = root ? nullptr : w.rlist[cur].as<unsigned long long>();
+      a.pid = root ? nullptr : w.row(cur).pid.as<int32_t>();
+      a.rgraph = root ? nullptr : w.row(cur).rgraph.as<int32_t>();
+      a.vt = root ? nullptr : w.row(cur).vt.as<uint8_t>();
+      a.rbound = root ? nullptr : w.row(cur).rbound.as<uint32_t>();
+      a.rlist = root ? nullptr : w.row(cur).rlist.as<unsigned long long>();
       a.S = S;
-      a.meta = root ? nullptr : reinterpret_cast<const PatMeta *>(w.upl[cur].as<char>());
+      a.meta = root ? nullptr : reinterpret_cast<const PatMeta *>(w.row(cur).upl.as<char>());
       a.mask_stride = root ? 0 : mask_stride;
-      a.pmask = a.mask_stride ? reinterpret_cast<const unsigned long long *>(w.upl[cur].as<char>() + mask_off)
+      a.pmask = a.mask_stride ? reinterpret_cast<const unsigned long long *>(w.row(cur).upl.as<char>() + mask_off)
                               : nullptr;
       hip_check(launch_clear_counters(a, s), "clear counters");
       const bool time_stage = opt.stage_events && !root && !w.arena;
@@ -772,8 +834,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         hip_check(hipEventRecord(w.ev[0], s), "hipEventRecord");
       }
       int64_t chunk0 = 0;
-      for (int64_t lo = 0; lo < P; lo += prange) {
-        const int64_t hi = std::min<int64_t>(P, lo + prange);
+      for (int64_t lo = p_begin; lo < p_end; lo += prange) {
+        const int64_t hi = std::min<int64_t>(p_end, lo + prange);
         a.p_lo = (int32_t)lo;
         a.p_hi = (int32_t)hi;
         a.row_lo = prow(lo);
@@ -783,10 +845,10 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         hip_check(launch_level(a, root, s), "k_level");
         chunk0 += a.n_chunks;
       }
-      // the recount and the write see the whole level (segment starts are
+      // the recount and the write see the whole piece (segment starts are
       // pattern changes at the launch boundaries anyway)
-      a.row_lo = 0;
-      a.row_hi = rows_now;
+      a.row_lo = prow_lo;
+      a.row_hi = prow_hi;
       a.chunk0 = 0;
       a.n_chunks = total_chunks;
       hip_check(launch_level_fix(a, root, s), "k_level_fix");
@@ -799,8 +861,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         c.freq_count = a.freq_count;
         c.freq_cap = fcap;
         c.meta = a.meta;
-        c.pcode = reinterpret_cast<const uint32_t *>(w.upl[cur].as<char>() + code_off);
-        c.pvlab = reinterpret_cast<const uint8_t *>(w.upl[cur].as<char>() + vlab_off);
+        c.pcode = reinterpret_cast<const uint32_t *>(w.row(cur).upl.as<char>() + code_off);
+        c.pvlab = reinterpret_cast<const uint8_t *>(w.row(cur).upl.as<char>() + vlab_off);
         c.len = L.len;
         c.K = K;
         c.S = S;
@@ -842,19 +904,18 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         ev_stage_ms += ev_ms;
       }
       if (cnt.fcount <= fcap) break;
-      // the list was too short: k_reduce cleared the counters, so the stage
-      // is run again with room for every frequent key
+      // the list was too short: the stage is run again with room for every
+      // frequent key, from cleared counters (k_reduce kept the frequent
+      // keys' replicas, listed or not)
       if (attempt > 0) throw Error(GM_E_DEVICE, "internal: frequent-key count changed between runs");
       fcap = cnt.fcount;
+      hip_check(launch_fill8(w.acc.p, w.acc.cap, s), "clear counters");
+      hip_check(launch_fill8(w.accb.p, w.accb.cap, s), "clear counters");
     }
     if (!root) {
       ext_ms += ms;
       ext_launches++;
-      st.e_cand += ecand;
     }
-    st.split_passes += failed;
-    st.pruned += pruned;
-    st.max_table_fill = std::max<uint64_t>(st.max_table_fill, max_fill);
     // a level with recounted chunks wants a larger table next time
     if (failed * 64 > (unsigned long long)total_chunks && table_bits < 11 && opt.table_bits == 0) ++table_bits;
     last_fcount = cnt.fcount;
@@ -874,6 +935,75 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       sort_freq(src, F, (uint64_t)nkeys, fr, frtmp);
     }
     dev_sec += tdev.sec();
+
+    // Under a budget: the children of the piece's keys not written yet
+    // (key_lo on) must fit beside the levels held.  Else the piece is cut at
+    // the last parent pattern whose children fit and its stage run again
+    // over that range (results of a run are per pattern, so the shorter run
+    // finds the same keys); a single pattern whose children do not fit is
+    // written in key ranges, each with at least one child.
+    const uint64_t key_lo = std::max<uint64_t>((uint64_t)p_begin * (uint64_t)K, Fr.key_next);
+    uint64_t key_cut = ~0ull;  // keys [key_lo, key_cut) become children
+    if (budget > 0 && !past_max && F > 0) {
+      const int CSb = (int)round4(root ? 2 : std::min(L.max_nv + 1, kMaxPatV));
+      auto kbytes = [&](int64_t i) { return rows_bytes((int64_t)fr[i].proj, CSb) + (int64_t)fr[i].items * 8; };
+      int64_t total = 0;
+      for (int64_t i = 0; i < F; ++i)
+        if (fr[i].key >= key_lo) total += kbytes(i);
+      if (total > avail) {
+        // the last parent boundary whose children fit
+        int64_t acc_b = 0, p_cut = p_begin;
+        for (int64_t i = 0; i < F;) {
+          const int64_t p = (int64_t)(fr[i].key / (uint64_t)K);
+          int64_t grp = 0, j = i;
+          for (; j < F && (int64_t)(fr[j].key / (uint64_t)K) == p; ++j)
+            if (fr[j].key >= key_lo) grp += kbytes(j);
+          if (acc_b + grp > avail) {
+            p_cut = p;
+            break;
+          }
+          acc_b += grp;
+          i = j;
+        }
+        const bool measured_again = p_end_measured > 0;
+        if ((p_cut > p_begin && p_cut < p_end) || (p_cut == p_begin && p_end > p_begin + 1)) {
+          // run the stage again over the patterns whose children fit (or
+          // the first pattern alone), from cleared counters
+          Fr.p_end_next = std::max<int64_t>(p_cut, p_begin + 1);
+          if (measured_again && Fr.p_end_next >= p_end) throw Error(GM_E_DEVICE, "internal: piece did not shrink");
+          hip_check(launch_fill8(w.acc.p, w.acc.cap, s), "clear counters");
+          hip_check(launch_fill8(w.accb.p, w.accb.cap, s), "clear counters");
+          continue;
+        }
+        if (p_end == p_begin + 1) {
+          // one pattern: its keys in order while they fit, at least one
+          int64_t acc_k = 0;
+          bool first = true;
+          for (int64_t i = 0; i < F; ++i) {
+            if (fr[i].key < key_lo) continue;
+            if (!first && acc_k + kbytes(i) > avail) {
+              key_cut = fr[i].key;
+              break;
+            }
+            acc_k += kbytes(i);
+            first = false;
+          }
+        }
+      }
+    }
+    // a key of this piece that becomes a child now
+    auto in_piece = [&](int64_t i) { return fr[i].key >= key_lo && fr[i].key < key_cut; };
+    // e_cand and the stage statistics count each pattern's enumeration once
+    // (a piece is either all new patterns or one pattern seen before)
+    if (p_begin < Fr.p_counted && p_end > Fr.p_counted)
+      throw Error(GM_E_DEVICE, "internal: piece mixes counted and new patterns");
+    if (p_begin >= Fr.p_counted) {
+      if (!root) st.e_cand += ecand;
+      st.split_passes += failed;
+      st.pruned += pruned;
+      st.max_table_fill = std::max<uint64_t>(st.max_table_fill, max_fill);
+      Fr.p_counted = p_end;
+    }
     Timer t_host;
 
     // decode the frequent keys into child tuples
@@ -962,7 +1092,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         const uint32_t sup = (uint32_t)fr[i].sup;
         const int cnv = knv[i];
         bool keep = false;
-        if (opt.queries) {  // follow the query codes only
+        if (!in_piece(i)) {  // a later piece's
+          keep = false;
+        } else if (opt.queries) {  // follow the query codes only
           const int32_t parent = root ? 0 : L.node[fr[i].key / K];
           const int32_t node = opt.queries->child(parent, ktup[i]);
           knode[i] = node;
@@ -980,7 +1112,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       });
       if (too_big) throw Error(GM_E_UNSUPPORTED, "pattern with more than 32 vertices is frequent");
     } else {
-      for (int64_t i = 0; i < F; ++i) kidx[i] = 0;  // decoded while the device writes
+      for (int64_t i = 0; i < F; ++i) kidx[i] = in_piece(i) ? 0 : -1;  // decoded while the device writes
     }
     canon_sec += tcan.sec();
     double us_canon = tcan.sec() * 1e6;
@@ -1002,6 +1134,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     double us_host = 0, us_overlap = 0;
     int64_t alive = 0, alive_rows = 0;  // minimal children and their embeddings
     int CS = 4;
+    int child_buf = 0, child_mask_stride = 0;  // the children's row set and upload layout
+    size_t child_mask_off = 0, child_code_off = 0, child_vlab_off = 0;
     uint64_t child_items = 0;  // the children's list entries
     // Upload (mapped staging, read by k_cursors and copied by it): the
     // children's metadata, then per frequent key its key and child id (-1:
@@ -1015,7 +1149,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const size_t lb_off = rb_off + sizeof(unsigned long long) * C;
     // path masks of the children with >= 2 edges: a forward and a
     // backward word per rightmost-path vertex (at most max_nv of them)
-    const bool masks = prune && new_len >= 2 && C > 0;
+    // (a 2-edge level written in pieces leaves no complete path table)
+    if (new_len == 2 && !(p_begin == 0 && p_end == P && key_lo == 0 && key_cut == ~0ull)) t2_partial = true;
+    const bool masks = prune && !t2_partial && new_len >= 2 && C > 0;
     const int mstride = masks ? max_nv : 0;
     const size_t masks_off = lb_off + sizeof(unsigned long long) * C;
     // then per child its packed code (from | to << 8 | elabel << 16 |
@@ -1063,15 +1199,24 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       // upload.
       Timer tdev2;
       CS = (int)round4(max_nv);
-      const int nxt = cur ^ 1;
+      // a row set no level on the stack holds (this one's is read by the write)
+      auto held_by_stack = [&](int b) {
+        for (const Frame &f : frames)
+          if (f.buf == b) return true;
+        return false;
+      };
+      int nxt = 0;
+      while (held_by_stack(nxt)) ++nxt;
+      if (w.arena && nxt > 1) throw Error(GM_E_DEVICE, "internal: row set beyond the static workspace's two");
+      child_buf = nxt;
       PatMeta *hm = reinterpret_cast<PatMeta *>(hu);
-      char *du = static_cast<char *>(grow(w.upl[nxt], up_bytes, s));
+      char *du = static_cast<char *>(grow(w.row(nxt).upl, up_bytes, s));
       const char *su = static_cast<const char *>(w.up.dev);
-      grow(w.pid[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
-      grow(w.rgraph[nxt], sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
-      grow(w.vt[nxt], (size_t)CS * std::max<int64_t>(child_rows, 1), s);
-      grow(w.rbound[nxt], sizeof(uint32_t) * std::max<int64_t>(child_rows, 1), s);
-      grow(w.rlist[nxt], sizeof(unsigned long long) * std::max<int64_t>(child_rows, 1), s);
+      grow(w.row(nxt).pid, sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
+      grow(w.row(nxt).rgraph, sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
+      grow(w.row(nxt).vt, (size_t)CS * std::max<int64_t>(child_rows, 1), s);
+      grow(w.row(nxt).rbound, sizeof(uint32_t) * std::max<int64_t>(child_rows, 1), s);
+      grow(w.row(nxt).rlist, sizeof(unsigned long long) * std::max<int64_t>(child_rows, 1), s);
       grow(w.cur_rows, sizeof(unsigned long long) * C * rep, s);
       grow(w.cur_list, sizeof(unsigned long long) * C * rep, s);
       grow(w.cursor, sizeof(unsigned long long) * C * rep, s);
@@ -1079,11 +1224,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.cur_list = w.cur_list.as<unsigned long long>();
       a.cursor = w.cursor.as<unsigned long long>();
       a.ccap = child_rows;
-      a.cpid = w.pid[nxt].as<int32_t>();
-      a.crgraph = w.rgraph[nxt].as<int32_t>();
-      a.cvt = w.vt[nxt].as<uint8_t>();
-      a.cbound = w.rbound[nxt].as<uint32_t>();
-      a.crlist = w.rlist[nxt].as<unsigned long long>();
+      a.cpid = w.row(nxt).pid.as<int32_t>();
+      a.crgraph = w.row(nxt).rgraph.as<int32_t>();
+      a.cvt = w.row(nxt).vt.as<uint8_t>();
+      a.cbound = w.row(nxt).rbound.as<uint32_t>();
+      a.crlist = w.row(nxt).rlist.as<unsigned long long>();
       a.CS = CS;
       a.chunk0 = 0;
       a.n_chunks = total_chunks;
@@ -1226,7 +1371,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       int64_t nonmin_rows = dead_rows;
       if (!spec && !root && !opt.queries)
         for (int64_t i = 0; i < F; ++i)
-          if (kidx[i] < 0) nonmin_rows += (int64_t)fr[i].proj;
+          if (kidx[i] < 0 && in_piece(i)) nonmin_rows += (int64_t)fr[i].proj;
       dead_share = (double)nonmin_rows / (double)std::max<int64_t>(1, alive_rows + nonmin_rows);
       if (w.arena) {
         hip_check(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, s), "upload children");
@@ -1238,15 +1383,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       dev_sec += tdev2.sec();
       size_t live = db.bytes + w.sup.cap + w.acc.cap + w.accb.cap + w.freq.cap + w.rccnt.cap + w.cand.cap +
                     w.chunks.cap + w.fail.cap + w.cur_rows.cap + w.cur_list.cap + w.cursor.cap + w.chunk_keys.cap;
-      for (int i = 0; i < 2; ++i)
-        live += w.pid[i].cap + w.rgraph[i].cap + w.vt[i].cap + w.rbound[i].cap + w.rlist[i].cap + w.upl[i].cap;
+      for (const RowSet &rs : w.rows) live += rs.bytes();
       peak = std::max(peak, live);
-      cur = nxt;
-      mask_off = masks_off;
-      code_off = codes_off;
-      vlab_off = cvl_off;
-      codes_ready = want_codes;
-      mask_stride = masks && t2_ready ? mstride : 0;
+      child_mask_off = masks_off;
+      child_code_off = codes_off;
+      child_vlab_off = cvl_off;
+      child_mask_stride = masks && t2_ready ? mstride : 0;
     }
     if (root)
       st.e_root += alive_rows;
@@ -1270,20 +1412,44 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     }
     if (trace_levels)
       fprintf(stderr,
-              "[gm level %d] patterns %lld rows %lld K %lld chunks %lld table_bits %d stage_us %.1f "
+              "[gm level %d] piece %lld-%lld of %lld patterns, rows %lld of %lld K %lld chunks %lld table_bits %d stage_us %.1f "
               "candidates %llu pruned %llu frequent %u children %lld child_rows %lld minimal %lld spec %d "
               "failed %llu max_fill %llu list_cap %llu canon_dev %d host_us canon %.1f host %.1f overlap %.1f\n",
-              level, (long long)P, (long long)rows_now, (long long)K, (long long)total_chunks, table_bits, ms * 1e3,
+              level, (long long)p_begin, (long long)p_end, (long long)P, (long long)(prow_hi - prow_lo), (long long)rows_now,
+              (long long)K, (long long)total_chunks, table_bits, ms * 1e3,
               ecand, pruned, cnt.fcount, (long long)C, (long long)child_rows, (long long)alive, spec ? 1 : 0, failed,
               max_fill, (unsigned long long)list_cap, canon_dev ? 1 : 0, us_canon, us_host, us_overlap);
     st.n_levels++;
     if (stopped) break;
-    std::swap(L, N);
-    S = CS;
-    n_rows = child_rows;
-    // the children's lists: exactly their item counts
-    list_cap = std::max<uint64_t>(child_items, 1);
-    ++level;
+    // the piece is written: the level continues after it, or is done (its
+    // rows are still read by the queued write only); the children, if any,
+    // are the next level
+    if (key_cut != ~0ull) {
+      Fr.key_next = key_cut;
+    } else {
+      Fr.p_next = p_end;
+      Fr.key_next = 0;
+    }
+    if (budget > 0 && key_cut == ~0ull && prow_hi > prow_lo) {
+      const int64_t cb = rows_bytes(child_rows, CS) + (int64_t)child_items * 8;
+      piece_bytes_per_row = std::max(1.0, (double)cb / (double)(prow_hi - prow_lo));
+    }
+    if (Fr.p_next >= P) frames.pop_back();
+    if (C > 0) {
+      frames.emplace_back();
+      Frame &cf = frames.back();
+      std::swap(cf.L, N);
+      cf.level = level + 1;
+      cf.buf = child_buf;
+      cf.S = CS;
+      cf.n_rows = child_rows;
+      cf.list_cap = std::max<uint64_t>(child_items, 1);  // the children's lists: exactly their item counts
+      cf.mask_off = child_mask_off;
+      cf.mask_stride = child_mask_stride;
+      cf.code_off = child_code_off;
+      cf.vlab_off = child_vlab_off;
+      cf.codes_ready = want_codes;
+    }
   }
   // the last write's row count
   if (written_expect) {

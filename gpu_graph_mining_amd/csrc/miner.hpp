@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <deque>
 #include <functional>
 #include <memory>
 #include <mutex>
@@ -75,13 +76,26 @@ struct HostBuf {
   T *as() const { return static_cast<T *>(p); }
 };
 
+// One level's rows (pid/rgraph/vt/rbound/rlist) and its pattern upload
+// (metadata, path masks, codes).
+struct RowSet {
+  DevBuf pid, rgraph, vt, rbound, rlist, upl;
+  std::vector<DevBuf *> bufs() { return {&pid, &rgraph, &vt, &rbound, &rlist, &upl}; }
+  size_t bytes() const { return pid.cap + rgraph.cap + vt.cap + rbound.cap + rlist.cap + upl.cap; }
+};
+
 // Device buffers of the level steps, kept across mining calls on one
 // database so that repeated calls allocate nothing once warm.
 struct Workspace {
   DevBuf minsup, sup, acc, accb, freq, rccnt, cur_rows, cur_list, cursor, fail, chunks, chunk_keys, cand;
-  // double-buffered by level: rows (pid/rgraph/vt/rbound/rlist) and the
-  // pattern upload (metadata, path masks)
-  DevBuf pid[2], rgraph[2], vt[2], rbound[2], rlist[2], upl[2];
+  // row sets: a level's rows stay in one while its children are written to
+  // another; two alternate, more are held while levels are mined in pieces
+  // under a memory budget
+  std::deque<RowSet> rows{2};
+  RowSet &row(int i) {
+    while ((int)rows.size() <= i) rows.emplace_back();
+    return rows[(size_t)i];
+  }
   HostBuf up, down;
   hipEvent_t ev[2] = {nullptr, nullptr};  // stage timing (MineOptions::stage_events)
   // the packed counters hold zeros outside a level step (k_reduce clears
@@ -124,7 +138,7 @@ struct Workspace {
   std::vector<DevBuf *> scratch_bufs() {
     return {&sup, &acc, &accb, &freq, &rccnt, &cur_rows, &cur_list, &cursor, &fail, &chunks, &chunk_keys, &cand};
   }
-  std::vector<DevBuf *> row_bufs(int i) { return {&pid[i], &rgraph[i], &vt[i], &rbound[i], &rlist[i], &upl[i]}; }
+  std::vector<DevBuf *> row_bufs(int i) { return row(i).bufs(); }
   ~Workspace() {
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -192,6 +206,11 @@ struct MineOptions {
   bool stage_events = false;    // time the extension + support stages with HIP events
   int32_t chunk_rows = 0;       // launch shape (gm_cfg::chunk_rows); 0 = default
   int32_t table_bits = 0;       // launch shape (gm_cfg::table_bits); 0 = default
+  // device bytes for the levels' rows and candidate lists (gm_cfg::
+  // mem_budget_bytes); a level whose children would pass it is written in
+  // pieces by parent pattern, each piece's subtree mined before the next
+  // (0: unlimited)
+  int64_t mem_budget = 0;
   // minimum-code check of a level's frequent keys: 0 on the device for the
   // levels whose children the host would check before writing them (no
   // speculation), else on the host pool overlapping the write; 1 always on

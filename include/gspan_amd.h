@@ -56,7 +56,11 @@ typedef struct gm_cfg {
   int32_t canon_threads;    /* host threads for the canonical check; 0 = auto */
   int32_t block_threads;    /* accepted for compatibility (0, 64, 128 or 256) and otherwise unused: the
                                level kernels run one 64-lane wave per chunk of rows */
-  int64_t mem_budget_bytes; /* device bytes the embedding store may use; 0 = auto */
+  int64_t mem_budget_bytes; /* device bytes for the embedding rows and candidate lists of the levels
+                               held at once; a level whose children would pass it is written in
+                               pieces by parent pattern, each piece's patterns mined to the end
+                               before the next (more when one pattern's own children need more);
+                               0 = unlimited (every level whole).  Ignored by gm_mine_noalloc */
   int32_t collect_proj;     /* 1 = also report |Proj(P)| per pattern (no extra pass) */
   int32_t candidate_lists;  /* accepted for compatibility and unused: the count pass always keeps each
                                embedding's candidates for the write pass */
