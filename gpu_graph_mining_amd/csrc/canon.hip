@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(64) k_canon_level(CanonArgs c) {
   for (uint32_t slot = blockIdx.x; slot < nf; slot += gridDim.x) {
     unsigned long long *f = c.freq + 4ull * slot;
     const unsigned long long key = f[0];
-    const int64_t p = (int64_t)(key / (unsigned long long)c.K);
+    const int64_t p = (int64_t)(key / (unsigned long long)c.K) + c.p_base;
     const int64_t x = (int64_t)(key % (unsigned long long)c.K);
     const int nvp = c.meta[p].nv;
     const int len = c.len;

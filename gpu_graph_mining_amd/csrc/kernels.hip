@@ -541,7 +541,7 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : 5)) k_level(StepArgs a) {  
     // commit the counts to the chunk's replica; keep the chunk's
     // (key, embeddings, item sum) records
     const int64_t gci = a.chunk0 + ci;
-    const int64_t roff = (gci & (a.rep - 1)) * a.nkeys + (int64_t)a.p_lo * a.K;
+    const int64_t roff = (gci & (a.rep - 1)) * a.nkeys + (int64_t)(a.p_lo - a.p_base) * a.K;
     uint4 *rec = a.chunk_keys + gci * a.chunk_key_cap;
     uint32_t nrec = 0;
     for (uint32_t i0 = 0; i0 < tab.cap; i0 += 64) {
@@ -676,7 +676,7 @@ __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
         if (k == ~0ull) continue;
         const int64_t hr = c.start + (int64_t)(k >> 32);
         const int32_t p = ROOT ? a.gjob[a.vgraph[hr]] : a.pid[hr];
-        const int64_t at = roff + (int64_t)p * a.K + (int64_t)(uint32_t)k;
+        const int64_t at = roff + (int64_t)(p - a.p_base) * a.K + (int64_t)(uint32_t)k;
         atomicAdd(&a.acc[at], (1ull << kSupShift) | (unsigned long long)tcnt[i]);
         atomicAdd(&a.accb[at], (unsigned long long)tbs[i]);
       }
@@ -706,7 +706,7 @@ __global__ void __launch_bounds__(256) k_reduce(StepArgs a) {
     a.sup[i] = s;
     bool hit = false;
     if (s > 0) {
-      const int64_t p = i / a.K;
+      const int64_t p = i / a.K + a.p_base;
       const int32_t owner = ROOT ? (int32_t)p : a.meta[p].job;
       hit = s >= a.minsup_job[owner];
     }
@@ -859,7 +859,7 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
       const uint4 *rec = a.chunk_keys + (int64_t)blockIdx.x * a.chunk_key_cap;
       for (uint32_t i = lane; i < c.nkeys; i += 64) {
         const uint4 kr = rec[i];
-        const int32_t ch = child_of((int64_t)c.p_lo * a.K + (int64_t)kr.x);
+        const int32_t ch = child_of((int64_t)(c.p_lo - a.p_base) * a.K + (int64_t)kr.x);
         if (ch >= 0 && !table_add((uint32_t)ch, kr.y, kr.z, cap)) atomicOr(&s_over, 1);
       }
     } else {
@@ -880,7 +880,7 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
           int key, w;
           const uint2 ce = a.cand[lo + j];
           cand_decode(ROOT, ce.x, key, w);
-          const int32_t ch = child_of((int64_t)p * a.K + key);
+          const int32_t ch = child_of((int64_t)(p - a.p_base) * a.K + key);
           if (ch < 0 || (mod > 1 && ((uint32_t)ch & (mod - 1)) != res)) continue;
           if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
           if (!table_add((uint32_t)ch, 1u, ce.y, cap - cap / 8)) atomicOr(&s_over, 1);
@@ -979,7 +979,7 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
           if (jb0 + u * 64 + lane < T) {
             int key, w;
             cand_decode(ROOT, ent[u].x, key, w);
-            chv[u] = child_of((int64_t)s_p[kk[u]] * a.K + key);
+            chv[u] = child_of((int64_t)(s_p[kk[u]] - a.p_base) * a.K + key);
           }
         }
 #pragma unroll

@@ -15,6 +15,9 @@ namespace gm {
 constexpr int kChunkRows = 256;
 constexpr int kMinChunkRows = 64;
 constexpr int64_t kLevelWaves = 16384;  // waves a level step aims for
+// Counter keys of one level piece at most (16 bytes per key and replica): a
+// level with more patterns x keys per pattern is mined in pieces.
+constexpr int64_t kPieceKeys = 1ll << 26;
 constexpr int kMaxTupleWords = kMaxPatV / 4;  // 8
 
 // Statistics counters that every wave bumps are spread over kSpread cache
@@ -70,6 +73,7 @@ struct StepArgs {
   int32_t mask_stride;
   int64_t K;              // keys per pattern (root: keys per job)
   int32_t p_lo, p_hi;     // patterns (root: jobs) of this launch; (p - p_lo) * K < 2^32
+  int32_t p_base;         // first pattern of the level piece: counter keys are (p - p_base) * K + key
   int32_t NEL, NVL;
   // chunks of this launch: chunk i at chunks[chunk0 + i], n_chunks of them,
   // chunk_rows nominal rows each
@@ -158,6 +162,7 @@ struct CanonArgs {
   // the level's patterns: nv from meta, len tuples each packed from | to << 8
   // | elabel << 16 | tolabel << 24 (dense labels), kMaxPatV vertex labels each
   const PatMeta *meta;
+  int32_t p_base;  // record keys are (pattern - p_base) * K + key
   const uint32_t *pcode;
   const uint8_t *pvlab;
   int32_t len;

@@ -672,7 +672,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     size_t code_off = 0, vlab_off = 0;  // packed codes and vertex labels for the device check
     bool codes_ready = false;
     int64_t p_next = 0;      // first pattern whose children are not all written
-    uint64_t key_next = 0;   // first key of p_next not written (a pattern split over pieces)
+    uint64_t key_next = 0;   // first key (relative to p_next) not written: a pattern split over pieces
     int64_t p_counted = 0;   // patterns whose candidates are in e_cand
     int64_t p_end_next = 0;  // > 0: the next piece ends there (its children were measured)
   };
@@ -707,7 +707,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     Timer tdev;
     const int64_t P = root ? db.n_jobs : L.n;
     const int64_t K = root ? (int64_t)NVL * NEL * NVL : (int64_t)S * NEL * (NVL + 1);
-    const int64_t nkeys = P * K;
     const int64_t rows_now = root ? db.n_vertices : n_rows;
     auto prow = [&](int64_t p) { return root ? job_row0[p] : L.row0[p]; };
     // launches over pattern ranges with range * K < 2^32 (32-bit table keys)
@@ -740,6 +739,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       }
       p_end = lo;
     }
+    // the counters cover the piece's keys only: (p - p_begin) * K + key,
+    // at most kPieceKeys of them (a level with more is cut into pieces)
+    p_end = std::min<int64_t>(p_end, p_begin + std::max<int64_t>(1, kPieceKeys / K));
+    const int64_t nkeys = (p_end - p_begin) * K;
+    auto parent_of = [&](uint64_t key) { return (int64_t)(key / (uint64_t)K) + p_begin; };
     const int64_t prow_lo = prow(p_begin), prow_hi = prow(p_end);
     // chunk length: kChunkRows, or shorter so that the piece has about
     // kLevelWaves chunks (a power of two, at least one block)
@@ -764,7 +768,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     grow(w.chunk_keys, sizeof(uint4) * level_chunk_keys(table_bits) * std::max<int64_t>(total_chunks, 1), s);
     grow(w.rccnt, sizeof(uint16_t) * std::max<int64_t>(rows_now, 1), s);
     grow(w.cand, sizeof(uint2) * std::max<uint64_t>(list_cap, 1), s);
-    uint32_t fcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(nkeys, 1), std::max<int64_t>(1 << 16, P * 64));
+    uint32_t fcap = (uint32_t)std::min<int64_t>(std::max<int64_t>(nkeys, 1), std::max<int64_t>(1 << 16, (p_end - p_begin) * 64));
     // the children would pass max_edges: none is kept
     const bool past_max = opt.max_edges > 0 && (root ? 1 : L.len + 1) > opt.max_edges;
     // the minimum-code check of this level's frequent keys runs on the
@@ -811,6 +815,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.error_flag = &dctr->err;
       a.sup = w.sup.as<int32_t>();
       a.nkeys = nkeys;
+      a.p_base = (int32_t)p_begin;
       a.K = K;
       a.minsup_job = w.minsup.as<int32_t>();
       a.freq = reinterpret_cast<unsigned long long *>(dctr + 1);
@@ -861,6 +866,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         c.freq_count = a.freq_count;
         c.freq_cap = fcap;
         c.meta = a.meta;
+        c.p_base = (int32_t)p_begin;
         c.pcode = reinterpret_cast<const uint32_t *>(w.row(cur).upl.as<char>() + code_off);
         c.pvlab = reinterpret_cast<const uint8_t *>(w.row(cur).upl.as<char>() + vlab_off);
         c.len = L.len;
@@ -942,7 +948,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // over that range (results of a run are per pattern, so the shorter run
     // finds the same keys); a single pattern whose children do not fit is
     // written in key ranges, each with at least one child.
-    const uint64_t key_lo = std::max<uint64_t>((uint64_t)p_begin * (uint64_t)K, Fr.key_next);
+    // (piece keys: the pattern's next key when it is split over pieces)
+    const uint64_t key_lo = Fr.key_next;
     uint64_t key_cut = ~0ull;  // keys [key_lo, key_cut) become children
     if (budget > 0 && !past_max && F > 0) {
       const int CSb = (int)round4(root ? 2 : std::min(L.max_nv + 1, kMaxPatV));
@@ -954,9 +961,9 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         // the last parent boundary whose children fit
         int64_t acc_b = 0, p_cut = p_begin;
         for (int64_t i = 0; i < F;) {
-          const int64_t p = (int64_t)(fr[i].key / (uint64_t)K);
+          const int64_t p = parent_of(fr[i].key);
           int64_t grp = 0, j = i;
-          for (; j < F && (int64_t)(fr[j].key / (uint64_t)K) == p; ++j)
+          for (; j < F && parent_of(fr[j].key) == p; ++j)
             if (fr[j].key >= key_lo) grp += kbytes(j);
           if (acc_b + grp > avail) {
             p_cut = p;
@@ -1023,7 +1030,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         t = DFS{0, 1, fl, el, tl};
         cnv = 2;
       } else {
-        const int64_t p = (int64_t)(fr[i].key / K);
+        const int64_t p = parent_of(fr[i].key);
         const uint8_t *pv = L.vl(p);
         const int nv = L.nv[p], maxtoc = nv - 1;
         if (x < (int64_t)S * NEL) {
@@ -1045,7 +1052,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const int max_nv_bound = root ? 2 : L.max_nv + 1;
     // the canonical check of child i (its parent's code plus its tuple)
     auto is_min_child = [&](int64_t i) {
-      const int64_t p = (int64_t)(fr[i].key / K);
+      const int64_t p = parent_of(fr[i].key);
       thread_local std::vector<DFS> code;
       code.assign(L.c(p), L.c(p) + L.len);
       code.push_back(ktup[i]);
@@ -1095,7 +1102,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         if (!in_piece(i)) {  // a later piece's
           keep = false;
         } else if (opt.queries) {  // follow the query codes only
-          const int32_t parent = root ? 0 : L.node[fr[i].key / K];
+          const int32_t parent = root ? 0 : L.node[parent_of(fr[i].key)];
           const int32_t node = opt.queries->child(parent, ktup[i]);
           knode[i] = node;
           keep = node >= 0 && query_reached(node, sup) && cnv <= kMaxPatV;
@@ -1250,8 +1257,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       auto build_child = [&](int64_t i) {
         const int64_t c = kidx[i];
         if (c < 0) return;
-        const uint64_t key = fr[i].key;
-        const int64_t p = key / K;
+        const int64_t p = parent_of(fr[i].key);
         DFS *cc = N.c(c);
         uint8_t *cv = N.vl(c);
         const DFS &t = ktup[i];
@@ -1425,7 +1431,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // rows are still read by the queued write only); the children, if any,
     // are the next level
     if (key_cut != ~0ull) {
-      Fr.key_next = key_cut;
+      Fr.key_next = key_cut;  // a key of pattern p_next, relative to it (its pieces start there)
     } else {
       Fr.p_next = p_end;
       Fr.key_next = 0;
