@@ -46,7 +46,7 @@ EXPORTED_SYMBOLS = (
     "gm_db_write_txt", "gm_db_counts", "gm_db_export", "gm_db_upload", "gm_db_destroy", "gm_mine",
     "gm_mine_batch", "gm_dfs_code_is_min", "gm_gen_random", "gm_gen_planted", "gm_gen_molecule", "gm_db_subset",
     "gm_batch_create", "gm_batch_mine", "gm_batch_destroy", "gm_support_queries", "gm_support_query",
-    "gm_db_write", "gm_db_load_part", "gm_mine_noalloc", "gm_dfs_codes_are_min",
+    "gm_db_write", "gm_db_load_part", "gm_mine_noalloc", "gm_dfs_codes_are_min", "gm_batch_create_subsets",
 )
 
 
@@ -118,6 +118,7 @@ def load_library(path: Optional[os.PathLike] = None) -> C.CDLL:
                                       C.c_int32, P(C.c_double), P(dbp)]),
         "gm_db_subset": (C.c_int, [dbp, i32p, C.c_int32, P(dbp)]),
         "gm_batch_create": (C.c_int, [C.c_int32, P(dbp), P(dbp)]),
+        "gm_batch_create_subsets": (C.c_int, [dbp, C.c_int32, i32p, i32p, P(dbp)]),
         "gm_batch_mine": (C.c_int, [dbp, i32p, P(gm_cfg), JOB_PATTERN_CB, C.c_void_p, P(gm_stats)]),
         "gm_batch_destroy": (None, [dbp]),
         "gm_support_queries": (C.c_int, [dbp, C.c_int32, i32p, i32p, P(gm_cfg), P(C.c_uint32), P(gm_stats)]),
@@ -622,6 +623,24 @@ class Batch:
         _check(lib.gm_batch_create(len(dbs), handles, C.byref(h)))
         self._h = h
         self.n_jobs = len(dbs)
+
+    @classmethod
+    def from_subsets(cls, db: GraphDB, subsets: Sequence[Sequence[int]]) -> "Batch":
+        """Jobs that are subsets of the resident database `db` (graph ids
+        per job, in order; they may overlap), gathered on the device from
+        db's own device copy (gm_batch_create_subsets)."""
+        lib = load_library()
+        begin = [0]
+        for ids in subsets:
+            begin.append(begin[-1] + len(ids))
+        flat = _i32([g for ids in subsets for g in ids] or [0])
+        b = _i32(begin)
+        h = C.c_void_p()
+        _check(lib.gm_batch_create_subsets(db._h, len(subsets), _ptr(b), _ptr(flat), C.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self.n_jobs = len(subsets)
+        return self
 
     def mine(self, minsups: Sequence[int], *, label_style: int = GM_LABELS_FULL, collect_proj: bool = False,
              canon_threads: int = 0, count_only: bool = False, path_pruning: bool = True, speculate: int = 0,

@@ -415,6 +415,21 @@ int gm_batch_create(int32_t n_jobs, gm_db *const *dbs, gm_batch **out) {
   });
 }
 
+int gm_batch_create_subsets(gm_db *db, int32_t n_jobs, const int32_t *job_begin, const int32_t *graph_ids,
+                            gm_batch **out) {
+  return guard([&]() {
+    if (!db || n_jobs < 1 || !job_begin || !out || (job_begin[n_jobs] > 0 && !graph_ids))
+      throw gm::Error(GM_E_ARG, "null argument");
+    *out = nullptr;
+    ensure_device(db);
+    std::unique_ptr<gm_batch> b(new gm_batch());
+    gm::hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
+    // the source's upload is complete (ensure_device synchronises its stream)
+    gm::gather_db(*db->dev, db->host, n_jobs, job_begin, graph_ids, b->dev, b->stream);
+    *out = b.release();
+  });
+}
+
 int gm_batch_mine(gm_batch *batch, const int32_t *abs_minsup, const gm_cfg *cfg, gm_job_pattern_cb cb, void *user,
                   gm_stats *stats) {
   return guard([&]() {

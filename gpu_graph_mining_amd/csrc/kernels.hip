@@ -1042,6 +1042,35 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
     atomicAdd(&a.spread[(blockIdx.x & (kSpread - 1)) * kSpreadStride + kSpreadWritten], written);
 }
 
+// ----------------------------------------------------------- SUBSETS
+// A packed database of selected graphs gathered from a resident one: one
+// workgroup per selected graph t (source graph sel[t]) copies its vertex
+// labels, its adjacency entries (local ids, labels and degrees unchanged)
+// and its vertices' adjacency offsets rebased to dab[t]; graph t starts at
+// vertex dvb[t] and belongs to job djob[t].  The last workgroup writes the
+// closing offsets.
+__global__ void __launch_bounds__(256) k_gather_graphs(GatherArgs g) {
+  const int64_t t = blockIdx.x;
+  const int32_t src = g.sel[t];
+  const int32_t v0 = g.gvbase[src], n = g.gvbase[src + 1] - v0;
+  const int32_t a0 = g.vadj[v0], na = g.vadj[v0 + n] - a0;
+  const int32_t dv = g.dvb[t], da = g.dab[t];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    g.out_vlab[dv + i] = g.vlab[v0 + i];
+    g.out_vadj[dv + i] = da + (g.vadj[v0 + i] - a0);
+    g.out_vgraph[dv + i] = (int32_t)t;
+  }
+  for (int i = threadIdx.x; i < na; i += blockDim.x) g.out_adj[da + i] = g.adj[a0 + i];
+  if (threadIdx.x == 0) {
+    g.out_gvbase[t] = dv;
+    g.out_gjob[t] = g.djob[t];
+    if (t == g.n_sel - 1) {
+      g.out_gvbase[t + 1] = dv + n;
+      g.out_vadj[dv + n] = da + na;
+    }
+  }
+}
+
 // ------------------------------------------------------------ primitives
 // Copies between device memory and mapped pinned host memory on the compute
 // queue (a copy engine transfer between two kernels costs a cross-engine
@@ -1164,6 +1193,12 @@ hipError_t launch_cursors(const StepArgs &a, void *dst, const void *src, size_t 
   hipLaunchKernelGGL(k_cursors, dim3(copy_grid(std::max(n, F))), dim3(256), 0, s, a,
                      static_cast<unsigned long long *>(dst), static_cast<const unsigned long long *>(src), n, fkeys,
                      fchild, row_base, list_base, F);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_graphs(const GatherArgs &g, hipStream_t s) {
+  if (g.n_sel <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_graphs, dim3((unsigned)g.n_sel), dim3(256), 0, s, g);
   return hipGetLastError();
 }
 

@@ -139,6 +139,20 @@ hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s)
 hipError_t launch_cursors(const StepArgs &a, void *dst, const void *src, size_t bytes,
                           const unsigned long long *fkeys, const int32_t *fchild, const unsigned long long *row_base,
                           const unsigned long long *list_base, int64_t F, hipStream_t s);
+// Gather selected graphs of a resident packed database into a new one
+// (k_gather_graphs): per selected graph t, its source graph, first vertex
+// and first adjacency entry in the output, and its job.
+struct GatherArgs {
+  const int32_t *gvbase, *vadj;
+  const uint32_t *adj;
+  const uint8_t *vlab;
+  const int32_t *sel, *dvb, *dab, *djob;
+  int64_t n_sel;
+  int32_t *out_gvbase, *out_vadj, *out_gjob, *out_vgraph;
+  uint32_t *out_adj;
+  uint8_t *out_vlab;
+};
+hipError_t launch_gather_graphs(const GatherArgs &g, hipStream_t s);
 // Zero `bytes` (a multiple of 8) of device memory with a kernel.
 hipError_t launch_fill8(void *dst, size_t bytes, hipStream_t s);
 // The static workspace of the entry point that allocates nothing: a

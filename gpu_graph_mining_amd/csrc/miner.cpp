@@ -278,6 +278,86 @@ void upload_db(const PackedDB &pk, DeviceDB &out, hipStream_t s) {
               4 * pk.vgraph.size();
 }
 
+void gather_db(const DeviceDB &src, const HostDB &h, int32_t n_jobs, const int32_t *begin, const int32_t *ids,
+               DeviceDB &out, hipStream_t s) {
+  if (n_jobs < 1 || begin[0] != 0) throw Error(GM_E_ARG, "job ranges must start at 0");
+  const int32_t n_sel = begin[n_jobs];
+  std::vector<int32_t> sel(n_sel), dvb(n_sel), dab(n_sel), djob(n_sel);
+  int64_t nv = 0, na = 0;
+  int32_t max_v = 0, max_a = 0;
+  out.n_jobs = n_jobs;
+  out.job_first_graph.assign(1, 0);
+  out.job_first_vertex.assign(1, 0);
+  for (int32_t j = 0; j < n_jobs; ++j) {
+    if (begin[j + 1] < begin[j]) throw Error(GM_E_ARG, "job ranges must not decrease");
+    for (int32_t t = begin[j]; t < begin[j + 1]; ++t) {
+      const int32_t g = ids[t];
+      if (g < 0 || g >= h.n_graphs) throw Error(GM_E_ARG, "graph id out of range");
+      const int32_t n = h.vbegin[g + 1] - h.vbegin[g], a = 2 * (h.ebegin[g + 1] - h.ebegin[g]);
+      sel[t] = g;
+      dvb[t] = (int32_t)nv;
+      dab[t] = (int32_t)na;
+      djob[t] = j;
+      nv += n;
+      na += a;
+      max_v = std::max(max_v, n);
+      max_a = std::max(max_a, a);
+    }
+    out.job_first_graph.push_back(begin[j + 1]);
+    out.job_first_vertex.push_back(nv);
+  }
+  if (n_sel >= (1 << 24)) throw Error(GM_E_UNSUPPORTED, "more than 16777215 graphs (24-bit support counters)");
+  out.n_graphs = n_sel;
+  out.n_vertices = nv;
+  out.n_adj = na;
+  out.max_graph_v = max_v;
+  out.max_graph_adj = max_a;
+  out.NVL = src.NVL;
+  out.NEL = src.NEL;
+  out.vl_values = src.vl_values;
+  out.el_values = src.el_values;
+  out.gvbase.ensure(4 * (size_t)(n_sel + 1));
+  out.vadj.ensure(4 * (size_t)(nv + 1));
+  out.adj.ensure(4 * (size_t)std::max<int64_t>(na, 1));
+  out.vlab.ensure((size_t)std::max<int64_t>(nv, 1));
+  out.gjob.ensure(4 * (size_t)std::max<int32_t>(n_sel, 1));
+  out.vgraph.ensure(4 * (size_t)std::max<int64_t>(nv, 1));
+  out.bytes = 4 * (size_t)(n_sel + 1) + 4 * (size_t)(nv + 1) + 4 * (size_t)na + (size_t)nv + 4 * (size_t)n_sel +
+              4 * (size_t)nv;
+  if (n_sel == 0) {
+    const int32_t zero = 0;
+    hip_check(hipMemcpyAsync(out.gvbase.p, &zero, 4, hipMemcpyHostToDevice, s), "gather");
+    hip_check(hipMemcpyAsync(out.vadj.p, &zero, 4, hipMemcpyHostToDevice, s), "gather");
+    hip_check(hipStreamSynchronize(s), "gather");
+    return;
+  }
+  DevBuf idx;  // sel, dvb, dab, djob
+  idx.ensure(16 * (size_t)n_sel);
+  int32_t *d = idx.as<int32_t>();
+  hip_check(hipMemcpyAsync(d, sel.data(), 4 * (size_t)n_sel, hipMemcpyHostToDevice, s), "gather");
+  hip_check(hipMemcpyAsync(d + n_sel, dvb.data(), 4 * (size_t)n_sel, hipMemcpyHostToDevice, s), "gather");
+  hip_check(hipMemcpyAsync(d + 2 * (size_t)n_sel, dab.data(), 4 * (size_t)n_sel, hipMemcpyHostToDevice, s), "gather");
+  hip_check(hipMemcpyAsync(d + 3 * (size_t)n_sel, djob.data(), 4 * (size_t)n_sel, hipMemcpyHostToDevice, s), "gather");
+  GatherArgs g;
+  g.gvbase = src.gvbase.as<int32_t>();
+  g.vadj = src.vadj.as<int32_t>();
+  g.adj = src.adj.as<uint32_t>();
+  g.vlab = src.vlab.as<uint8_t>();
+  g.sel = d;
+  g.dvb = d + n_sel;
+  g.dab = d + 2 * (size_t)n_sel;
+  g.djob = d + 3 * (size_t)n_sel;
+  g.n_sel = n_sel;
+  g.out_gvbase = out.gvbase.as<int32_t>();
+  g.out_vadj = out.vadj.as<int32_t>();
+  g.out_gjob = out.gjob.as<int32_t>();
+  g.out_vgraph = out.vgraph.as<int32_t>();
+  g.out_adj = out.adj.as<uint32_t>();
+  g.out_vlab = out.vlab.as<uint8_t>();
+  hip_check(launch_gather_graphs(g, s), "k_gather_graphs");
+  hip_check(hipStreamSynchronize(s), "k_gather_graphs");  // before the index buffer is freed
+}
+
 namespace {
 
 // Growable host array without value-initialisation, reused across levels

@@ -175,6 +175,12 @@ struct PackedDB {
 void pack_db(const std::vector<const HostDB *> &dbs, DeviceDB &out, PackedDB &pk);
 // Copy a packed database to the GPU.
 void upload_db(const PackedDB &pk, DeviceDB &out, hipStream_t s);
+// A database of jobs whose graphs are subsets of a resident one, built on
+// the device (no host packing, no upload of graph data): job j holds graphs
+// ids[begin[j] .. begin[j+1]) of `src` (whose host copy gives their sizes),
+// in that order, with src's dense labels.
+void gather_db(const DeviceDB &src, const HostDB &src_host, int32_t n_jobs, const int32_t *begin, const int32_t *ids,
+               DeviceDB &out, hipStream_t s);
 
 // Reported pattern: job, fully labelled code in original labels, support,
 // |Proj| (0 unless collected).

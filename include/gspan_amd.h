@@ -220,6 +220,14 @@ int gm_mine_batch(int32_t n_jobs, gm_db *const *dbs, const int32_t *abs_minsup, 
  * per process). */
 typedef struct gm_batch gm_batch;
 int gm_batch_create(int32_t n_jobs, gm_db *const *dbs, gm_batch **out);
+/* A resident batch whose jobs are subsets of one resident database, built
+ * on the device from db's own device copy (no host packing, no upload of
+ * graph data): job j holds graphs graph_ids[job_begin[j] ..
+ * job_begin[j+1]) of db, in that order (job_begin[0] = 0; subsets may
+ * overlap).  Mining it gives each job the results of mining its subset on
+ * its own.  No reference counterpart (serving). */
+int gm_batch_create_subsets(gm_db *db, int32_t n_jobs, const int32_t *job_begin, const int32_t *graph_ids,
+                            gm_batch **out);
 /* Mine every job of the batch in one schedule with per-job min supports;
  * identical per-job results to gm_mine on each database. */
 int gm_batch_mine(gm_batch *batch, const int32_t *abs_minsup, const gm_cfg *cfg, gm_job_pattern_cb cb, void *user,
