@@ -325,7 +325,9 @@ __device__ __forceinline__ void count_insert(const CountTable &t, uint32_t gk, i
 // region and counts it in the wave's table.
 template <bool ROOT, int TW>
 __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : 5)) k_level(StepArgs a) {  // waves per SIMD the registers allow
-  extern __shared__ uint32_t dyn[];
+  // 16-byte aligned whatever static LDS precedes it: the table's segment
+  // masks take 64-bit LDS atomics
+  extern __shared__ __attribute__((aligned(16))) uint32_t dyn[];
   __shared__ int s_scal[2];
   const int lane = threadIdx.x;
   const int64_t ci = blockIdx.x;
