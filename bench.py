@@ -97,9 +97,21 @@ def _cpu_model() -> str:
     return platform.processor()
 
 
+def _tuned(config_id: str) -> dict:
+    """The autotuner's launch shapes for a workload (gpu_graph_mining_amd/
+    tuned.csv, written by `python -m gpu_graph_mining_amd.tune`), or {}."""
+    from gpu_graph_mining_amd import tune
+    p = ROOT / "gpu_graph_mining_amd" / "tuned.csv"
+    try:
+        return tune.read_config(p, config_id) if p.exists() else {}
+    except KeyError:
+        return {}
+
+
 def _batch_section(n_jobs: int, out: dict) -> None:
-    """Config 3: n_jobs independent jobs, batched in one schedule vs one job
-    at a time on the same GPU (same library, same results)."""
+    """Config 3: n_jobs independent jobs, batched in one schedule (with the
+    autotuned launch shapes) vs one job at a time with the defaults on the
+    same GPU (same library, same results)."""
     import gpu_graph_mining_amd as gm
     from gpu_graph_mining_amd import workloads as wl
     jobs = wl.config3_jobs()[:n_jobs]
@@ -108,8 +120,9 @@ def _batch_section(n_jobs: int, out: dict) -> None:
     for d in dbs:
         d.upload()
     batch = gm.Batch(dbs)  # packed and uploaded once, like the per-job databases
+    tuned = _tuned("config3")
     # parity: the batched schedule reports exactly each job's own patterns
-    res, _ = batch.mine(mins)
+    res, _ = batch.mine(mins, **tuned)
     alone = [gm.mine(d, m)[0] for d, m in zip(dbs, mins)]
     same = all(sorted((tuple(c), s) for c, s, _ in r) == sorted((tuple(c), s) for c, s, _ in a)
                for r, a in zip(res, alone))
@@ -120,11 +133,11 @@ def _batch_section(n_jobs: int, out: dict) -> None:
         seq.append(st["n_patterns"])
     t_seq = time.perf_counter() - t0
     t0 = time.perf_counter()
-    _, st = batch.mine(mins, count_only=True)
+    _, st = batch.mine(mins, count_only=True, **tuned)
     t_bat = time.perf_counter() - t0
     out["batch"] = {"jobs": n_jobs, "jobs_per_s_batched": n_jobs / t_bat, "jobs_per_s_sequential": n_jobs / t_seq,
                     "speedup": t_seq / t_bat, "patterns": int(sum(seq)), "results_identical": same,
-                    "batched_patterns": int(st["n_patterns"])}
+                    "batched_patterns": int(st["n_patterns"]), "tuned": tuned}
 
 
 def _serving_section(n_requests: int, rate: float, out: dict) -> None:

@@ -644,7 +644,7 @@ class Batch:
 
     def mine(self, minsups: Sequence[int], *, label_style: int = GM_LABELS_FULL, collect_proj: bool = False,
              canon_threads: int = 0, count_only: bool = False, path_pruning: bool = True, speculate: int = 0,
-             canon_host: int = 0, mem_budget: int = 0):
+             canon_host: int = 0, mem_budget: int = 0, chunk_rows: int = 0, table_bits: int = 0):
         lib = load_library()
         res: List[list] = [[] for _ in range(self.n_jobs)]
 
@@ -657,7 +657,7 @@ class Batch:
         ccb = JOB_PATTERN_CB() if count_only else JOB_PATTERN_CB(guard)
         ms = _i32(minsups)
         cfg = _make_cfg(label_style, 0, canon_threads, collect_proj, path_pruning=path_pruning, speculate=speculate,
-                        canon_host=canon_host, mem_budget=mem_budget)
+                        canon_host=canon_host, mem_budget=mem_budget, chunk_rows=chunk_rows, table_bits=table_bits)
         st = gm_stats()
         rc = lib.gm_batch_mine(self._h, _ptr(ms), C.byref(cfg), ccb, None, C.byref(st))
         guard.raise_pending()

@@ -27,6 +27,7 @@
 #define GSPAN_AMD_HPP
 
 #include <cstdint>
+#include <cstdlib>
 #include <exception>
 #include <fstream>
 #include <map>
@@ -152,13 +153,46 @@ class graph_database {
 // Launch parameters by name, as cudapp::cuda_configurator holds the
 // reference's per-kernel block sizes.  The names that mean something here
 // are the gm_cfg fields (label_style, max_edges, canon_threads,
-// path_pruning, speculate, chunk_rows, table_bits, collect_proj); the
+// path_pruning, speculate, chunk_rows, table_bits, canon_host,
+// collect_proj); the
 // reference's own kernel names are accepted and have no effect (their
 // kernels do not exist here).
 class cuda_configurator {
   std::map<std::string, int> p_;
+  std::string file_, id_;
 
  public:
+  cuda_configurator() = default;
+  // a configuration file as the reference's reads it
+  // (src/cuda_pp/cuda_configurator.cpp:26-76): header `id,<name>,...`, then
+  // one row per configuration id; parse_config_file() takes row `config_id`
+  // (gpu_graph_mining_amd/tune.py writes such files)
+  cuda_configurator(const std::string &config_file, const std::string &config_id)
+      : file_(config_file), id_(config_id) {}
+  void parse_config_file() {
+    std::ifstream is(file_.c_str());
+    if (!is) throw std::runtime_error("cuda_configurator: could not open config file " + file_);
+    auto split = [](const std::string &line) {
+      std::vector<std::string> out;
+      std::stringstream ss(line);
+      std::string cell;
+      while (std::getline(ss, cell, ',')) {
+        const size_t a = cell.find_first_not_of(" \t\r"), b = cell.find_last_not_of(" \t\r");
+        out.push_back(a == std::string::npos ? std::string() : cell.substr(a, b - a + 1));
+      }
+      return out;
+    };
+    std::string line;
+    if (!std::getline(is, line)) throw std::runtime_error("cuda_configurator: empty config file " + file_);
+    const std::vector<std::string> header = split(line);
+    while (std::getline(is, line)) {
+      const std::vector<std::string> row = split(line);
+      if (row.empty() || row[0] != id_) continue;
+      for (size_t i = 1; i < row.size() && i < header.size(); ++i) p_[header[i]] = std::atoi(row[i].c_str());
+      return;
+    }
+    throw std::runtime_error("cuda_configurator: no configuration " + id_ + " in " + file_);
+  }
   void set_parameter(const std::string &name, int value) { p_[name] = value; }
   int get_parameter(const std::string &name) const {
     auto it = p_.find(name);
@@ -178,6 +212,7 @@ class cuda_configurator {
     get("speculate", cfg.speculate);
     get("chunk_rows", cfg.chunk_rows);
     get("table_bits", cfg.table_bits);
+    get("canon_host", cfg.canon_host);
     get("collect_proj", cfg.collect_proj);
   }
 };

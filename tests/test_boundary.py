@@ -130,3 +130,38 @@ def test_cli_usage_and_exit_codes():
     assert r.returncode == 2 and "greater then the database size" in r.stderr
     r = _cli("-txt", chem, 0)
     assert r.returncode == 3 and "absolute_min_support < 1" in r.stderr
+
+
+def test_tuned_configuration_file_reads_alike_in_python_and_cpp(tmp_path):
+    """The autotuner's configuration file (the layout of the reference's
+    cuda_configurator: header `id,<name>,...`, one row per id) reads back the
+    same through tune.read_config and gspan_amd::cuda_configurator."""
+    from gpu_graph_mining_amd import tune
+    path = tmp_path / "tuned.csv"
+    tune.write_config(path, "config2", {"chunk_rows": 128, "table_bits": 9, "canon_host": 1})
+    tune.write_config(path, "config3", {"chunk_rows": 64, "table_bits": 8, "canon_host": 2})
+    tune.write_config(path, "config2", {"chunk_rows": 256, "table_bits": 8, "canon_host": 0})  # replaced
+    assert tune.read_config(path, "config2") == {"chunk_rows": 256, "table_bits": 8, "canon_host": 0}
+    assert tune.read_config(path, "config3") == {"chunk_rows": 64, "table_bits": 8, "canon_host": 2}
+    src = tmp_path / "cfg.cpp"
+    src.write_text('''#include <cstdio>
+#include "gspan_amd.hpp"
+int main(int argc, char **argv) {
+  gspan_amd::cuda_configurator cc(argv[1], argv[2]);
+  cc.parse_config_file();
+  gm_cfg cfg;
+  cc.fill(cfg);
+  std::printf("%d %d %d\\n", cfg.chunk_rows, cfg.table_bits, cfg.canon_host);
+  try {
+    gspan_amd::cuda_configurator(argv[1], "nosuch").parse_config_file();
+  } catch (const std::runtime_error &) {
+    std::printf("missing\\n");
+  }
+  return 0;
+}
+''')
+    exe = tmp_path / "cfg"
+    subprocess.run(["g++", "-std=c++17", f"-I{ROOT / 'include'}", str(src), f"-L{gbuild.LIB_DIR}", "-lgspan_amd",
+                    f"-Wl,-rpath,{gbuild.LIB_DIR}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe), str(path), "config3"], check=True, capture_output=True, text=True).stdout.split()
+    assert out == ["64", "8", "2", "missing"]
