@@ -443,23 +443,54 @@ class graph_counting_output(graph_output):
         return self.counter
 
 
+def dfs_order_key(code) -> tuple:
+    """Sort key of the sequential gSpan's report order (depth first; the
+    children of a pattern in the order of its extension maps,
+    src/gspan/gspan.cpp:251-271: backward tuples by `to` then edge label,
+    before forward tuples by `from` descending, then edge label, then
+    to-label; roots by (fromlabel, elabel, tolabel)).  A pattern precedes
+    its extensions, so sorting by this key gives the pre-order."""
+    key = []
+    for i, d in enumerate(code):
+        if i == 0:
+            key.append((d[2], d[3], d[4]))
+        elif d[0] > d[1]:
+            key.append((0, d[1], d[3], 0))
+        else:
+            key.append((1, -d[0], d[3], d[4]))
+    return tuple(key)
+
+
 class graph_file_output(graph_output):
     """One ``code.to_string(false)`` line per pattern
-    (src/globals/graph_output.cpp:145-150)."""
+    (src/globals/graph_output.cpp:90-95), as they come or (dfs_order=True)
+    all at close() in the sequential gSpan's report order, which with
+    wildcard labels makes the file byte-identical to that program's."""
 
-    def __init__(self, fname: os.PathLike):
+    def __init__(self, fname: os.PathLike, dfs_order: bool = False):
         self.filename = str(fname)
         self._f = open(self.filename, "w")
         self.count = 0.0
+        self.dfs_order = dfs_order
+        self._held: list = []
 
     def output_graph(self, code, support=0):
-        self._f.write(code.to_string(False) + "\n")
+        if self.dfs_order:
+            self._held.append(code)
+        else:
+            self._f.write(code.to_string(False) + "\n")
         self.count += 1.0
 
     def get_count(self):
         return self.count
 
     def close(self):
+        if self._f.closed:
+            return
+        if self.dfs_order:
+            for c in sorted(self._held, key=dfs_order_key):
+                self._f.write(DFSCode(c).to_string(False) + "\n")
+            self._held = []
         self._f.close()
 
     def to_string(self):
@@ -727,6 +758,7 @@ class gspan_cuda:
 
 __all__ = [
     "GraphDB", "Batch", "DFS", "DFSCode", "graph_output", "graph_counting_output", "graph_file_output",
+    "dfs_order_key",
     "graph_collecting_output", "gspan_cuda", "mine", "mine_batch", "load_library", "device_count", "version",
     "GMError", "codes_are_min", "GM_LABELS_FULL", "GM_LABELS_WILDCARD", "EXPORTED_SYMBOLS",
 ]

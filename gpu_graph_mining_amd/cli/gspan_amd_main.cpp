@@ -14,7 +14,8 @@
 // absolute_min_support, output, total_time (seconds, fixed),
 // total_frequent_graphs.  The optional fourth argument writes one
 // to_string(false) line per pattern, as the sequential gSpan's CLI does
-// (src/gspan/main.cpp:23-33, 64-67).
+// (src/gspan/main.cpp:23-33, 64-67), in that program's report order
+// (fully labelled, as gspan_cuda reports codes).
 #include <sys/time.h>
 #include <unistd.h>
 
@@ -70,10 +71,13 @@ int main(int argc, char **argv) {
   }
 
   std::unique_ptr<gspan_amd::graph_output> out;
-  if (argc == 5)
-    out.reset(new gspan_amd::graph_file_output(argv[4]));
-  else
+  gspan_amd::graph_file_output *file_out = nullptr;
+  if (argc == 5) {
+    file_out = new gspan_amd::graph_file_output(argv[4], true);
+    out.reset(file_out);
+  } else {
     out.reset(new gspan_amd::graph_counting_output());
+  }
 
   // the reference's launch parameters are accepted (common_main.cpp:75-84);
   // none of its kernels exist here, so they change nothing
@@ -109,6 +113,7 @@ int main(int argc, char **argv) {
 
     gspan->delete_database_from_device();
     delete gspan;
+    if (file_out) file_out->close();  // the patterns, in order
   } catch (const std::exception &e) {
     std::cerr << "caught exception: " << e.what() << std::endl;
     rc = 4;

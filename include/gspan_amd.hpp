@@ -26,6 +26,7 @@
 #ifndef GSPAN_AMD_HPP
 #define GSPAN_AMD_HPP
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <exception>
@@ -92,20 +93,76 @@ class graph_counting_output : public graph_output {
   }
 };
 
-// One code.to_string(false) line per pattern (graph_output.cpp:90-95).
+// The order in which the sequential gSpan reports patterns: depth first,
+// the children of a pattern in the order of its extension maps
+// (src/gspan/gspan.cpp:251-271): backward tuples by `to` then edge label
+// ascending, before forward tuples by `from` descending, then edge label,
+// then to-label ascending; roots by (fromlabel, elabel, tolabel).  A
+// pattern precedes its extensions, so pre-order is this lexicographic order
+// of the codes.
+inline bool dfs_less(const DFSCode &a, const DFSCode &b) {
+  const size_t n = a.size() < b.size() ? a.size() : b.size();
+  for (size_t i = 0; i < n; ++i) {
+    const DFS &x = a[i], &y = b[i];
+    if (x.from == y.from && x.to == y.to && x.fromlabel == y.fromlabel && x.elabel == y.elabel &&
+        x.tolabel == y.tolabel)
+      continue;
+    if (i == 0) {
+      if (x.fromlabel != y.fromlabel) return x.fromlabel < y.fromlabel;
+      if (x.elabel != y.elabel) return x.elabel < y.elabel;
+      return x.tolabel < y.tolabel;
+    }
+    const bool xb = x.from > x.to, yb = y.from > y.to;
+    if (xb != yb) return xb;  // backward first
+    if (xb) return x.to != y.to ? x.to < y.to : x.elabel < y.elabel;
+    if (x.from != y.from) return x.from > y.from;
+    if (x.elabel != y.elabel) return x.elabel < y.elabel;
+    return x.tolabel < y.tolabel;
+  }
+  return a.size() < b.size();
+}
+
+// One code.to_string(false) line per pattern (graph_output.cpp:90-95), as
+// they come, or (dfs_order) all of them at close() / destruction in the
+// sequential gSpan's order (dfs_less), which makes the file with wildcard
+// labels byte-identical to that program's output file.
 class graph_file_output : public graph_output {
   std::ofstream out;
   std::string filename;
   double count = 0;
+  bool dfs_order;
+  std::vector<DFSCode> held;
 
  public:
-  explicit graph_file_output(const std::string &fname) : out(fname.c_str()), filename(fname) {
+  explicit graph_file_output(const std::string &fname, bool dfs_order_ = false)
+      : out(fname.c_str()), filename(fname), dfs_order(dfs_order_) {
     if (!out) throw std::runtime_error("cannot open " + fname);
   }
+  ~graph_file_output() override {
+    try {
+      close();
+    } catch (...) {
+    }
+  }
   void output_graph(const DFSCode &code, size_t = 0) override {
-    out << code.to_string(false) << "\n";
-    if (!out) throw std::runtime_error("cannot write " + filename);
+    if (dfs_order) {
+      held.push_back(code);
+    } else {
+      out << code.to_string(false) << "\n";
+      if (!out) throw std::runtime_error("cannot write " + filename);
+    }
     count += 1.0;
+  }
+  // write what is held and close the file
+  void close() {
+    if (!out.is_open()) return;
+    if (dfs_order) {
+      std::stable_sort(held.begin(), held.end(), dfs_less);
+      for (const DFSCode &c : held) out << c.to_string(false) << "\n";
+      held.clear();
+    }
+    out.close();
+    if (out.fail()) throw std::runtime_error("cannot write " + filename);
   }
   double get_count() override { return count; }
   std::string to_string() override { return "graph_file_output, filename: " + filename; }

@@ -165,3 +165,44 @@ int main(int argc, char **argv) {
                     f"-Wl,-rpath,{gbuild.LIB_DIR}", "-o", str(exe)], check=True)
     out = subprocess.run([str(exe), str(path), "config3"], check=True, capture_output=True, text=True).stdout.split()
     assert out == ["64", "8", "2", "missing"]
+
+
+def test_dfs_report_order_matches_the_sequential_gspan(tmp_path):
+    """The sort key (Python) and comparator (C++) of the sequential gSpan's
+    report order reproduce the oracle's emission order -- the order of
+    src/gspan's output file -- from a shuffled list of its patterns."""
+    import random
+    from oracle import py_oracle as po
+    db = gm.GraphDB.load(GOLDEN / "Chemical_340.txt", "txt")
+    pats, _ = po.mine_arrays(db.arrays(), 34)
+    codes = [tuple(map(tuple, c)) for c, _, _ in pats]
+    shuffled = codes[:]
+    random.Random(3).shuffle(shuffled)
+    assert sorted(shuffled, key=gm.dfs_order_key) == codes
+    # the same with the C++ comparator
+    src = tmp_path / "order.cpp"
+    src.write_text('''#include <cstdio>
+#include <iostream>
+#include "gspan_amd.hpp"
+int main() {
+  std::vector<gspan_amd::DFSCode> codes;
+  int n;
+  while (std::cin >> n) {
+    gspan_amd::DFSCode c;
+    for (int i = 0; i < n; ++i) {
+      gspan_amd::DFS d;
+      std::cin >> d.from >> d.to >> d.fromlabel >> d.elabel >> d.tolabel;
+      c.push_back(d);
+    }
+    codes.push_back(c);
+  }
+  std::stable_sort(codes.begin(), codes.end(), gspan_amd::dfs_less);
+  for (const auto &c : codes) std::printf("%s\\n", c.to_string(false).c_str());
+  return 0;
+}
+''')
+    exe = tmp_path / "order"
+    subprocess.run(["g++", "-std=c++17", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    inp = "\n".join(f"{len(c)} " + " ".join(" ".join(map(str, t)) for t in c) for c in shuffled) + "\n"
+    out = subprocess.run([str(exe)], input=inp, check=True, capture_output=True, text=True).stdout.splitlines()
+    assert out == [gm.DFSCode(c).to_string(False) for c in codes]
