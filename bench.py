@@ -140,6 +140,24 @@ def _batch_section(n_jobs: int, out: dict) -> None:
                     "batched_patterns": int(st["n_patterns"]), "tuned": tuned}
 
 
+def _batched_only(n_jobs: int, runs: int, out: dict) -> None:
+    """The 256-job mix as one batched schedule only (with the autotuned
+    launch shapes), `runs` times: the workload of the batch profile."""
+    import gpu_graph_mining_amd as gm
+    from gpu_graph_mining_amd import workloads as wl
+    jobs = wl.config3_jobs()[:n_jobs]
+    batch = gm.Batch([j.database() for j in jobs])
+    mins = [j.minsup for j in jobs]
+    tuned = _tuned("config3")
+    batch.mine(mins, count_only=True, **tuned)  # warm
+    t0 = time.perf_counter()
+    for _ in range(runs):
+        _, st = batch.mine(mins, count_only=True, **tuned)
+    dt = (time.perf_counter() - t0) / max(1, runs)
+    out["batched_only"] = {"jobs": n_jobs, "runs": runs, "seconds_per_run": dt, "jobs_per_s": n_jobs / dt,
+                           "patterns": int(st["n_patterns"]), "tuned": tuned}
+
+
 def _serving_section(n_requests: int, rate: float, out: dict) -> None:
     """Config 4: Poisson arrivals of 90 % pattern-support queries (3-12
     edges, random-walk subgraphs) and 10 % mining jobs (1k-10k graph
@@ -314,7 +332,8 @@ def main() -> int:
                          "atomic8) over --calibrate-mib MiB each, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--calibrate-mib", type=int, default=256)
     ap.add_argument("--sections", default="main,batch,serving",
-                    help="comma-separated parts to run: main (the headline workload), batch, serving")
+                    help="comma-separated parts to run: main (the headline workload), batch, serving, "
+                         "batchonly (the batched schedule alone, --steps times)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single GPU; "
@@ -334,6 +353,8 @@ def main() -> int:
                 "sections": sorted(sections)}
     if args.jobs and "batch" in sections:
         _batch_section(args.jobs, line)
+    if args.jobs and "batchonly" in sections:
+        _batched_only(args.jobs, args.steps, line)
     if args.serve_requests and "serving" in sections:
         _serving_section(args.serve_requests, args.serve_rate, line)
     if args.calibrate:
