@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(64) k_canon_level(CanonArgs c) {
     const unsigned long long key = f[0];
     const int64_t p = (int64_t)(key / (unsigned long long)c.K) + c.p_base;
     const int64_t x = (int64_t)(key % (unsigned long long)c.K);
-    const int nvp = c.meta[p].nv;
+    const int nvp = meta_at(c.meta, c.S, p)->nv;
     const int len = c.len;
     __syncthreads();  // the previous key's LDS is no longer read
     for (int t = lane; t < len && t < kCE; t += 64) {
@@ -362,7 +362,7 @@ __global__ void __launch_bounds__(64) k_canon_level(CanonArgs c) {
       L.wel[t] = (int32_t)((w >> 16) & 0xFF);
       L.wtl[t] = (int32_t)(w >> 24);
     }
-    for (int v = lane; v < nvp && v < kCV; v += 64) L.lab[v] = c.pvlab[p * kMaxPatV + v];
+    for (int v = lane; v < nvp && v < kCV; v += 64) L.lab[v] = c.pvlab[p * c.S + v];
     int nv = nvp;
     if (lane == 0 && len < kCE) {
       if (x < (int64_t)c.S * c.NEL) {  // backward: rightmost -> to
@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(64) k_canon_level(CanonArgs c) {
         L.wfrom[len] = (uint8_t)(nvp - 1);
         L.wto[len] = (uint8_t)to;
         L.wel[len] = (int32_t)(x % c.NEL);
-        L.wtl[len] = c.pvlab[p * kMaxPatV + to];
+        L.wtl[len] = c.pvlab[p * c.S + to];
       } else {  // forward: from -> new vertex nvp
         const int64_t y = x - (int64_t)c.S * c.NEL;
         L.wfrom[len] = (uint8_t)(y / c.NVL / c.NEL);

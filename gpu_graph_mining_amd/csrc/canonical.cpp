@@ -20,7 +20,7 @@
 //      minimal"; reaching the end answers "minimal".
 // Design: an embedding is a vertex map (min-code vertex -> pattern vertex)
 // plus a bitmask of used pattern edges, both in flat arrays; no per-step
-// maps or history rebuilds.  Patterns are capped at 64 edges / 64 vertices.
+// maps or history rebuilds.  Patterns are capped at 128 edges / 64 vertices.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -32,7 +32,7 @@ namespace gm {
 namespace {
 
 constexpr int kMaxV = 64;
-constexpr int kMaxE = 64;
+constexpr int kMaxE = 128;
 
 struct PGraph {
   int nv = 0;
@@ -46,8 +46,10 @@ struct PGraph {
 
 struct Emb {
   int8_t map[kMaxV];  // min-code vertex -> pattern vertex
-  uint64_t used;      // pattern edge ids already matched
-  uint64_t vused;     // pattern vertices already matched
+  uint64_t used[kMaxE / 64];  // pattern edge ids already matched
+  uint64_t vused;             // pattern vertices already matched
+  bool has(int eid) const { return used[eid >> 6] >> (eid & 63) & 1; }
+  void add(int eid) { used[eid >> 6] |= 1ull << (eid & 63); }
 };
 
 struct MinCode {
@@ -78,7 +80,7 @@ inline bool same_tuple(const DFS &a, const DFS &b) {
 // src/globals/dfs_code.cpp:245-265): vertex labels from the tuples that
 // carry them (-1 carries none), edges in code order.
 void code_graph(const DFS *code, int n, PGraph &G) {
-  if (n > kMaxE) throw Error(GM_E_UNSUPPORTED, "pattern larger than 64 edges");
+  if (n > kMaxE) throw Error(GM_E_UNSUPPORTED, "pattern larger than 128 edges");
   G.nv = 0;
   for (int i = 0; i < n; ++i) G.nv = std::max(G.nv, std::max(code[i].from, code[i].to) + 1);
   if (G.nv > kMaxV) throw Error(GM_E_UNSUPPORTED, "pattern larger than 64 vertices");
@@ -132,7 +134,8 @@ bool grow_min(const PGraph &G, int n, const DFS *want_code, DFS *out) {
         Emb e;
         e.map[0] = (int8_t)u;
         e.map[1] = (int8_t)v;
-        e.used = 1ull << G.eid[u][k];
+        std::fill(e.used, e.used + kMaxE / 64, 0ull);
+        e.add(G.eid[u][k]);
         e.vused = (1ull << u) | (1ull << v);
         cur.push_back(e);
       }
@@ -157,7 +160,7 @@ bool grow_min(const PGraph &G, int n, const DFS *want_code, DFS *out) {
         int r = e.map[e2.to], target = e.map[e1.from];
         int e1_tolabel = G.label[e.map[e1.to]], e2_tolabel = G.label[r];
         for (int k = 0; k < G.deg[r]; ++k) {
-          if (e.used >> G.eid[r][k] & 1) continue;
+          if (e.has(G.eid[r][k])) continue;
           int lab = G.el[r][k];
           if (G.nbr[r][k] == target && (e1.elabel < lab || (e1.elabel == lab && e1_tolabel <= e2_tolabel))) {
             best = std::min(best, lab);
@@ -173,12 +176,12 @@ bool grow_min(const PGraph &G, int n, const DFS *want_code, DFS *out) {
           int r = e.map[e2.to], target = e.map[e1.from];
           int e1_tolabel = G.label[e.map[e1.to]], e2_tolabel = G.label[r];
           for (int k = 0; k < G.deg[r]; ++k) {
-            if (e.used >> G.eid[r][k] & 1) continue;
+            if (e.has(G.eid[r][k])) continue;
             int lab = G.el[r][k];
             if (G.nbr[r][k] == target && (e1.elabel < lab || (e1.elabel == lab && e1_tolabel <= e2_tolabel))) {
               if (lab == best) {
                 Emb x = e;
-                x.used |= 1ull << G.eid[r][k];
+                x.add(G.eid[r][k]);
                 nxt.push_back(x);
               }
               break;
@@ -255,7 +258,7 @@ bool grow_min(const PGraph &G, int n, const DFS *want_code, DFS *out) {
         if (lab == bel2 && tl == btl2) {
           Emb y = e;
           y.map[maxtoc + 1] = (int8_t)w;
-          y.used |= 1ull << G.eid[u][k];
+          y.add(G.eid[u][k]);
           y.vused |= 1ull << w;
           nxt.push_back(y);
         }

@@ -359,7 +359,7 @@ int gm_support_queries(gm_db *db, int32_t n_queries, const int32_t *dfs5, const 
       gm::min_dfs_code(code.data(), n, mins[q]);
       int32_t nv = 0;
       for (const gm::DFS &t : mins[q]) nv = std::max(nv, std::max(t.from, t.to) + 1);
-      if (nv > gm::kMaxPatV) throw gm::Error(GM_E_UNSUPPORTED, "query pattern with more than 32 vertices");
+      if (nv > gm::kMaxPatV) throw gm::Error(GM_E_UNSUPPORTED, "query pattern with more than 64 vertices");
     }
     ensure_device(db);
     const gm::DeviceDB &d = *db->dev;

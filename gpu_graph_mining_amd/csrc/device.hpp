@@ -28,33 +28,55 @@
 // counterpart of the reference's column store (embedding_lists.hpp:12-85)
 // with whole tuples instead of back-linked columns.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace gm {
 
-constexpr int kMaxPatV = 32;  // vertices per pattern (tuple stride <= 32 bytes)
+constexpr int kMaxPatV = 64;     // vertices per pattern
+constexpr int kNarrowPatV = 32;  // up to here tuple strides are multiples of 4 bytes; beyond, 64
 constexpr int kMaxGraphV = 256;
-constexpr uint32_t kPatDead = 1;  // PatMeta::flags
+constexpr uint32_t kPatDead = 1;  // PatMetaT::flags
+
+// Tuple stride in bytes of a level whose largest pattern has nv vertices.
+__host__ __device__ constexpr int tuple_stride(int nv) { return nv <= kNarrowPatV ? (nv + 3) & ~3 : kMaxPatV; }
 
 // Per-pattern facts the extension rules need (src/globals/graph_types.cpp:
 // 305-413 evaluated on the pattern side).  Rightmost-path edge i (i = 0 is
 // the last forward edge, as DFSCode::buildRMPath orders them) is packed in
 // rm[i]: bits 0-7 its from vertex, 8-15 its edge label, 16-23 the label of
-// its to vertex.
-struct alignas(16) PatMeta {
+// its to vertex.  Two layouts with the same first 16 bytes: V = 32 for the
+// levels whose tuple stride is at most 32 bytes (every level of the common
+// workloads, 192 bytes a pattern) and V = 64 for the levels of stride 64
+// (patterns of 33-64 vertices, 352 bytes).  The vertex masks are split in
+// two 32-bit halves; the high halves are zero in the narrow layout.
+template <int V>
+struct alignas(16) PatMetaT {
   int32_t job;
   uint8_t nv;        // vertex count; the rightmost vertex is nv-1
   uint8_t nrm;       // rightmost-path edges
   uint8_t minlabel;  // dense label of vertex 0
   uint8_t maxlabel;  // dense label of the rightmost vertex
-  uint32_t bwd_mask; // bit t: the pattern already has edge (rightmost, t)
+  uint32_t bwd_mask; // bit t < 32: the pattern already has edge (rightmost, t)
   uint32_t flags;    // kPatDead: the code is not minimal (its rows are not extended)
-  uint32_t rm[kMaxPatV];
-  uint32_t bwd_from;     // bit f: f is the from vertex of rightmost-path edge q >= 1 (a backward target)
-  uint8_t rmq[kMaxPatV]; // that q, by vertex
-  uint32_t pad[3];
+  uint32_t rm[V];
+  uint32_t bwd_from;     // bit f < 32: f is the from vertex of rightmost-path edge q >= 1 (a backward target)
+  uint8_t rmq[V];        // that q, by vertex
+  uint32_t bwd_mask_hi;  // bits 32-63 of the two masks
+  uint32_t bwd_from_hi;
+  uint32_t pad;
 };
-static_assert(sizeof(PatMeta) == 192, "PatMeta layout");
+using PatMeta = PatMetaT<kNarrowPatV>;
+using PatMetaWide = PatMetaT<kMaxPatV>;
+static_assert(sizeof(PatMeta) == 192 && sizeof(PatMetaWide) == 352, "PatMeta layouts");
+// bytes per pattern of the metadata of a level of tuple stride S
+__host__ __device__ constexpr int meta_bytes(int S) { return S > kNarrowPatV ? (int)sizeof(PatMetaWide) : (int)sizeof(PatMeta); }
+// The first 16 bytes (job, nv, nrm, labels, low mask half, flags) of
+// pattern p's metadata in a level of tuple stride S, either layout.
+__host__ __device__ inline const PatMeta *meta_at(const PatMeta *m, int S, int64_t p) {
+  return reinterpret_cast<const PatMeta *>(reinterpret_cast<const char *>(m) + p * meta_bytes(S));
+}
 
 // Extension key of one pattern at a level whose tuple stride is S:
 //   backward to vertex t with edge label e   : t*NEL + e

@@ -62,6 +62,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "device.hpp"
 #include "kernels.hpp"
@@ -127,7 +128,7 @@ __device__ __forceinline__ void stamp_end(const StepArgs &a, int64_t ci) {
 
 // Candidate list entries: {code, child item count}.
 //   root codes   key | w << 24          (key < 255^3 < 2^24)
-//   later codes  key | (w + 1) << 23    (key < 32 * 255 * 256 < 2^23,
+//   later codes  key | (w + 1) << 23    (key < 64 * 255 * 256 < 2^23,
 //                                        w = -1 for backward edges)
 __device__ __forceinline__ uint32_t cand_code(bool root, int key, int w) {
   return root ? ((uint32_t)key | ((uint32_t)w << 24)) : ((uint32_t)key | ((uint32_t)(w + 1) << 23));
@@ -324,7 +325,7 @@ __device__ __forceinline__ void count_insert(const CountTable &t, uint32_t gk, i
 // applies the extension rules, appends each candidate to the row's list
 // region and counts it in the wave's table.
 template <bool ROOT, int TW>
-__global__ void __launch_bounds__(64, (TW <= 2 ? 6 : 5)) k_level(StepArgs a) {  // waves per SIMD the registers allow
+__global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(StepArgs a) {  // waves per SIMD the registers allow
   // 16-byte aligned whatever static LDS precedes it: the table's segment
   // masks take 64-bit LDS atomics
   extern __shared__ __attribute__((aligned(16))) uint32_t dyn[];
@@ -408,22 +409,27 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : 5)) k_level(StepArgs a) {  
         }
       }
     } else {
+      // the level's metadata layout follows its tuple stride (S = 4 * TW)
+      constexpr bool kWide = 4 * TW > kNarrowPatV;
+      using Meta = PatMetaT<kWide ? kMaxPatV : kNarrowPatV>;
+      using VMask = typename std::conditional<kWide, uint64_t, uint32_t>::type;
+      const Meta *metas = reinterpret_cast<const Meta *>(a.meta);
       int32_t p = 0, g = 0;
       uint4 hd = make_uint4(0, 0, 0, 0);
       if (live) {
         p = a.pid[r];
         g = a.rgraph[r];
-        hd = *reinterpret_cast<const uint4 *>(a.meta + p);
+        hd = *reinterpret_cast<const uint4 *>(metas + p);
       }
       const bool work = live && !(hd.w & kPatDead);  // a dead pattern's rows list nothing
       const int nv = (int)(hd.y & 0xFF), nrm = (int)((hd.y >> 8) & 0xFF);
       const int minlabel = (int)((hd.y >> 16) & 0xFF), maxlabel = (int)(hd.y >> 24);
-      const uint32_t bwd_mask = hd.z;
+      VMask bwd_mask = hd.z;
       Tuple<TW> t;
       int32_t vb = 0, ca = 0, ce = 0, na = 0, ne = 0;
       uint32_t nrme = 0, rme = 0;
-      uint32_t bwd_from = 0;
-      const PatMeta *pm = a.meta + p;
+      VMask bwd_from = 0;
+      const Meta *pm = metas + p;
       const unsigned long long *mkp = ms ? a.pmask + (int64_t)p * 2 * ms : nullptr;
       unsigned long long mk = ~0ull;
       const int n = (int)rb;
@@ -431,6 +437,10 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : 5)) k_level(StepArgs a) {  
         t.load(a.vt, r, a.S, nv);
         vb = a.gvbase[g];
         bwd_from = pm->bwd_from;
+        if constexpr (kWide) {
+          bwd_mask |= (VMask)pm->bwd_mask_hi << 32;
+          bwd_from |= (VMask)pm->bwd_from_hi << 32;
+        }
         const int rv = t.get(nv - 1);
         ca = a.vadj[vb + rv];
         ce = a.vadj[vb + rv + 1];
@@ -462,12 +472,12 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : 5)) k_level(StepArgs a) {  
               bound = (uint32_t)n + dw;
               keep = (mk >> (el * NVL + nl)) & 1ull;
             }
-          } else if ((bwd_from >> fw) & 1u) {
+          } else if ((bwd_from >> fw) & 1) {
             // backward to a rightmost-path vertex f = rm[q].from, q >= 1
             const int q = pm->rmq[fw];
             const uint32_t rq = pm->rm[q];
             const int e1 = (rq >> 8) & 0xFF, l1 = (rq >> 16) & 0xFF;
-            if (!((bwd_mask >> fw) & 1u) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
+            if (!((bwd_mask >> fw) & 1) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
               key = fw * NEL + el;
               wout = -1;
               bound = (uint32_t)n;
@@ -709,7 +719,7 @@ __global__ void __launch_bounds__(256) k_reduce(StepArgs a) {
     bool hit = false;
     if (s > 0) {
       const int64_t p = i / a.K + a.p_base;
-      const int32_t owner = ROOT ? (int32_t)p : a.meta[p].job;
+      const int32_t owner = ROOT ? (int32_t)p : meta_at(a.meta, a.S, p)->job;
       hit = s >= a.minsup_job[owner];
     }
     if (v && !hit)
@@ -940,7 +950,7 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
           p = a.pid[r];
           g = a.rgraph[r];
           if (n) {
-            nvp = a.meta[p].nv;
+            nvp = meta_at(a.meta, a.S, p)->nv;
             const uint32_t *src = reinterpret_cast<const uint32_t *>(a.vt + r * a.S);
 #pragma unroll
             for (int q = 0; q < TW; ++q) s_tup[lane * TW + q] = src[q];
@@ -1115,6 +1125,7 @@ hipError_t level_tw(const StepArgs &a, hipStream_t s, size_t lds) {
     case 6: hipLaunchKernelGGL((k_level<false, 6>), dim3(grid), dim3(64), lds, s, a); break;
     case 7: hipLaunchKernelGGL((k_level<false, 7>), dim3(grid), dim3(64), lds, s, a); break;
     case 8: hipLaunchKernelGGL((k_level<false, 8>), dim3(grid), dim3(64), lds, s, a); break;
+    case 16: hipLaunchKernelGGL((k_level<false, 16>), dim3(grid), dim3(64), lds, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1170,6 +1181,7 @@ hipError_t launch_write(const StepArgs &a, bool root, hipStream_t s) {
     case 6: hipLaunchKernelGGL((k_write<false, 6>), dim3(grid), dim3(64), 0, s, a); break;
     case 7: hipLaunchKernelGGL((k_write<false, 7>), dim3(grid), dim3(64), 0, s, a); break;
     case 8: hipLaunchKernelGGL((k_write<false, 8>), dim3(grid), dim3(64), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_write<false, 16>), dim3(grid), dim3(64), 0, s, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

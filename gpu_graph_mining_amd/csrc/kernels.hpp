@@ -18,7 +18,6 @@ constexpr int64_t kLevelWaves = 16384;  // waves a level step aims for
 // Counter keys of one level piece at most (16 bytes per key and replica): a
 // level with more patterns x keys per pattern is mined in pieces.
 constexpr int64_t kPieceKeys = 1ll << 26;
-constexpr int kMaxTupleWords = kMaxPatV / 4;  // 8
 
 // Statistics counters that every wave bumps are spread over kSpread cache
 // lines: atomics on one address serialise across the device.
@@ -64,7 +63,7 @@ struct StepArgs {
   const uint32_t *rbound;
   const unsigned long long *rlist;
   int32_t S;  // tuple stride in bytes
-  const PatMeta *meta;
+  const PatMeta *meta;  // meta_bytes(S) bytes per pattern (the wide layout when S = 64)
   // path pruning masks (2 * mask_stride words per pattern; 0: none):
   // fwd[0] new arms at the rightmost vertex, fwd[1 + i] at rm[i].from
   // (bit elabel * NVL + label), bwd[1 + i] backward edges to rm[i].from (bit
@@ -174,7 +173,8 @@ struct CanonArgs {
   const uint32_t *freq_count;
   uint32_t freq_cap;
   // the level's patterns: nv from meta, len tuples each packed from | to << 8
-  // | elabel << 16 | tolabel << 24 (dense labels), kMaxPatV vertex labels each
+  // | elabel << 16 | tolabel << 24 (dense labels), S vertex labels each
+  // (S: the level's tuple stride; meta_bytes(S) bytes of metadata each)
   const PatMeta *meta;
   int32_t p_base;  // record keys are (pattern - p_base) * K + key
   const uint32_t *pcode;

@@ -412,8 +412,9 @@ struct Level {
   uint8_t *vl(int64_t i) { return vlab.data() + i * kMaxPatV; }
 };
 
-PatMeta make_meta(const DFS *code, int len, const uint8_t *vlab, int nv, int32_t job) {
-  PatMeta m;
+template <int V>
+PatMetaT<V> make_meta(const DFS *code, int len, const uint8_t *vlab, int nv, int32_t job) {
+  PatMetaT<V> m;
   std::memset(&m, 0, sizeof(m));
   m.job = job;
   m.nv = (uint8_t)nv;
@@ -429,14 +430,19 @@ PatMeta make_meta(const DFS *code, int len, const uint8_t *vlab, int nv, int32_t
     }
   }
   m.nrm = (uint8_t)nrm;
+  uint64_t bwd_mask = 0, bwd_from = 0;
   for (int i = 0; i < len; ++i)
-    if (code[i].from == nv - 1 && code[i].to < code[i].from) m.bwd_mask |= 1u << code[i].to;
+    if (code[i].from == nv - 1 && code[i].to < code[i].from) bwd_mask |= 1ull << code[i].to;
   // backward targets: the from vertices of rightmost-path edges q >= 1
   for (int q = 1; q < nrm; ++q) {
     const int f = (int)(m.rm[q] & 0xFF);
-    m.bwd_from |= 1u << f;
+    bwd_from |= 1ull << f;
     m.rmq[f] = (uint8_t)q;
   }
+  m.bwd_mask = (uint32_t)bwd_mask;
+  m.bwd_mask_hi = (uint32_t)(bwd_mask >> 32);
+  m.bwd_from = (uint32_t)bwd_from;
+  m.bwd_from_hi = (uint32_t)(bwd_from >> 32);
   return m;
 }
 
@@ -1032,7 +1038,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const uint64_t key_lo = Fr.key_next;
     uint64_t key_cut = ~0ull;  // keys [key_lo, key_cut) become children
     if (budget > 0 && !past_max && F > 0) {
-      const int CSb = (int)round4(root ? 2 : std::min(L.max_nv + 1, kMaxPatV));
+      const int CSb = tuple_stride(root ? 2 : std::min(L.max_nv + 1, kMaxPatV));
       auto kbytes = [&](int64_t i) { return rows_bytes((int64_t)fr[i].proj, CSb) + (int64_t)fr[i].items * 8; };
       int64_t total = 0;
       for (int64_t i = 0; i < F; ++i)
@@ -1197,7 +1203,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         kidx[i] = keep ? 0 : -1;
         kalive[i] = keep ? 1 : 0;
       });
-      if (too_big) throw Error(GM_E_UNSUPPORTED, "pattern with more than 32 vertices is frequent");
+      if (too_big) throw Error(GM_E_UNSUPPORTED, "pattern with more than 64 vertices is frequent");
     } else {
       for (int64_t i = 0; i < F; ++i) kidx[i] = in_piece(i) ? 0 : -1;  // decoded while the device writes
     }
@@ -1216,11 +1222,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       if (!late_decode) max_nv = std::max(max_nv, (int)knv[i]);
     }
     if (late_decode) max_nv = max_nv_bound;
+    const int CS = C > 0 ? tuple_stride(max_nv) : 4;  // the children's tuple stride
+    const bool wide = CS > kNarrowPatV;                 // and metadata layout
     N.resize(C, new_len);
     nproj.resize(C);
     double us_host = 0, us_overlap = 0;
     int64_t alive = 0, alive_rows = 0;  // minimal children and their embeddings
-    int CS = 4;
     int child_buf = 0, child_mask_stride = 0;  // the children's row set and upload layout
     size_t child_mask_off = 0, child_code_off = 0, child_vlab_off = 0;
     uint64_t child_items = 0;  // the children's list entries
@@ -1229,8 +1236,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // dropped), per child its first row and first list entry, then the
     // children's path masks.  Child c's rows follow the earlier children's
     // (embedding counts), its list entries likewise (item sums).
-    const size_t meta_bytes = sizeof(PatMeta) * C;
-    const size_t fk_off = (meta_bytes + 7) & ~size_t(7);
+    const size_t meta_all = (size_t)meta_bytes(CS) * C;
+    const size_t fk_off = (meta_all + 7) & ~size_t(7);
     const size_t fc_off = fk_off + sizeof(unsigned long long) * F;
     const size_t rb_off = (fc_off + sizeof(int32_t) * F + 7) & ~size_t(7);
     const size_t lb_off = rb_off + sizeof(unsigned long long) * C;
@@ -1246,7 +1253,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // minimum-code check on the device
     const size_t codes_off = masks_off + sizeof(uint64_t) * 2 * mstride * C;
     const size_t cvl_off = (codes_off + (want_codes ? sizeof(uint32_t) * new_len * C : 0) + 7) & ~size_t(7);
-    const size_t up_bytes = cvl_off + (want_codes ? (size_t)kMaxPatV * C : 0);
+    const size_t up_bytes = (cvl_off + (want_codes ? (size_t)CS * C : 0) + 7) & ~size_t(7);  // copied in 8-byte words
     char *hu = F > 0 ? static_cast<char *>(w.up.ensure(up_bytes)) : nullptr;
     if (F > 0) {
       unsigned long long *hfk = reinterpret_cast<unsigned long long *>(hu + fk_off);
@@ -1285,7 +1292,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       // starts; the children's metadata (the next level's) follows in one
       // upload.
       Timer tdev2;
-      CS = (int)round4(max_nv);
       // a row set no level on the stack holds (this one's is read by the write)
       auto held_by_stack = [&](int b) {
         for (const Frame &f : frames)
@@ -1296,7 +1302,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       while (held_by_stack(nxt)) ++nxt;
       if (w.arena && nxt > 1) throw Error(GM_E_DEVICE, "internal: row set beyond the static workspace's two");
       child_buf = nxt;
-      PatMeta *hm = reinterpret_cast<PatMeta *>(hu);
+      PatMeta *hm = reinterpret_cast<PatMeta *>(hu);  // narrow layout (else hmw)
+      PatMetaWide *hmw = reinterpret_cast<PatMetaWide *>(hu);
       char *du = static_cast<char *>(grow(w.row(nxt).upl, up_bytes, s));
       const char *su = static_cast<const char *>(w.up.dev);
       grow(w.row(nxt).pid, sizeof(int32_t) * std::max<int64_t>(child_rows, 1), s);
@@ -1359,14 +1366,19 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         N.node[c] = knode[i];
         N.alive[c] = kalive[i];
         nproj[c] = fr[i].proj;
-        hm[c] = make_meta(cc, new_len, cv, cnv, N.job[c]);
-        if (!kalive[i]) hm[c].flags |= kPatDead;
+        if (wide) {
+          hmw[c] = make_meta<kMaxPatV>(cc, new_len, cv, cnv, N.job[c]);
+          if (!kalive[i]) hmw[c].flags |= kPatDead;
+        } else {
+          hm[c] = make_meta<kNarrowPatV>(cc, new_len, cv, cnv, N.job[c]);
+          if (!kalive[i]) hm[c].flags |= kPatDead;
+        }
         if (want_codes) {
           uint32_t *pc = reinterpret_cast<uint32_t *>(hu + codes_off) + (size_t)c * new_len;
           for (int k = 0; k < new_len; ++k)
             pc[k] = (uint32_t)cc[k].from | (uint32_t)cc[k].to << 8 | (uint32_t)cc[k].elabel << 16 |
                     (uint32_t)cc[k].tolabel << 24;
-          std::memcpy(hu + cvl_off + (size_t)c * kMaxPatV, cv, kMaxPatV);
+          std::memcpy(hu + cvl_off + (size_t)c * CS, cv, CS);
         }
       };
       auto mask_child = [&](int64_t c) {
@@ -1374,22 +1386,24 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         uint64_t *fw = hmask + (size_t)c * 2 * mstride, *bw = fw + mstride;
         const DFS *cc = N.c(c);
         const uint8_t *cv = N.vl(c);
-        const PatMeta &m = hm[c];
+        // nv, nrm and the rightmost path's from vertices, either layout
+        const int m_nv = N.nv[c], m_nrm = wide ? hmw[c].nrm : hm[c].nrm;
+        const uint32_t *m_rm = wide ? hmw[c].rm : hm[c].rm;
         const uint64_t *tj = &t2[(size_t)N.job[c] * NVL * AR];
         // per vertex: the arms that form a frequent path with every
         // pattern edge at it (one pass over the edges)
         uint64_t vm[kMaxPatV];
-        for (int v = 0; v < m.nv; ++v) vm[v] = ~0ull;
+        for (int v = 0; v < m_nv; ++v) vm[v] = ~0ull;
         for (int k = 0; k < new_len; ++k) {
           const int u = cc[k].from, v = cc[k].to, e = cc[k].elabel * NVL;
           vm[u] &= tj[(size_t)cv[u] * AR + e + cv[v]];
           vm[v] &= tj[(size_t)cv[v] * AR + e + cv[u]];
         }
-        const int rv = m.nv - 1;
+        const int rv = m_nv - 1;
         fw[0] = vm[rv];
         bw[0] = 0;
-        for (int i = 0; i < m.nrm; ++i) {
-          const int t = m.rm[i] & 0xFF;
+        for (int i = 0; i < m_nrm; ++i) {
+          const int t = m_rm[i] & 0xFF;
           fw[1 + i] = vm[t];
           uint64_t b = 0;  // backward rv -> t: a new arm at both ends
           if (i > 0)

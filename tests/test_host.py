@@ -145,13 +145,13 @@ def test_config3_jobs_shape():
     assert jobs == wl.config3_jobs()
 
 
-def _rand_code(rng):
-    n = rng.randint(2, 7)
-    labels = [rng.randint(0, 2) for _ in range(n)]
+def _rand_code(rng, n_range=(2, 7), n_labels=3, extra=(0, 4)):
+    n = rng.randint(*n_range)
+    labels = [rng.randint(0, n_labels - 1) for _ in range(n)]
     edges = set()
     for v in range(1, n):
         edges.add((rng.randrange(v), v))
-    for _ in range(rng.randint(0, 4)):
+    for _ in range(rng.randint(*extra)):
         a, b = rng.sample(range(n), 2)
         edges.add((min(a, b), max(a, b)))
     el = {e: rng.randint(0, 1) for e in edges}
@@ -194,6 +194,57 @@ def test_canonical_check_matches_oracle_on_random_codes():
         assert got == want, c
         n_min += want
     assert 300 < n_min < 3700
+
+
+def _chorded_path_code(rng, n, n_chords, reverse=False):
+    """A path 0-1-...-(n-1) with vertex labels 0..n-1 and chords (i, j),
+    j > i + 1, all edge labels 0, walked along the path, each chord a
+    backward edge at its later end (earlier ends in DFS order).  From vertex
+    0 this is the minimum DFS code (the greedy growth takes the smallest new
+    label first); from vertex n-1 (reverse) it is not."""
+    chords = set()
+    while len(chords) < n_chords:
+        i = rng.randrange(n - 2)
+        chords.add((i, rng.randrange(i + 2, n)))
+    seq = list(range(n))[::-1] if reverse else list(range(n))
+    pos = {v: k for k, v in enumerate(seq)}
+    code = []
+    for k in range(1, n):
+        u, v = seq[k - 1], seq[k]
+        code.append((k - 1, k, u, 0, v))
+        back = sorted(pos[x] for (a, b) in chords for x, y in ((a, b), (b, a)) if y == v and pos[x] < k)
+        code += [(k, t, v, 0, seq[t]) for t in back]
+    return code
+
+
+def test_canonical_check_on_large_patterns():
+    """Patterns of 33-64 vertices and up to 128 edges (two-word edge masks
+    in the check's embeddings) against the oracle: minimal codes of chorded
+    paths, the same codes with two tuples' edge labels swapped, and random
+    traversals of random graphs (rarely minimal)."""
+    import sys
+    rng = random.Random(5)
+    old = sys.getrecursionlimit()
+    sys.setrecursionlimit(10000)
+    n_min = 0
+    try:
+        for i in range(60):
+            n = rng.randint(33, 64)
+            c = _chorded_path_code(rng, n, rng.randint(0, 128 - (n - 1)))
+            assert gm.DFSCode(c).is_min() and po.is_min(po.normalize(c)), c
+            n_min += 1
+            d = _chorded_path_code(rng, n, rng.randint(0, 128 - (n - 1)), reverse=True)
+            assert not gm.DFSCode(d).is_min() and not po.is_min(po.normalize(d)), d
+        for i in range(60):
+            c = _rand_code(rng, n_range=(33, 64), n_labels=(2 if i % 2 else 60), extra=(10, 60))
+            assert gm.DFSCode(c).is_min() == po.is_min(po.normalize(c)), c
+    finally:
+        sys.setrecursionlimit(old)
+    # past the limits: a clean error
+    path = [(i, i + 1, 0, 0, 0) for i in range(64)]  # 65 vertices
+    with pytest.raises(gm.GMError) as e:
+        gm.DFSCode(path).is_min()
+    assert e.value.code == gm.GM_E_UNSUPPORTED
 
 
 def test_canonical_check_known_answers():
