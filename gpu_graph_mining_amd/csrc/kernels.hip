@@ -800,8 +800,10 @@ template <bool ROOT, int TW>
 __global__ void __launch_bounds__(64) k_write(StepArgs a) {
   constexpr uint32_t cap = 1u << kWriteTableBits;
   __shared__ uint32_t ckey[cap];
+  // per child: its rows and list entries in the chunk while counting, then
+  // (cleared at the reservation) the rows and entries written so far
   __shared__ uint32_t ccnt[cap], cbs[cap];
-  __shared__ uint32_t crun[cap], crunl[cap];
+  uint32_t *const crun = ccnt, *const crunl = cbs;
   __shared__ unsigned long long cbase[cap], clbase[cap];
   __shared__ int s_pre[64];
   __shared__ unsigned long long s_lo[64];
@@ -858,8 +860,6 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
       ckey[i] = kEmpty32;
       ccnt[i] = 0;
       cbs[i] = 0;
-      crun[i] = 0;
-      crunl[i] = 0;
     }
     if (lane == 0) {
       s_fill = 0;
@@ -930,6 +930,8 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
             atomicAdd(&a.cursor[at], ((unsigned long long)ccnt[i] << 32) | (unsigned long long)cbs[i]);
         cbase[i] = a.cur_rows[at] + (v >> 32);
         clbase[i] = a.cur_list[at] + (v & 0xFFFFFFFFull);
+        crun[i] = 0;  // the counts' storage becomes the write runs
+        crunl[i] = 0;
       }
     wave_sync();
     // write, candidates 64 at a time in list order
@@ -1007,19 +1009,24 @@ __global__ void __launch_bounds__(64) k_write(StepArgs a) {
           // list regions: any disjoint order within the child's reserved range
           unsigned long long loff = 0;
           if (ch >= 0) loff = clbase[h] + (unsigned long long)atomicAdd(&crunl[h], ent[u].y);
-          // row positions in list order: one round per distinct child of
-          // the window
-          unsigned long long pos = 0;
-          while (active) {
-            const int leader = __ffsll((long long)active) - 1;
-            const int32_t cl = __shfl(ch, leader, 64);
-            const uint64_t m = __ballot(ch == cl);
-            if (ch == cl) pos = cbase[h] + (unsigned long long)crun[h] + (unsigned long long)__popcll(m & lanes_below(lane));
-            wave_sync();
-            if (lane == leader) crun[h] += (uint32_t)__popcll(m);
-            wave_sync();
-            active &= ~m;
+          // row positions in list order: the lanes of the window with the
+          // same child (the same table slot h) found with one ballot per
+          // slot bit; each takes its rank among them, and the lowest
+          // advances the child's run
+          const bool sel = ch >= 0;
+          uint64_t peers = active;
+#pragma unroll
+          for (int bit = 0; bit < kWriteTableBits; ++bit) {
+            const bool hb = (h >> bit) & 1u;
+            const uint64_t m = __ballot(sel && hb);
+            peers &= hb ? m : ~m;
           }
+          const uint32_t rank = (uint32_t)__popcll(peers & lanes_below(lane));
+          const uint32_t run = sel ? crun[h] : 0u;
+          wave_sync();
+          if (sel && rank == 0) crun[h] = run + (uint32_t)__popcll(peers);
+          wave_sync();
+          const unsigned long long pos = sel ? cbase[h] + (unsigned long long)run + rank : 0ull;
           // then the rows, every selected lane at once
           if (ch >= 0 && (int64_t)pos < a.ccap) {
             a.cpid[pos] = ch;
