@@ -263,8 +263,11 @@ struct Tuple {
 
 // ----------------------------------------------------------- LDS tables
 // k_level count table (per wave, cap = 2^bits entries): key (pattern -
-// p_lo) * K + extension key, support, embeddings, and the mask of the
-// block's segments (by head lane) that counted the key already.
+// p_lo) * K + extension key, support, embeddings, the candidates' item sum,
+// and the mask of the block's segments (by head lane) that hold the key.
+// The support is settled at each block's end from the mask (a candidate
+// only sets its segment's bit); bit 31 of sup then marks a mask whose bit 0
+// belongs to a segment carried into the next block (counted already).
 struct CountTable {
   uint32_t *key, *sup, *emb, *bsum;
   unsigned long long *mask;
@@ -305,9 +308,7 @@ __device__ __forceinline__ void count_insert(const CountTable &t, uint32_t gk, i
       }
     }
     if (cur == gk) {
-      const unsigned long long bit = 1ull << head;
-      const unsigned long long old = atomicOr(&t.mask[h], bit);
-      if (!(old & bit)) atomicAdd(&t.sup[h], 1u);
+      atomicOr(&t.mask[h], 1ull << head);  // results unused: the no-return forms
       atomicAdd(&t.emb[h], 1u);
       atomicAdd(&t.bsum[h], bound);
       return;
@@ -359,7 +360,8 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
   const int NEL = a.NEL, NVL = a.NVL;
   const int fwd_base = a.S * NEL;
   const int ms = a.mask_stride;
-  unsigned long long n_cand = 0, n_pruned = 0, bsum = 0;
+  uint32_t n_cand = 0, n_pruned = 0;  // per lane: at most its rows' items (< 2^32)
+  unsigned long long bsum = 0;
   int err = 0;
   for (int64_t b = start; b < end;) {
     uint64_t starts;
@@ -544,7 +546,18 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
     if (live) a.rccnt[r] = (uint16_t)cnt;
     wave_sync();
     // the block's segments are done (but a carried one, at head lane 0)
-    for (uint32_t i = lane; i < tab.cap; i += 64) tab.mask[i] = carry ? (tab.mask[i] & 1ull) : 0ull;
+    // support: one per segment of the block that holds the key, less a
+    // segment carried in from the previous block; a carried-out segment
+    // keeps its bit (at head lane 0 of the next block)
+    for (uint32_t i = lane; i < tab.cap; i += 64) {
+      const unsigned long long m = tab.mask[i];
+      if (m) {
+        const uint32_t sv = tab.sup[i];
+        const uint32_t keep = carry ? (uint32_t)(m & 1ull) : 0u;
+        tab.sup[i] = ((sv & 0x7FFFFFFFu) + (uint32_t)__popcll(m) - (sv >> 31)) | (keep << 31);
+        tab.mask[i] = keep;
+      }
+    }
     wave_sync();
     b += nb;
   }
@@ -562,9 +575,10 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       const bool used = k != kEmpty32;
       const uint64_t m = __ballot(used);
       if (used) {
-        atomicAdd(&a.acc[roff + (int64_t)k], ((unsigned long long)tab.sup[i] << kSupShift) | (unsigned long long)tab.emb[i]);
-        atomicAdd(&a.accb[roff + (int64_t)k], (unsigned long long)tab.bsum[i]);
-        rec[nrec + (uint32_t)__popcll(m & lanes_below(lane))] = make_uint4(k, tab.emb[i], tab.bsum[i], 0u);
+        const uint32_t emb = tab.emb[i], bs = tab.bsum[i];
+        atomicAdd(&a.acc[roff + (int64_t)k], ((unsigned long long)tab.sup[i] << kSupShift) | (unsigned long long)emb);
+        atomicAdd(&a.accb[roff + (int64_t)k], (unsigned long long)bs);
+        rec[nrec + (uint32_t)__popcll(m & lanes_below(lane))] = make_uint4(k, emb, bs, 0u);
       }
       nrec += (uint32_t)__popcll(m);
     }
@@ -575,13 +589,13 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
     info->flags = kChunkFailed;
   }
   if (__ballot(err != 0) && lane == 0) atomicOr(a.error_flag, 1);
-  n_cand = wave_sum(n_cand);
-  n_pruned = wave_sum(n_pruned);
+  const unsigned long long w_cand = wave_sum((unsigned long long)n_cand);
+  const unsigned long long w_pruned = wave_sum((unsigned long long)n_pruned);
   bsum = wave_sum(bsum);
   if (lane == 0) {
     unsigned long long *line = a.spread + (ci & (kSpread - 1)) * kSpreadStride;
-    if (n_cand) atomicAdd(&line[kSpreadCand], n_cand);
-    if (n_pruned) atomicAdd(&line[kSpreadPruned], n_pruned);
+    if (w_cand) atomicAdd(&line[kSpreadCand], w_cand);
+    if (w_pruned) atomicAdd(&line[kSpreadPruned], w_pruned);
     if (bsum) atomicAdd(&line[kSpreadBound], bsum);
     atomicMax(&line[kSpreadFill], (unsigned long long)s_scal[0]);
     if (failed) atomicAdd(&line[kSpreadFailed], 1ull);
@@ -701,47 +715,128 @@ __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
 
 // ----------------------------------------------------------------- REDUCE
 // Sum every key's replicas; keys reaching their job's min support go to the
-// frequent list as (key, support, embeddings, item sum) and keep their
-// replicas (k_cursors splits the children's rows by them, then clears
-// them); the other keys' replicas are cleared here.
+// frequent list as (key, support, embeddings, item sum), in key order, and
+// keep their replicas (k_cursors splits the children's rows by them, then
+// clears them); the other keys' replicas are cleared here.  Three kernels:
+// k_reduce counts the frequent keys of each tile of kReduceTile keys,
+// k_reduce_scan turns the counts into the tiles' first slots (and the
+// list's length), k_reduce_list writes each tile's records in key order.
+constexpr int kReduceThreads = 256;
+constexpr int kReduceSteps = kReduceTile / kReduceThreads;
+static_assert(kReduceSteps * kReduceThreads == kReduceTile, "reduce tile");
+
 template <bool ROOT>
-__global__ void __launch_bounds__(256) k_reduce(StepArgs a) {
-  const int lane = threadIdx.x & 63;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nkeys; i += (int64_t)gridDim.x * blockDim.x) {
-    unsigned long long v = 0, bs = 0;
-    for (int r = 0; r < a.rep; ++r) {
-      v += a.acc[(int64_t)r * a.nkeys + i];
-      bs += a.accb[(int64_t)r * a.nkeys + i];
+__device__ __forceinline__ bool reduce_frequent(const StepArgs &a, int64_t i, int32_t s) {
+  if (s <= 0) return false;
+  const int64_t p = i / a.K + a.p_base;
+  const int32_t owner = ROOT ? (int32_t)p : meta_at(a.meta, a.S, p)->job;
+  return s >= a.minsup_job[owner];
+}
+
+// sum of x over the workgroup (kReduceThreads lanes), read by every lane
+__device__ __forceinline__ uint32_t block_sum(uint32_t x, uint32_t *s_part) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  x = wave_sum(x);
+  if (lane == 0) s_part[wave] = x;
+  __syncthreads();
+  uint32_t t = 0;
+#pragma unroll
+  for (int w = 0; w < kReduceThreads / 64; ++w) t += s_part[w];
+  __syncthreads();
+  return t;
+}
+
+template <bool ROOT>
+__global__ void __launch_bounds__(kReduceThreads) k_reduce(StepArgs a) {
+  __shared__ uint32_t s_part[kReduceThreads / 64];
+  const int64_t n_tiles = reduce_tiles(a.nkeys);
+  for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    uint32_t hits = 0;
+    for (int j = 0; j < kReduceSteps; ++j) {
+      const int64_t i = t * kReduceTile + (int64_t)j * kReduceThreads + threadIdx.x;
+      if (i >= a.nkeys) continue;
+      unsigned long long v = 0;
+      for (int r = 0; r < a.rep; ++r) v += a.acc[(int64_t)r * a.nkeys + i];
+      const int32_t s = (int32_t)(v >> kSupShift);
+      a.sup[i] = s;
+      const bool hit = reduce_frequent<ROOT>(a, i, s);
+      hits += hit ? 1u : 0u;
+      if (v && !hit)
+        for (int r = 0; r < a.rep; ++r) {
+          a.acc[(int64_t)r * a.nkeys + i] = 0;
+          a.accb[(int64_t)r * a.nkeys + i] = 0;
+        }
     }
-    const int32_t s = (int32_t)(v >> kSupShift);
-    const unsigned long long c = v & ((1ull << kSupShift) - 1);
-    a.sup[i] = s;
-    bool hit = false;
-    if (s > 0) {
-      const int64_t p = i / a.K + a.p_base;
-      const int32_t owner = ROOT ? (int32_t)p : meta_at(a.meta, a.S, p)->job;
-      hit = s >= a.minsup_job[owner];
+    hits = block_sum(hits, s_part);
+    if (threadIdx.x == 0) a.tile_count[t] = hits;
+  }
+}
+
+// One workgroup: exclusive scan of the tile counts in place; the total is
+// the list's length.
+__global__ void __launch_bounds__(1024) k_reduce_scan(StepArgs a) {
+  __shared__ uint32_t s_wave[16];
+  const int64_t n = reduce_tiles(a.nkeys);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (int64_t b = 0; b < n; b += 1024) {
+    const int64_t i = b + threadIdx.x;
+    const uint32_t x = i < n ? a.tile_count[i] : 0u;
+    uint32_t wt;
+    const uint32_t ex = wave_exclusive_scan<uint32_t>(x, wt);
+    if (lane == 0) s_wave[wave] = wt;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      before += w < wave ? s_wave[w] : 0u;
+      total += s_wave[w];
     }
-    if (v && !hit)
-      for (int r = 0; r < a.rep; ++r) {
-        a.acc[(int64_t)r * a.nkeys + i] = 0;
-        a.accb[(int64_t)r * a.nkeys + i] = 0;
+    if (i < n) a.tile_count[i] = carry + before + ex;
+    carry += total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *a.freq_count = carry;
+}
+
+template <bool ROOT>
+__global__ void __launch_bounds__(kReduceThreads) k_reduce_list(StepArgs a) {
+  __shared__ uint32_t s_part[kReduceThreads / 64];
+  const int64_t n_tiles = reduce_tiles(a.nkeys);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    uint32_t next = a.tile_count[t];  // the tile's first slot
+    for (int j = 0; j < kReduceSteps; ++j) {
+      const int64_t i = t * kReduceTile + (int64_t)j * kReduceThreads + threadIdx.x;
+      const int32_t s = i < a.nkeys ? a.sup[i] : 0;
+      const bool hit = i < a.nkeys && reduce_frequent<ROOT>(a, i, s);
+      // rank among the tile's keys of this step, in key order
+      const uint64_t m = __ballot(hit);
+      if (lane == 0) s_part[wave] = (uint32_t)__popcll(m);
+      __syncthreads();
+      uint32_t before = 0, step = 0;
+#pragma unroll
+      for (int w = 0; w < kReduceThreads / 64; ++w) {
+        before += w < wave ? s_part[w] : 0u;
+        step += s_part[w];
       }
-    // one atomic per wave for its frequent keys
-    const unsigned long long mask = __ballot(hit);
-    uint32_t first = 0;
-    const int leader = mask ? __ffsll((long long)mask) - 1 : 0;
-    if (mask && lane == leader) first = atomicAdd(a.freq_count, (uint32_t)__popcll(mask));
-    first = __shfl(first, leader, 64);
-    if (hit) {
-      const uint32_t slot = first + (uint32_t)__popcll(mask & lanes_below(lane));
-      if (slot < a.freq_cap) {
-        unsigned long long *f = a.freq + 4 * (int64_t)slot;
-        f[0] = (unsigned long long)i;
-        f[1] = (unsigned long long)s;
-        f[2] = c;
-        f[3] = bs;
+      __syncthreads();
+      if (hit) {
+        const uint32_t slot = next + before + (uint32_t)__popcll(m & lanes_below(lane));
+        if (slot < a.freq_cap) {
+          unsigned long long v = 0, bs = 0;
+          for (int r = 0; r < a.rep; ++r) {
+            v += a.acc[(int64_t)r * a.nkeys + i];
+            bs += a.accb[(int64_t)r * a.nkeys + i];
+          }
+          unsigned long long *f = a.freq + 4 * (int64_t)slot;
+          f[0] = (unsigned long long)i;
+          f[1] = (unsigned long long)s;
+          f[2] = v & ((1ull << kSupShift) - 1);
+          f[3] = bs;
+        }
       }
+      next += step;
     }
   }
 }
@@ -1164,11 +1259,16 @@ hipError_t launch_level_fix(const StepArgs &a, bool root, hipStream_t s) {
 
 hipError_t launch_reduce(const StepArgs &a, bool root, hipStream_t s) {
   if (a.nkeys <= 0) return hipSuccess;
-  const unsigned g = (unsigned)std::min<int64_t>((a.nkeys + 255) / 256, 16384);
+  const unsigned g = (unsigned)std::min<int64_t>(reduce_tiles(a.nkeys), 8192);
   if (root)
-    hipLaunchKernelGGL(k_reduce<true>, dim3(g), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_reduce<true>, dim3(g), dim3(kReduceThreads), 0, s, a);
   else
-    hipLaunchKernelGGL(k_reduce<false>, dim3(g), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_reduce<false>, dim3(g), dim3(kReduceThreads), 0, s, a);
+  hipLaunchKernelGGL(k_reduce_scan, dim3(1), dim3(1024), 0, s, a);
+  if (root)
+    hipLaunchKernelGGL(k_reduce_list<true>, dim3(g), dim3(kReduceThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_reduce_list<false>, dim3(g), dim3(kReduceThreads), 0, s, a);
   return hipGetLastError();
 }
 

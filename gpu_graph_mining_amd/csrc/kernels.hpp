@@ -101,9 +101,10 @@ struct StepArgs {
   int32_t *sup;
   int64_t nkeys;
   const int32_t *minsup_job;
-  unsigned long long *freq;  // (key, support, embeddings, item sum) of frequent keys
+  unsigned long long *freq;  // (key, support, embeddings, item sum) of frequent keys, in key order
   uint32_t *freq_count;
   uint32_t freq_cap;
+  uint32_t *tile_count;  // k_reduce: frequent keys per tile of kReduceTile keys, then the tiles' first slots
   // k_write: child rows; cur_rows / cur_list[c * rep + r] are the first row
   // and the first list entry of child c for the chunks of replica r, and
   // cursor[c * rep + r] = rows << 32 | list entries reserved so far
@@ -125,6 +126,13 @@ hipError_t launch_level(const StepArgs &a, bool root, hipStream_t s);
 // Recount the chunks k_level listed as failed (persistent grid; cheap when
 // none failed).
 hipError_t launch_level_fix(const StepArgs &a, bool root, hipStream_t s);
+// The level's supports and its frequent-key list in key order: a pass that
+// counts the frequent keys of each tile of kReduceTile keys (and clears the
+// other keys' replicas), a scan of the tile counts, and a pass that writes
+// each tile's records from its first slot on.  tile_count holds
+// reduce_tiles(nkeys) entries.
+constexpr int kReduceTile = 256;
+__host__ __device__ inline int64_t reduce_tiles(int64_t nkeys) { return (nkeys + kReduceTile - 1) / kReduceTile; }
 hipError_t launch_reduce(const StepArgs &a, bool root, hipStream_t s);
 // Children of every chunk in chunks[0, n_chunks) (the whole level).
 hipError_t launch_write(const StepArgs &a, bool root, hipStream_t s);

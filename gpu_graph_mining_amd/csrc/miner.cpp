@@ -452,42 +452,6 @@ struct Freq {
   unsigned long long key, sup, proj, items;
 };
 
-// The frequent keys in ascending key order (the order of the level's
-// children, so that pattern ids do not depend on the device's append
-// order).  Keys are unique and below key_end; LSD radix sort over the bits
-// in use, 8-bit digits for short lists, 11-bit digits for long ones.
-void sort_freq(const Freq *in, int64_t n, uint64_t key_end, Arr<Freq> &out, Arr<Freq> &tmp) {
-  out.resize(n);
-  if (n < 2) {
-    if (n) out[0] = in[0];
-    return;
-  }
-  int bits = 1;
-  while (bits < 64 && (key_end >> bits)) ++bits;
-  const int db = n < 4096 ? 8 : 11, nb = 1 << db;
-  const int passes = (bits + db - 1) / db;
-  tmp.resize(n);
-  Freq *bufs[2] = {out.data(), tmp.data()};
-  int dst = (passes & 1) ? 0 : 1;  // the last pass writes `out`
-  const Freq *src = in;
-  std::vector<uint32_t> cnt(nb);  // on the heap: an 8 KB frame here slowed the level's host passes 2x
-  for (int ps = 0; ps < passes; ++ps) {
-    const int sh = db * ps;
-    std::fill(cnt.begin(), cnt.end(), 0u);
-    for (int64_t i = 0; i < n; ++i) cnt[(src[i].key >> sh) & (nb - 1)]++;
-    uint32_t run = 0;
-    for (int d = 0; d < nb; ++d) {
-      const uint32_t x = cnt[d];
-      cnt[d] = run;
-      run += x;
-    }
-    Freq *d = bufs[dst];
-    for (int64_t i = 0; i < n; ++i) d[cnt[(src[i].key >> sh) & (nb - 1)]++] = src[i];
-    src = d;
-    dst ^= 1;
-  }
-}
-
 struct Timer {
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double sec() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
@@ -695,7 +659,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
   if (wall_khz <= 0) wall_khz = 100000;
 
   Level N;                // patterns of the next level
-  Arr<Freq> frall, fr, frtmp;
+  Arr<Freq> frall, fr;
   Arr<int64_t> kidx;      // per frequent key: child index, -1 if dropped
   Arr<DFS> ktup;          // per frequent key: the tuple it adds
   Arr<uint8_t> knv;       // per frequent key: the child's vertices
@@ -847,6 +811,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     while (!w.arena && rep < 64 && (int64_t)rep * 2 * nkeys * 16 <= (64ll << 20) && (int64_t)rep * 2 * 32 <= total_chunks)
       rep *= 2;
     grow(w.sup, sizeof(int32_t) * std::max<int64_t>(nkeys, 1), s);
+    grow(w.tiles, sizeof(uint32_t) * std::max<int64_t>(reduce_tiles(nkeys), 1), s);
     grow_zero(w.acc, sizeof(unsigned long long) * std::max<int64_t>(nkeys * rep, 1), s);
     grow_zero(w.accb, sizeof(unsigned long long) * std::max<int64_t>(nkeys * rep, 1), s);
     grow(w.chunks, sizeof(ChunkInfo) * std::max<int64_t>(total_chunks, 1), s);
@@ -907,6 +872,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.freq = reinterpret_cast<unsigned long long *>(dctr + 1);
       a.freq_cap = fcap;
       a.freq_count = &dctr->fcount;
+      a.tile_count = w.tiles.as<uint32_t>();
       a.pid = root ? nullptr : w.row(cur).pid.as<int32_t>();
       a.rgraph = root ? nullptr : w.row(cur).rgraph.as<int32_t>();
       a.vt = root ? nullptr : w.row(cur).vt.as<uint8_t>();
@@ -1024,7 +990,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         std::memcpy(frall.data() + guess, h2, rest);
         src = frall.data();
       }
-      sort_freq(src, F, (uint64_t)nkeys, fr, frtmp);
+      // in key order already (k_reduce_list)
+      fr.resize(F);
+      std::memcpy(fr.data(), src, sizeof(Freq) * F);
+      for (int64_t i = 1; i < F; ++i)
+        if (fr[i].key <= fr[i - 1].key) throw Error(GM_E_DEVICE, "internal: frequent keys out of order");
     }
     dev_sec += tdev.sec();
 
@@ -1481,7 +1451,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
         hip_check(launch_copy8(du, su, up_bytes, s), "upload children");
       us_overlap = t_over.sec() * 1e6;
       dev_sec += tdev2.sec();
-      size_t live = db.bytes + w.sup.cap + w.acc.cap + w.accb.cap + w.freq.cap + w.rccnt.cap + w.cand.cap +
+      size_t live = db.bytes + w.sup.cap + w.tiles.cap + w.acc.cap + w.accb.cap + w.freq.cap + w.rccnt.cap + w.cand.cap +
                     w.chunks.cap + w.fail.cap + w.cur_rows.cap + w.cur_list.cap + w.cursor.cap + w.chunk_keys.cap;
       for (const RowSet &rs : w.rows) live += rs.bytes();
       peak = std::max(peak, live);

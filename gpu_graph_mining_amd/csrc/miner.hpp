@@ -87,7 +87,7 @@ struct RowSet {
 // Device buffers of the level steps, kept across mining calls on one
 // database so that repeated calls allocate nothing once warm.
 struct Workspace {
-  DevBuf minsup, sup, acc, accb, freq, rccnt, cur_rows, cur_list, cursor, fail, chunks, chunk_keys, cand;
+  DevBuf minsup, sup, acc, accb, freq, tiles, rccnt, cur_rows, cur_list, cursor, fail, chunks, chunk_keys, cand;
   // row sets: a level's rows stay in one while its children are written to
   // another; two alternate, more are held while levels are mined in pieces
   // under a memory budget
@@ -136,7 +136,7 @@ struct Workspace {
     }
   }
   std::vector<DevBuf *> scratch_bufs() {
-    return {&sup, &acc, &accb, &freq, &rccnt, &cur_rows, &cur_list, &cursor, &fail, &chunks, &chunk_keys, &cand};
+    return {&sup, &acc, &accb, &freq, &tiles, &rccnt, &cur_rows, &cur_list, &cursor, &fail, &chunks, &chunk_keys, &cand};
   }
   std::vector<DevBuf *> row_bufs(int i) { return row(i).bufs(); }
   ~Workspace() {
