@@ -841,20 +841,21 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce_list(StepArgs a) {
   }
 }
 
-// After the host chose the children: frequent key i (fkeys[i]) becomes child
-// fchild[i] (>= 0; -1: dropped), its sup entry points at the child, and
+// After the host chose the children: frequent key i (the key of record i of
+// the frequent list) becomes child fchild[i] (>= 0; -1: dropped), its sup
+// entry points at the child, and
 // every (child, replica) gets its first row and list entry -- the child's
 // bases plus the earlier replicas' embeddings and item sums -- and a zero
 // reservation cursor; the frequent keys' replicas are cleared.  Also copies
 // the children's upload.
 __global__ void k_cursors(StepArgs a, unsigned long long *dst, const unsigned long long *src, int64_t n,
-                          const unsigned long long *fkeys, const int32_t *fchild, const unsigned long long *row_base,
-                          const unsigned long long *list_base, int64_t F) {
+                          const int32_t *fchild, const unsigned long long *row_base, const unsigned long long *list_base,
+                          int64_t F) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t i = t; i < n; i += stride) dst[i] = src[i];
   for (int64_t i = t; i < F; i += stride) {
-    const int64_t key = (int64_t)fkeys[i];
+    const int64_t key = (int64_t)a.freq[4 * i];
     const int32_t ch = fchild[i];
     unsigned long long re = ch >= 0 ? row_base[ch] : 0ull, rl = ch >= 0 ? list_base[ch] : 0ull;
     if (ch >= 0) a.sup[key] = -(ch + 1);
@@ -1305,15 +1306,15 @@ hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_cursors(const StepArgs &a, void *dst, const void *src, size_t bytes,
-                          const unsigned long long *fkeys, const int32_t *fchild, const unsigned long long *row_base,
-                          const unsigned long long *list_base, int64_t F, hipStream_t s) {
+hipError_t launch_cursors(const StepArgs &a, void *dst, const void *src, size_t bytes, const int32_t *fchild,
+                          const unsigned long long *row_base, const unsigned long long *list_base, int64_t F,
+                          hipStream_t s) {
   if (bytes % 8) return hipErrorInvalidValue;
   const int64_t n = (int64_t)(bytes / 8);
   if (n == 0 && F == 0) return hipSuccess;
   hipLaunchKernelGGL(k_cursors, dim3(copy_grid(std::max(n, F))), dim3(256), 0, s, a,
-                     static_cast<unsigned long long *>(dst), static_cast<const unsigned long long *>(src), n, fkeys,
-                     fchild, row_base, list_base, F);
+                     static_cast<unsigned long long *>(dst), static_cast<const unsigned long long *>(src), n, fchild,
+                     row_base, list_base, F);
   return hipGetLastError();
 }
 

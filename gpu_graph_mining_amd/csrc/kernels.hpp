@@ -140,12 +140,13 @@ hipError_t launch_write(const StepArgs &a, bool root, hipStream_t s);
 // host memory with a kernel on stream s.
 hipError_t launch_copy8(void *dst, const void *src, size_t bytes, hipStream_t s);
 // Copy the children's upload (bytes, a multiple of 8) and, for the F
-// frequent keys fkeys[i]: sup[key] = -(fchild[i] + 1) for the chosen ones
-// (fchild >= 0), their (child, replica) cursors from row_base / list_base
-// [child] and the replicas' counts, and clear the keys' replicas.
-hipError_t launch_cursors(const StepArgs &a, void *dst, const void *src, size_t bytes,
-                          const unsigned long long *fkeys, const int32_t *fchild, const unsigned long long *row_base,
-                          const unsigned long long *list_base, int64_t F, hipStream_t s);
+// frequent keys (the keys of the frequent list a.freq, in its order):
+// sup[key] = -(fchild[i] + 1) for the chosen ones (fchild >= 0), their
+// (child, replica) cursors from row_base / list_base[child] and the
+// replicas' counts, and clear the keys' replicas.
+hipError_t launch_cursors(const StepArgs &a, void *dst, const void *src, size_t bytes, const int32_t *fchild,
+                          const unsigned long long *row_base, const unsigned long long *list_base, int64_t F,
+                          hipStream_t s);
 // Gather selected graphs of a resident packed database into a new one
 // (k_gather_graphs): per selected graph t, its source graph, first vertex
 // and first adjacency entry in the output, and its job.

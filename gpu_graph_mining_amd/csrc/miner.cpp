@@ -1104,8 +1104,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       knv[i] = (uint8_t)std::min(cnv, 255);
       knode[i] = 0;
     };
-    // a child has at most one vertex more than its parent
-    const int max_nv_bound = root ? 2 : L.max_nv + 1;
     // the canonical check of child i (its parent's code plus its tuple)
     auto is_min_child = [&](int64_t i) {
       const int64_t p = parent_of(fr[i].key);
@@ -1123,17 +1121,30 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // adaptive: speculate while the last level's non-minimal children held
     // at most 30 % of its rows (each costs a written, unused row; molecule-
     // like databases, rich in symmetric patterns, hold about half)
-    const bool want_spec = spec_wanted;
-    // short lists are decoded while the device writes (a serial pass on the
-    // critical path otherwise); long ones in parallel before the write
-    const bool early_decode = F >= 4096 || max_nv_bound > kMaxPatV;
-    if (early_decode) Pool::get().run(F, threads, decode, 4096);
+    const bool want_spec = spec_wanted && !canon_dev && !root && !opt.queries && !past_max;
+    // A speculative level decodes its keys while the device writes; the
+    // children's tuple stride needs only their largest vertex count: the
+    // parent's plus one for a forward key, from the keys in order (the
+    // parent's key range advances with them, no division).  Otherwise the
+    // keys are decoded now, in parallel when there are many.
     int max_nv_exact = 0;
-    if (early_decode)
-      for (int64_t i = 0; i < F; ++i) max_nv_exact = std::max(max_nv_exact, (int)knv[i]);
-    const bool spec = want_spec && !canon_dev && !root && !opt.queries && !past_max &&
-                      (early_decode ? max_nv_exact : max_nv_bound) <= kMaxPatV;
-    const bool late_decode = spec && !early_decode;
+    if (want_spec) {
+      int64_t p = p_begin;
+      uint64_t p_end_key = (uint64_t)K;  // keys of pattern p are below this
+      const uint64_t bwd_keys = (uint64_t)S * NEL;
+      for (int64_t i = 0; i < F; ++i) {
+        const uint64_t key = fr[i].key;
+        while (key >= p_end_key) {
+          ++p;
+          p_end_key += (uint64_t)K;
+        }
+        const uint64_t x = key - (p_end_key - (uint64_t)K);
+        max_nv_exact = std::max(max_nv_exact, (int)L.nv[p] + (x >= bwd_keys ? 1 : 0));
+      }
+    }
+    const bool spec = want_spec && max_nv_exact <= kMaxPatV;
+    const bool late_decode = spec;
+    if (!spec) Pool::get().run(F, threads, decode, 4096);
     // the device's answer for frequent key i (written by k_canon_level as
     // answer + 1 in bits 32-39 of the support word); undecided keys are
     // checked here
@@ -1149,7 +1160,6 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       return code - 1 == kCanonMin;
     };
     if (!spec) {
-      if (!early_decode) Pool::get().run(F, threads, decode, 4096);
       std::atomic<bool> too_big(false);
       Pool::get().run(F, threads, [&](int64_t i) {
         const uint32_t sup = (uint32_t)fr[i].sup;
@@ -1191,7 +1201,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       child_rows += (int64_t)fr[i].proj;
       if (!late_decode) max_nv = std::max(max_nv, (int)knv[i]);
     }
-    if (late_decode) max_nv = max_nv_bound;
+    if (late_decode) max_nv = max_nv_exact;
     const int CS = C > 0 ? tuple_stride(max_nv) : 4;  // the children's tuple stride
     const bool wide = CS > kNarrowPatV;                 // and metadata layout
     N.resize(C, new_len);
@@ -1202,13 +1212,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     size_t child_mask_off = 0, child_code_off = 0, child_vlab_off = 0;
     uint64_t child_items = 0;  // the children's list entries
     // Upload (mapped staging, read by k_cursors and copied by it): the
-    // children's metadata, then per frequent key its key and child id (-1:
+    // children's metadata, then per frequent key its child id (-1:
     // dropped), per child its first row and first list entry, then the
     // children's path masks.  Child c's rows follow the earlier children's
     // (embedding counts), its list entries likewise (item sums).
     const size_t meta_all = (size_t)meta_bytes(CS) * C;
-    const size_t fk_off = (meta_all + 7) & ~size_t(7);
-    const size_t fc_off = fk_off + sizeof(unsigned long long) * F;
+    const size_t fc_off = (meta_all + 7) & ~size_t(7);
     const size_t rb_off = (fc_off + sizeof(int32_t) * F + 7) & ~size_t(7);
     const size_t lb_off = rb_off + sizeof(unsigned long long) * C;
     // path masks of the children with >= 2 edges: a forward and a
@@ -1226,14 +1235,12 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     const size_t up_bytes = (cvl_off + (want_codes ? (size_t)CS * C : 0) + 7) & ~size_t(7);  // copied in 8-byte words
     char *hu = F > 0 ? static_cast<char *>(w.up.ensure(up_bytes)) : nullptr;
     if (F > 0) {
-      unsigned long long *hfk = reinterpret_cast<unsigned long long *>(hu + fk_off);
       int32_t *hfc = reinterpret_cast<int32_t *>(hu + fc_off);
       unsigned long long *hrb = reinterpret_cast<unsigned long long *>(hu + rb_off);
       unsigned long long *hlb = reinterpret_cast<unsigned long long *>(hu + lb_off);
       int64_t run = 0;
       for (int64_t i = 0; i < F; ++i) {
         const int64_t c = kidx[i];
-        hfk[i] = fr[i].key;
         hfc[i] = (int32_t)c;
         if (c < 0) continue;
         hrb[c] = (unsigned long long)run;
@@ -1247,11 +1254,11 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       }
       N.row0[C] = run;
     }
-    if (F > 0) stage_up(w, w.up, fk_off, masks_off - fk_off, s);  // keys, child ids, bases
+    if (F > 0) stage_up(w, w.up, fc_off, masks_off - fc_off, s);  // child ids, bases
     if (F > 0 && C == 0) {  // nothing to write: the frequent keys' replicas are cleared
       const char *su = static_cast<const char *>(w.up.dev);
-      hip_check(launch_cursors(a, nullptr, nullptr, 0, reinterpret_cast<const unsigned long long *>(su + fk_off),
-                               reinterpret_cast<const int32_t *>(su + fc_off), nullptr, nullptr, F, s),
+      hip_check(launch_cursors(a, nullptr, nullptr, 0, reinterpret_cast<const int32_t *>(su + fc_off), nullptr, nullptr,
+                               F, s),
                 "k_cursors");
     }
     if (C > 0) {
@@ -1298,8 +1305,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       a.n_chunks = total_chunks;
       // child ids and cursors, then the write pass (launched back to back:
       // the device does not wait on the host between)
-      hip_check(launch_cursors(a, du, su, 0, reinterpret_cast<const unsigned long long *>(su + fk_off),
-                               reinterpret_cast<const int32_t *>(su + fc_off),
+      hip_check(launch_cursors(a, du, su, 0, reinterpret_cast<const int32_t *>(su + fc_off),
                                reinterpret_cast<const unsigned long long *>(su + rb_off),
                                reinterpret_cast<const unsigned long long *>(su + lb_off), F, s),
                 "k_cursors");
