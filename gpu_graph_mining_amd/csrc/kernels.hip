@@ -604,11 +604,14 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
 }
 
 // ------------------------------------------------------------- LEVEL FIX
-// Chunks whose count table filled up, recounted from their lists by one
-// wave each (persistent grid over the failure list): (segment, key) pairs
-// are deduplicated in a 2^kFixTableBits-entry LDS table, one device atomic
-// per distinct pair.  A pass that fills the table is split in two by the
-// residue of the key (every key lands in exactly one committed pass).
+// Chunks whose count table filled up, recounted from their lists (persistent
+// grid over the failure list): each chunk in parts of about kFixPartRows
+// rows cut at segment starts, one wave per part (a segment's pairs stay in
+// one part); (segment, key) pairs are deduplicated in a 2^kFixTableBits-
+// entry LDS table, one device atomic per distinct pair.  A pass that fills
+// the table is split in two by the residue of the key (every key lands in
+// exactly one committed pass).
+constexpr int kFixPartRows = 64;
 template <bool ROOT>
 __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
   constexpr uint32_t cap = 1u << kFixTableBits, limit = cap - cap / 4;
@@ -618,9 +621,23 @@ __global__ void __launch_bounds__(64) k_level_fix(StepArgs a) {
   __shared__ int s_fill, s_over;
   const int lane = threadIdx.x;
   const uint32_t nfail = *a.fail_count;
-  for (uint32_t fi = blockIdx.x; fi < nfail; fi += gridDim.x) {
+  const int64_t parts = (a.chunk_rows + kFixPartRows - 1) / kFixPartRows;
+  const int64_t units = (int64_t)nfail * parts;
+  for (int64_t unit = blockIdx.x; unit < units; unit += gridDim.x) {
+    const uint32_t fi = (uint32_t)(unit / parts);
+    const int64_t part = unit % parts;
     const uint32_t gci = a.fail_list[fi];
-    const ChunkInfo c = a.chunks[gci];
+    ChunkInfo c = a.chunks[gci];
+    // the part's rows: nominal cuts moved to segment starts (the last part
+    // takes the rest of the chunk, which may run past its nominal rows)
+    {
+      const int64_t lo = part == 0 ? c.start : min(c.end, seg_at_or_after<ROOT>(a, c.start + part * kFixPartRows));
+      const int64_t hi =
+          part == parts - 1 ? c.end : min(c.end, seg_at_or_after<ROOT>(a, c.start + (part + 1) * kFixPartRows));
+      c.start = lo;
+      c.end = hi;
+    }
+    if (c.start >= c.end) continue;
     const int64_t roff = (int64_t)(gci & (uint32_t)(a.rep - 1)) * a.nkeys;
     int top = 1;
     if (lane == 0) {

@@ -235,3 +235,28 @@ def test_gpu_callback_exception_propagates(golden):
     with pytest.raises(KeyError):
         gm.mine(db, case["minsup"], on_pattern=sink)
     assert len(seen) == 3
+
+
+def test_gpu_recount_of_full_count_tables():
+    """Stars of 150 leaves with distinct labels: a center row lists 150
+    distinct keys, more than a 128-entry count table holds, so root chunks
+    and the 2-edge level's chunks overflow and are recounted (k_level_fix
+    with key-residue split passes; k_write counts their children itself).
+    Patterns, supports, |Proj|, E_child and E_cand equal the oracle's.
+    (Recounts of chunks longer than one part run on config2's full
+    workload, whose listing is checked against the reference's.)"""
+    vb, vl, eb, ef, et, el = [0], [], [0], [], [], []
+    for g in range(40):
+        leaves = 150 - (g % 3)
+        vl += [0] + list(range(1, leaves + 1))
+        vb.append(len(vl))
+        ef += [0] * leaves
+        et += list(range(1, leaves + 1))
+        el += [v % 2 for v in range(leaves)]
+        eb.append(len(ef))
+    db = gm.GraphDB.from_arrays(vb, vl, eb, ef, et, el)
+    pats, st = gm.mine(db, 30, collect_proj=True, max_edges=2, table_bits=7)
+    opats, ost = po.mine_arrays(db.arrays(), 30, max_edges=2)
+    assert st["split_passes"] > 0  # chunks were recounted
+    assert _by_code(pats) == _by_code(opats)
+    assert st["e_child"] == ost["e_child"] and st["e_cand"] == ost["e_cand"]
