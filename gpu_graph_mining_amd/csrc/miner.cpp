@@ -1452,9 +1452,17 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       if (w.arena) {
         hip_check(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, s), "upload children");
         hip_check(hipStreamSynchronize(s), "upload children");
+      } else {
+        // a DMA copy from the pinned staging buffer on the side stream, so
+        // that it overlaps the write pass; the next level's launches wait
+        // for it (and the host rewrites the staging buffer only after the
+        // next level's synchronisation, which covers it)
+        if (!w.side) hip_check(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking), "hipStreamCreate");
+        if (!w.up_done) hip_check(hipEventCreateWithFlags(&w.up_done, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, w.side), "upload children");
+        hip_check(hipEventRecord(w.up_done, w.side), "upload children");
+        hip_check(hipStreamWaitEvent(s, w.up_done, 0), "upload children");
       }
-      else
-        hip_check(launch_copy8(du, su, up_bytes, s), "upload children");
       us_overlap = t_over.sec() * 1e6;
       dev_sec += tdev2.sec();
       size_t live = db.bytes + w.sup.cap + w.tiles.cap + w.acc.cap + w.accb.cap + w.freq.cap + w.rccnt.cap + w.cand.cap +

@@ -98,6 +98,10 @@ struct Workspace {
   }
   HostBuf up, down;
   hipEvent_t ev[2] = {nullptr, nullptr};  // stage timing (MineOptions::stage_events)
+  // the children's upload runs on a copy engine beside the write pass:
+  // its stream (default priority) and the event the next level waits on
+  hipStream_t side = nullptr;
+  hipEvent_t up_done = nullptr;
   // the packed counters hold zeros outside a level step (k_reduce clears
   // what it reads); false after a call that ended early
   bool acc_zero = false;
@@ -142,6 +146,8 @@ struct Workspace {
   ~Workspace() {
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
+    if (up_done) (void)hipEventDestroy(up_done);
+    if (side) (void)hipStreamDestroy(side);
   }
 };
 
