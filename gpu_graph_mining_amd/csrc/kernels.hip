@@ -359,6 +359,7 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
 
   const int NEL = a.NEL, NVL = a.NVL;
   const int fwd_base = a.S * NEL;
+  const int nel_nvl = NEL * NVL;
   const int ms = a.mask_stride;
   uint32_t n_cand = 0, n_pruned = 0;  // per lane: at most its rows' items (< 2^32)
   unsigned long long bsum = 0;
@@ -429,7 +430,7 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       VMask bwd_mask = hd.z;
       Tuple<TW> t;
       int32_t vb = 0, ca = 0, ce = 0, na = 0, ne = 0;
-      uint32_t nrme = 0, rme = 0;
+      uint32_t nrme = 0;  // the next source's rightmost-path edge
       VMask bwd_from = 0;
       const Meta *pm = metas + p;
       const unsigned long long *mkp = ms ? a.pmask + (int64_t)p * 2 * ms : nullptr;
@@ -458,6 +459,10 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       const int maxtoc = nv - 1;
       int s = 0;      // current source: 0 the rightmost vertex, q >= 1 rm[q - 1].from
       int pre_s = 0;  // items before the current source
+      // the current source's forward keys start at src_kbase (its vertex's
+      // key block); the order rule against its rightmost-path edge (label
+      // src_e1, to-label src_l1) applies from the second source on
+      int src_kbase = fwd_base + maxtoc * nel_nvl, src_e1 = -1, src_l1 = 0;
       int32_t cb = ca;  // first item of the current source
       auto eval = [&](uint32_t e) {
         const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
@@ -466,33 +471,26 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
         int key = -1, wout = 0;
         uint32_t bound = 0;
         bool keep = true;
-        if (s == 0) {
-          if (fw < 0) {
-            if (nl >= minlabel) {
-              key = fwd_base + (maxtoc * NEL + el) * NVL + nl;
-              wout = w;
-              bound = (uint32_t)n + dw;
-              keep = (mk >> (el * NVL + nl)) & 1ull;
-            }
-          } else if ((bwd_from >> fw) & 1) {
-            // backward to a rightmost-path vertex f = rm[q].from, q >= 1
-            const int q = pm->rmq[fw];
-            const uint32_t rq = pm->rm[q];
-            const int e1 = (rq >> 8) & 0xFF, l1 = (rq >> 16) & 0xFF;
-            if (!((bwd_mask >> fw) & 1) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
-              key = fw * NEL + el;
-              wout = -1;
-              bound = (uint32_t)n;
-              if (mkp) keep = (mkp[ms + 1 + q] >> el) & 1ull;
-            }
-          }
-        } else {
-          const int f = rme & 0xFF, e1 = (rme >> 8) & 0xFF, l1 = (rme >> 16) & 0xFF;
-          if (fw < 0 && nl >= minlabel && (e1 < el || (e1 == el && l1 <= nl))) {
-            key = fwd_base + (f * NEL + el) * NVL + nl;
+        const int lab = el * NVL + nl;  // the edge and new-vertex labels' place in a key and a mask
+        if (fw < 0) {
+          // forward to a new vertex w, from the current source (the source's
+          // order rule; none from the rightmost vertex, src_e1 = -1)
+          if (nl >= minlabel && (src_e1 < el || (src_e1 == el && src_l1 <= nl))) {
+            key = src_kbase + lab;
             wout = w;
             bound = (uint32_t)(n - pre_s) + dw;  // sources s - 1 .. root, plus w
-            keep = (mk >> (el * NVL + nl)) & 1ull;
+            keep = (mk >> lab) & 1ull;
+          }
+        } else if (s == 0 && ((bwd_from >> fw) & 1)) {
+          // backward to a rightmost-path vertex f = rm[q].from, q >= 1
+          const int q = pm->rmq[fw];
+          const uint32_t rq = pm->rm[q];
+          const int e1 = (rq >> 8) & 0xFF, l1 = (rq >> 16) & 0xFF;
+          if (!((bwd_mask >> fw) & 1) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
+            key = fw * NEL + el;
+            wout = -1;
+            bound = (uint32_t)n;
+            if (mkp) keep = (mkp[ms + 1 + q] >> el) & 1ull;
           }
         }
         if (key >= 0) {
@@ -529,7 +527,9 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
             } else {
               ca = cb = na;
               ce = ne;
-              rme = nrme;
+              src_kbase = fwd_base + (int)(nrme & 0xFF) * nel_nvl;
+              src_e1 = (int)((nrme >> 8) & 0xFF);
+              src_l1 = (int)((nrme >> 16) & 0xFF);
               if (mkp) mk = mkp[s];
               if (s < nrm) {
                 nrme = pm->rm[s];
