@@ -325,6 +325,7 @@ __device__ __forceinline__ void count_insert(const CountTable &t, uint32_t gk, i
 // rightmost-path source's (the next source's adjacency range prefetched) --
 // applies the extension rules, appends each candidate to the row's list
 // region and counts it in the wave's table.
+constexpr int kStepItems = 4;  // items of one source a lane takes per step
 template <bool ROOT, int TW>
 __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(StepArgs a) {  // waves per SIMD the registers allow
   // 16-byte aligned whatever static LDS precedes it: the table's segment
@@ -361,8 +362,8 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
   const int fwd_base = a.S * NEL;
   const int nel_nvl = NEL * NVL;
   const int ms = a.mask_stride;
-  uint32_t n_cand = 0, n_pruned = 0;  // per lane: at most its rows' items (< 2^32)
-  unsigned long long bsum = 0;
+  uint32_t n_pruned = 0;        // per lane: at most its rows' items (< 2^32)
+  unsigned long long listed = 0;  // candidates listed by the wave (uniform): with the pruned, the enumerated
   int err = 0;
   for (int64_t b = start; b < end;) {
     uint64_t starts;
@@ -374,7 +375,7 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
     const uint32_t rb = live ? row_items<ROOT>(a, r) : 0u;
     const unsigned long long lo = live ? row_list<ROOT>(a, r) : 0ull;
     int cnt = 0;  // candidates listed for this row
-    // Each step a lane takes up to 4 items of its current source, their
+    // Each step a lane takes up to kStepItems items of its current source, their
     // adjacency entries loaded together, then evaluates them in order.
     if (ROOT) {
       int32_t ca = 0, ce = 0, lu = 0, job = 0, n = 0;
@@ -394,19 +395,17 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
           const uint32_t bound = (uint32_t)n + (e >> 24);
           a.cand[lo + cnt] = make_uint2(cand_code(true, key, w), bound);
           ++cnt;
-          ++n_cand;
-          bsum += bound;
           count_insert(tab, kb + (uint32_t)key, head, bound);
         }
       };
       while (__ballot(ca < ce)) {
         if (ca < ce) {
-          const int c4 = min(4, ce - ca);
-          uint32_t e[4];
+          const int c4 = min(kStepItems, ce - ca);
+          uint32_t e[kStepItems];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) e[u] = u < c4 ? a.adj[ca + u] : 0u;
+          for (int u = 0; u < kStepItems; ++u) e[u] = u < c4 ? a.adj[ca + u] : 0u;
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+          for (int u = 0; u < kStepItems; ++u)
             if (u < c4) eval(e[u]);
           ca += c4;
         }
@@ -432,37 +431,41 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       int32_t vb = 0, ca = 0, ce = 0, na = 0, ne = 0;
       uint32_t nrme = 0;  // the next source's rightmost-path edge
       VMask bwd_from = 0;
-      const Meta *pm = metas + p;
-      const unsigned long long *mkp = ms ? a.pmask + (int64_t)p * 2 * ms : nullptr;
+      // the pattern's metadata and path masks, addressed from p where they
+      // are read (two 64-bit pointers held through the item loop cost the
+      // registers of a seventh wave per SIMD)
+      auto pat = [&]() -> const Meta & { return metas[p]; };
+      auto pmask = [&](int i) { return a.pmask[(int64_t)p * 2 * ms + i]; };
       unsigned long long mk = ~0ull;
       const int n = (int)rb;
       if (work) {
         t.load(a.vt, r, a.S, nv);
         vb = a.gvbase[g];
-        bwd_from = pm->bwd_from;
+        bwd_from = pat().bwd_from;
         if constexpr (kWide) {
-          bwd_mask |= (VMask)pm->bwd_mask_hi << 32;
-          bwd_from |= (VMask)pm->bwd_from_hi << 32;
+          bwd_mask |= (VMask)pat().bwd_mask_hi << 32;
+          bwd_from |= (VMask)pat().bwd_from_hi << 32;
         }
         const int rv = t.get(nv - 1);
         ca = a.vadj[vb + rv];
         ce = a.vadj[vb + rv + 1];
         if (nrm > 0) {
-          nrme = pm->rm[0];
+          nrme = pat().rm[0];
           const int u = t.get(nrme & 0xFF);
           na = a.vadj[vb + u];
           ne = a.vadj[vb + u + 1];
         }
-        if (mkp) mk = mkp[0];
+        if (ms) mk = pmask(0);
       }
       const uint32_t kb = (uint32_t)((int64_t)(p - a.p_lo) * a.K);
       const int maxtoc = nv - 1;
       int s = 0;      // current source: 0 the rightmost vertex, q >= 1 rm[q - 1].from
       int pre_s = 0;  // items before the current source
       // the current source's forward keys start at src_kbase (its vertex's
-      // key block); the order rule against its rightmost-path edge (label
-      // src_e1, to-label src_l1) applies from the second source on
-      int src_kbase = fwd_base + maxtoc * nel_nvl, src_e1 = -1, src_l1 = 0;
+      // key block); from the second source on, the order rule against its
+      // rightmost-path edge rme (label bits 8-15, to-label bits 16-23)
+      int src_kbase = fwd_base + maxtoc * nel_nvl;
+      uint32_t rme = 0;
       int32_t cb = ca;  // first item of the current source
       auto eval = [&](uint32_t e) {
         const int w = e & 0xFF, el = (e >> 8) & 0xFF, nl = (e >> 16) & 0xFF;
@@ -474,8 +477,9 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
         const int lab = el * NVL + nl;  // the edge and new-vertex labels' place in a key and a mask
         if (fw < 0) {
           // forward to a new vertex w, from the current source (the source's
-          // order rule; none from the rightmost vertex, src_e1 = -1)
-          if (nl >= minlabel && (src_e1 < el || (src_e1 == el && src_l1 <= nl))) {
+          // order rule; none from the rightmost vertex)
+          const int e1 = (int)((rme >> 8) & 0xFF), l1 = (int)((rme >> 16) & 0xFF);
+          if (nl >= minlabel && (s == 0 || e1 < el || (e1 == el && l1 <= nl))) {
             key = src_kbase + lab;
             wout = w;
             bound = (uint32_t)(n - pre_s) + dw;  // sources s - 1 .. root, plus w
@@ -483,24 +487,22 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
           }
         } else if (s == 0 && ((bwd_from >> fw) & 1)) {
           // backward to a rightmost-path vertex f = rm[q].from, q >= 1
-          const int q = pm->rmq[fw];
-          const uint32_t rq = pm->rm[q];
+          const int q = pat().rmq[fw];
+          const uint32_t rq = pat().rm[q];
           const int e1 = (rq >> 8) & 0xFF, l1 = (rq >> 16) & 0xFF;
           if (!((bwd_mask >> fw) & 1) && (e1 < el || (e1 == el && l1 <= maxlabel))) {
             key = fw * NEL + el;
             wout = -1;
             bound = (uint32_t)n;
-            if (mkp) keep = (mkp[ms + 1 + q] >> el) & 1ull;
+            if (ms) keep = (pmask(ms + 1 + q) >> el) & 1ull;
           }
         }
         if (key >= 0) {
-          ++n_cand;
           if (!keep) {
             ++n_pruned;  // counted as a candidate, never listed
           } else if (cnt < n) {
             a.cand[lo + cnt] = make_uint2(cand_code(false, key, wout), bound);
             ++cnt;
-            bsum += bound;
             count_insert(tab, kb + (uint32_t)key, head, bound);
           } else {
             err = 1;  // more candidates than items: the row's item count is wrong
@@ -510,12 +512,12 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       bool more = work && ca < ce;
       while (__ballot(more)) {
         if (more) {
-          const int c4 = min(4, ce - ca);
-          uint32_t e[4];
+          const int c4 = min(kStepItems, ce - ca);
+          uint32_t e[kStepItems];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) e[u] = u < c4 ? a.adj[ca + u] : 0u;
+          for (int u = 0; u < kStepItems; ++u) e[u] = u < c4 ? a.adj[ca + u] : 0u;
 #pragma unroll
-          for (int u = 0; u < 4; ++u)
+          for (int u = 0; u < kStepItems; ++u)
             if (u < c4) eval(e[u]);
           ca += c4;
           if (ca >= ce) {  // next source (each source has at least one item)
@@ -527,12 +529,11 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
             } else {
               ca = cb = na;
               ce = ne;
-              src_kbase = fwd_base + (int)(nrme & 0xFF) * nel_nvl;
-              src_e1 = (int)((nrme >> 8) & 0xFF);
-              src_l1 = (int)((nrme >> 16) & 0xFF);
-              if (mkp) mk = mkp[s];
+              rme = nrme;
+              src_kbase = fwd_base + (int)(rme & 0xFF) * nel_nvl;
+              if (ms) mk = pmask(s);
               if (s < nrm) {
-                nrme = pm->rm[s];
+                nrme = pat().rm[s];
                 const int u = t.get(nrme & 0xFF);
                 na = a.vadj[vb + u];
                 ne = a.vadj[vb + u + 1];
@@ -544,6 +545,7 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       }
     }
     if (live) a.rccnt[r] = (uint16_t)cnt;
+    listed += (unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane(wave_sum(cnt));
     wave_sync();
     // the block's segments are done (but a carried one, at head lane 0)
     // support: one per segment of the block that holds the key, less a
@@ -589,14 +591,12 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
     info->flags = kChunkFailed;
   }
   if (__ballot(err != 0) && lane == 0) atomicOr(a.error_flag, 1);
-  const unsigned long long w_cand = wave_sum((unsigned long long)n_cand);
+  const unsigned long long w_cand = listed + wave_sum((unsigned long long)n_pruned);
   const unsigned long long w_pruned = wave_sum((unsigned long long)n_pruned);
-  bsum = wave_sum(bsum);
   if (lane == 0) {
     unsigned long long *line = a.spread + (ci & (kSpread - 1)) * kSpreadStride;
     if (w_cand) atomicAdd(&line[kSpreadCand], w_cand);
     if (w_pruned) atomicAdd(&line[kSpreadPruned], w_pruned);
-    if (bsum) atomicAdd(&line[kSpreadBound], bsum);
     atomicMax(&line[kSpreadFill], (unsigned long long)s_scal[0]);
     if (failed) atomicAdd(&line[kSpreadFailed], 1ull);
   }

@@ -25,7 +25,7 @@ constexpr int kSpread = 64;
 constexpr int kSpreadStride = 16;  // 64-bit words per line
 constexpr int kSpreadCand = 0;     // extension candidates enumerated (listed + pruned)
 constexpr int kSpreadFill = 1;     // largest count-table fill of a chunk (atomicMax)
-constexpr int kSpreadBound = 2;    // sum of the listed candidates' child bounds
+// (slot 2 unused)
 constexpr int kSpreadPruned = 3;   // candidates dropped by the 2-edge path rule
 constexpr int kSpreadStart = 4;    // ~(earliest start) of the extension + support stage (atomicMax)
 constexpr int kSpreadEnd = 5;      // latest end of the stage (atomicMax); device wall clock

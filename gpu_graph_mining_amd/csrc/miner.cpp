@@ -832,7 +832,7 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     Counters cnt;
     const char *hbuf = nullptr;
     double ms = 0;
-    unsigned long long ecand = 0, max_fill = 0, cand_bound = 0, pruned = 0, failed = 0;
+    unsigned long long ecand = 0, max_fill = 0, pruned = 0, failed = 0;
     uint32_t guess = 0;  // frequent keys in the first download
     // the stage; rerun with a longer frequent-key list when it overflowed
     for (int attempt = 0;; ++attempt) {
@@ -938,14 +938,13 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       hip_check(hipStreamSynchronize(s), "k_level");
       std::memcpy(&cnt, hbuf, sizeof(cnt));
       if (cnt.err) throw Error(GM_E_DEVICE, "internal: level step error " + std::to_string(cnt.err));
-      ecand = max_fill = cand_bound = pruned = failed = 0;
+      ecand = max_fill = pruned = failed = 0;
       unsigned long long written = 0, t_start = 0, t_end = 0;
       for (int i = 0; i < kSpread; ++i) {
         const unsigned long long *line = cnt.spread + i * kSpreadStride;
         ecand += line[kSpreadCand];
         pruned += line[kSpreadPruned];
         max_fill = std::max(max_fill, line[kSpreadFill]);
-        cand_bound += line[kSpreadBound];
         failed += line[kSpreadFailed];
         written += line[kSpreadWritten];
         t_start = std::max(t_start, line[kSpreadStart]);
