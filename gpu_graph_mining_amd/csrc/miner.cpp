@@ -616,6 +616,9 @@ void stage_up(Workspace &w, HostBuf &hb, size_t off, size_t bytes, hipStream_t s
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// children's uploads from this size on go by DMA on the workspace's side stream
+constexpr size_t kSideUploadBytes = 1 << 20;
+
 }  // namespace
 
 bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats *stats, hipStream_t s,
@@ -1451,11 +1454,15 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
       if (w.arena) {
         hip_check(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, s), "upload children");
         hip_check(hipStreamSynchronize(s), "upload children");
+      } else if (up_bytes < kSideUploadBytes) {
+        hip_check(launch_copy8(du, su, up_bytes, s), "upload children");
       } else {
         // a DMA copy from the pinned staging buffer on the side stream, so
         // that it overlaps the write pass; the next level's launches wait
         // for it (and the host rewrites the staging buffer only after the
-        // next level's synchronisation, which covers it)
+        // next level's synchronisation, which covers it).  Small uploads
+        // stay on the stream: handles mining side by side (serving slots)
+        // would otherwise double their streams on the few hardware queues.
         if (!w.side) hip_check(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking), "hipStreamCreate");
         if (!w.up_done) hip_check(hipEventCreateWithFlags(&w.up_done, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipMemcpyAsync(du, hu, up_bytes, hipMemcpyHostToDevice, w.side), "upload children");
