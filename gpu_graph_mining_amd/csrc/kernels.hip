@@ -73,7 +73,12 @@ namespace {
 
 constexpr uint32_t kEmpty32 = 0xFFFFFFFFu;
 constexpr int kSupShift = 40;  // packed counters: support << 40 | embeddings
-constexpr int kWriteTableBits = 8;  // k_write: children per chunk in one pass (>= k_level's table)
+// k_write: children per chunk in one pass.  A chunk with more frequent
+// children than that is counted again in passes split by child residue;
+// the small table (1.75 KB with its row and list bases; 4 KB of LDS per
+// wave in all) lets 8 waves share a SIMD, which the write's dependent
+// gathers need more than they need one pass per chunk.
+constexpr int kWriteTableBits = 6;
 constexpr int kFixTableBits = 11;   // k_level_fix: (segment, key) pairs per pass
 
 // ----------------------------------------------------------- wave helpers
@@ -910,7 +915,7 @@ __global__ void k_cursors(StepArgs a, unsigned long long *dst, const unsigned lo
 constexpr int kWriteUnroll = 2;  // candidate windows whose loads are issued together
 
 template <bool ROOT, int TW>
-__global__ void __launch_bounds__(64) k_write(StepArgs a) {
+__global__ void __launch_bounds__(64, (TW <= 8 ? 8 : 6)) k_write(StepArgs a) {
   constexpr uint32_t cap = 1u << kWriteTableBits;
   __shared__ uint32_t ckey[cap];
   // per child: its rows and list entries in the chunk while counting, then
