@@ -79,6 +79,10 @@ constexpr int kSupShift = 40;  // packed counters: support << 40 | embeddings
 // wave in all) lets 8 waves share a SIMD, which the write's dependent
 // gathers need more than they need one pass per chunk.
 constexpr int kWriteTableBits = 6;
+// The root level's chunks span many graphs and every label triple: there a
+// chunk often holds more than 64 frequent children, and a 256-entry table
+// (5 waves per SIMD) beats the residue passes.
+constexpr int kWriteTableBitsRoot = 8;
 constexpr int kFixTableBits = 11;   // k_level_fix: (segment, key) pairs per pass
 
 // ----------------------------------------------------------- wave helpers
@@ -915,8 +919,9 @@ __global__ void k_cursors(StepArgs a, unsigned long long *dst, const unsigned lo
 constexpr int kWriteUnroll = 2;  // candidate windows whose loads are issued together
 
 template <bool ROOT, int TW>
-__global__ void __launch_bounds__(64, (TW <= 8 ? 8 : 6)) k_write(StepArgs a) {
-  constexpr uint32_t cap = 1u << kWriteTableBits;
+__global__ void __launch_bounds__(64, (ROOT ? 5 : TW <= 8 ? 8 : 6)) k_write(StepArgs a) {
+  constexpr int kBits = ROOT ? kWriteTableBitsRoot : kWriteTableBits;
+  constexpr uint32_t cap = 1u << kBits;
   __shared__ uint32_t ckey[cap];
   // per child: its rows and list entries in the chunk while counting, then
   // (cleared at the reservation) the rows and entries written so far
@@ -948,7 +953,7 @@ __global__ void __launch_bounds__(64, (TW <= 8 ? 8 : 6)) k_write(StepArgs a) {
     return v < 0 ? -v - 1 : -1;
   };
   auto lookup = [&](uint32_t ch) -> uint32_t {
-    uint32_t h = hash_slot32(ch, kWriteTableBits);
+    uint32_t h = hash_slot32(ch, kBits);
     for (uint32_t probe = 0; probe < cap; ++probe) {
       if (ckey[h] == ch) return h;
       h = (h + 1) & (cap - 1);
@@ -959,7 +964,7 @@ __global__ void __launch_bounds__(64, (TW <= 8 ? 8 : 6)) k_write(StepArgs a) {
   // fill limit; a chunk's records, at most one per table entry of k_level,
   // may fill it completely)
   auto table_add = [&](uint32_t ch, uint32_t n, uint32_t bs, uint32_t limit) -> bool {
-    uint32_t h = hash_slot32(ch, kWriteTableBits);
+    uint32_t h = hash_slot32(ch, kBits);
     for (uint32_t probe = 0; probe < cap; ++probe) {
       const uint32_t old = atomicCAS(&ckey[h], kEmpty32, ch);
       if (old == kEmpty32 || old == ch) {
@@ -1134,7 +1139,7 @@ __global__ void __launch_bounds__(64, (TW <= 8 ? 8 : 6)) k_write(StepArgs a) {
           const bool sel = ch >= 0;
           uint64_t peers = active;
 #pragma unroll
-          for (int bit = 0; bit < kWriteTableBits; ++bit) {
+          for (int bit = 0; bit < kBits; ++bit) {
             const bool hb = (h >> bit) & 1u;
             const uint64_t m = __ballot(sel && hb);
             peers &= hb ? m : ~m;
