@@ -481,6 +481,9 @@ class Pool {
     auto job = std::make_shared<Job>();
     job->fn = &fn;
     job->n = n;
+    // items per grab: enough that the threads do not queue on the shared
+    // counters (two atomics per grab), small enough to balance
+    job->grain = std::max<int64_t>(32, n / ((int64_t)threads * 16));
     {
       std::lock_guard<std::mutex> lk(m_);
       while ((int)workers_.size() < threads - 1) workers_.emplace_back([this]() { loop(); });
@@ -505,13 +508,13 @@ class Pool {
   }
 
  private:
-  // One run: items are taken 32 at a time; `finished` counts the items
+  // One run: items are taken `grain` at a time; `finished` counts the items
   // done (or given up after an exception), so the caller waits for items,
   // not for threads.  A worker holds the job it woke for, so one that runs
   // late finds every item taken and never calls fn after the run returned.
   struct Job {
     const std::function<void(int64_t)> *fn = nullptr;
-    int64_t n = 0;
+    int64_t n = 0, grain = 32;
     std::atomic<int64_t> next{0}, finished{0};
     std::mutex m, err_m;
     std::condition_variable cv;
@@ -524,9 +527,9 @@ class Pool {
     }
     void work() {
       for (;;) {
-        const int64_t i = next.fetch_add(32);
+        const int64_t i = next.fetch_add(grain);
         if (i >= n) break;
-        const int64_t e = std::min(n, i + 32);
+        const int64_t e = std::min(n, i + grain);
         try {
           for (int64_t j = i; j < e; ++j) (*fn)(j);
         } catch (...) {
@@ -615,6 +618,24 @@ void stage_up(Workspace &w, HostBuf &hb, size_t off, size_t bytes, hipStream_t s
 }
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Division by a run-time constant d with one 64 x 64 -> 128-bit multiply by
+// M = ceil(2^64 / d) (Lemire, Kaser and Kurz, "Faster remainder by direct
+// computation", 2019), exact for every dividend below 2^32; larger ones
+// divide the ordinary way.  The host decodes every frequent key (parent
+// pattern, source vertex, labels) with a handful of these per level.
+struct FastDiv {
+  uint64_t m = 0;
+  uint64_t d = 1;
+  FastDiv() = default;
+  explicit FastDiv(uint64_t d_) : m(d_ > 1 && d_ < (1ull << 32) ? ~0ull / d_ + 1 : 0), d(d_) {}
+  uint64_t div(uint64_t a) const {
+    if (d == 1) return a;
+    if (m == 0 || a >= (1ull << 32)) return a / d;
+    return (uint64_t)(((unsigned __int128)m * a) >> 64);
+  }
+  uint64_t mod(uint64_t a) const { return a - div(a) * d; }
+};
 
 // children's uploads from this size on go by DMA on the workspace's side stream
 constexpr size_t kSideUploadBytes = 1 << 20;
@@ -796,7 +817,8 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     // at most kPieceKeys of them (a level with more is cut into pieces)
     p_end = std::min<int64_t>(p_end, p_begin + std::max<int64_t>(1, kPieceKeys / K));
     const int64_t nkeys = (p_end - p_begin) * K;
-    auto parent_of = [&](uint64_t key) { return (int64_t)(key / (uint64_t)K) + p_begin; };
+    const FastDiv fdK((uint64_t)K), fdNEL((uint64_t)NEL), fdNVL((uint64_t)NVL);
+    auto parent_of = [&](uint64_t key) { return (int64_t)fdK.div(key) + p_begin; };
     const int64_t prow_lo = prow(p_begin), prow_hi = prow(p_end);
     // chunk length: kChunkRows, or shorter so that the piece has about
     // kLevelWaves chunks (a power of two, at least one block)
@@ -1080,25 +1102,25 @@ bool mine(DeviceDB &db, const MineOptions &opt, const ReportFn &report, gm_stats
     knode.resize(F);
     kalive.resize(F);
     auto decode = [&](int64_t i) {
-      const int64_t x = (int64_t)(fr[i].key % K);
+      const uint64_t key = fr[i].key, q = fdK.div(key);
+      const uint64_t x = key - q * (uint64_t)K;  // the key within its pattern's
       DFS t;
       int cnv;
       if (root) {
-        const int fl = (int)(x / (NEL * NVL)), el = (int)((x / NVL) % NEL), tl = (int)(x % NVL);
-        t = DFS{0, 1, fl, el, tl};
+        const uint64_t x1 = fdNVL.div(x), fl = fdNEL.div(x1);
+        t = DFS{0, 1, (int)fl, (int)(x1 - fl * NEL), (int)(x - x1 * NVL)};
         cnv = 2;
       } else {
-        const int64_t p = parent_of(fr[i].key);
+        const int64_t p = (int64_t)q + p_begin;
         const uint8_t *pv = L.vl(p);
         const int nv = L.nv[p], maxtoc = nv - 1;
-        if (x < (int64_t)S * NEL) {
-          const int to = (int)(x / NEL), el = (int)(x % NEL);
-          t = DFS{maxtoc, to, pv[maxtoc], el, pv[to]};
+        if (x < (uint64_t)S * NEL) {
+          const uint64_t to = fdNEL.div(x);
+          t = DFS{maxtoc, (int)to, pv[maxtoc], (int)(x - to * NEL), pv[to]};
           cnv = nv;
         } else {
-          const int64_t y = x - (int64_t)S * NEL;
-          const int nl = (int)(y % NVL), el = (int)((y / NVL) % NEL), f = (int)(y / NVL / NEL);
-          t = DFS{f, nv, pv[f], el, nl};
+          const uint64_t y = x - (uint64_t)S * NEL, y1 = fdNVL.div(y), f = fdNEL.div(y1);
+          t = DFS{(int)f, nv, pv[f], (int)(y1 - f * NEL), (int)(y - y1 * NVL)};
           cnv = nv + 1;
         }
       }
