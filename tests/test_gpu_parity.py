@@ -260,3 +260,15 @@ def test_gpu_recount_of_full_count_tables():
     assert st["split_passes"] > 0  # chunks were recounted
     assert _by_code(pats) == _by_code(opats)
     assert st["e_child"] == ost["e_child"] and st["e_cand"] == ost["e_cand"]
+
+
+def test_gpu_candidate_counts_closed_form():
+    """The device enumerates the hand-counted candidates: none for an edge
+    a-b with labels 0 and 1; 18 for a triangle with equal labels (12 at
+    the root's children, 6 at the path's), with E_child 12 and E_root 6."""
+    db = gm.GraphDB.from_arrays([0, 2], [0, 1], [0, 1], [0], [1], [0])
+    pats, st = gm.mine(db, 1)
+    assert len(pats) == 1 and (st["e_cand"], st["e_child"], st["e_root"]) == (0, 0, 1)
+    db = gm.GraphDB.from_arrays([0, 3], [0, 0, 0], [0, 3], [0, 1, 0], [1, 2, 2], [0, 0, 0])
+    pats, st = gm.mine(db, 1)
+    assert len(pats) == 3 and (st["e_cand"], st["e_child"], st["e_root"]) == (18, 12, 6)

@@ -114,3 +114,26 @@ def test_oracle_complete_and_unique_against_brute_force(seed):
             gadjs.append((gl, adj))
         assert n == sum(monomorphisms(labels, edges, gl, adj) for gl, adj in gadjs)
     assert forms == brute_force_frequent(graphs, minsup, 4)
+
+
+def test_oracle_candidate_counts_and_bytes_closed_form():
+    """E_cand, E_child and the algorithmic bytes worked out by hand
+    (B_ext(P) = sum over rows of 8k + sum over rightmost-path vertices v of
+    (8 + 12 deg v), plus 12 E_cand(P); B_sup = 12 E_cand):
+    - an edge a-b, labels 0 and 1: one root row (0 <= 1), k = 2, both
+      vertices of degree 1 on the rightmost path, no extension:
+      B_ext = 16 + 2 * 20 = 56, E_cand = 0;
+    - a triangle with equal labels: 6 directed root rows, each with a
+      forward extension from v1 and one from v0 (12 candidates; rows
+      16 + 2 * 32 = 80 bytes); the path child (6 rows, k = 3, three
+      degree-2 vertices: 24 + 3 * 32 = 120 bytes) has one backward
+      candidate per row (6); the triangle (6 rows, 120 bytes) has none.
+      E_cand = 18, B_ext = 480 + 144 + 720 + 72 + 720 = 2136, B_sup = 216,
+      E_child = 6 + 6."""
+    db = gm.GraphDB.from_arrays([0, 2], [0, 1], [0, 1], [0], [1], [0])
+    _, st = po.mine_arrays(db.arrays(), 1)
+    assert (st["e_cand"], st["b_ext"], st["b_sup"], st["e_child"], st["e_root"]) == (0, 56, 0, 0, 1)
+    db = gm.GraphDB.from_arrays([0, 3], [0, 0, 0], [0, 3], [0, 1, 0], [1, 2, 2], [0, 0, 0])
+    pats, st = po.mine_arrays(db.arrays(), 1)
+    assert len(pats) == 3
+    assert (st["e_cand"], st["b_ext"], st["b_sup"], st["e_child"], st["e_root"]) == (18, 2136, 216, 12, 6)
