@@ -18,7 +18,10 @@
 // Embedding store, one level at a time, pattern-major: the rows of a
 // pattern are contiguous, and within them the rows of one graph (a segment):
 //   pid[r]               pattern of row r
-//   rgraph[r]            graph of row r
+//   rgraph[r]            graph of row r, as the global id of the graph's
+//                        first vertex (unique among graphs that have
+//                        vertices; the row's vertices are found without a
+//                        lookup of the graph's base)
 //   vt[r*S .. r*S+S)     the row's vertex tuple: byte i = local id of pattern
 //                        vertex i (DFS discovery order), S = stride in bytes
 //   rbound[r]            the row's item count (adjacency entries of its

@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       const int n = (int)rb;
       if (work) {
         t.load(a.vt, r, a.S, nv);
-        vb = a.gvbase[g];
+        vb = g;  // the row's graph is stored as its first vertex
         bwd_from = pat().bwd_from;
         if constexpr (kWide) {
           bwd_mask |= (VMask)pat().bwd_mask_hi << 32;
@@ -1068,9 +1068,10 @@ __global__ void __launch_bounds__(64, (ROOT ? 5 : TW <= 8 ? 8 : 6)) k_write(Step
       if (live) {
         n = a.rccnt[r];
         if (ROOT) {
-          g = a.vgraph[r];
-          p = a.gjob[g];
-          nvp = (int)(r - a.gvbase[g]);  // the child tuple starts with the vertex
+          const int32_t gi = a.vgraph[r];
+          p = a.gjob[gi];
+          g = a.gvbase[gi];  // the children's graph, as its first vertex
+          nvp = (int)(r - g);  // the child tuple starts with the vertex
         } else {
           p = a.pid[r];
           g = a.rgraph[r];
