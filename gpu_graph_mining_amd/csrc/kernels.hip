@@ -912,10 +912,11 @@ __global__ void k_cursors(StepArgs a, unsigned long long *dst, const unsigned lo
 // child, its list offset likewise with the item counts.  The child's rows
 // of one chunk are thus consecutive and in the parent's row order, so every
 // child's rows stay graph-contiguous.
-// A chunk that k_level_fix recounted has no records: it is counted here, a
-// pass over the children c with c mod `mod` == res at a time (lane per row),
-// and a pass whose table fills up is split in two by child residue before
-// anything is reserved.
+// A chunk that k_level_fix recounted has no records, and a chunk whose
+// records name more children than the table holds cannot use them: either
+// is counted here, a pass over the children c with c mod `mod` == res at a
+// time (lane per row), and a pass whose table fills up is split in two by
+// child residue before anything is reserved.
 constexpr int kWriteUnroll = 2;  // candidate windows whose loads are issued together
 
 template <bool ROOT, int TW>
