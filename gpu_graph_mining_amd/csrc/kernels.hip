@@ -234,9 +234,9 @@ template <int TW>
 struct Tuple {
   uint32_t w[TW > 0 ? TW : 1];
   int nv;
-  __device__ __forceinline__ void load(const uint8_t *vt, int64_t r, int S, int nv_) {
+  // the words only (nv set apart): they depend on the row alone
+  __device__ __forceinline__ void load(const uint8_t *vt, int64_t r, int S) {
     const uint32_t *p = reinterpret_cast<const uint32_t *>(vt + r * S);
-    nv = nv_;
 #pragma unroll
     for (int q = 0; q < TW; ++q) w[q] = p[q];
   }
@@ -330,11 +330,11 @@ __device__ __forceinline__ void count_insert(const CountTable &t, uint32_t gk, i
 // ------------------------------------------------------------------ LEVEL
 // One wave per chunk.  Per block of <= 64 segment-aligned rows, lane t owns
 // row b + t: it loads the row (pattern, graph, tuple), then walks its items
-// one per iteration -- the rightmost vertex's adjacency, then each
+// a few per step -- the rightmost vertex's adjacency, then each
 // rightmost-path source's (the next source's adjacency range prefetched) --
 // applies the extension rules, appends each candidate to the row's list
 // region and counts it in the wave's table.
-constexpr int kStepItems = 4;  // items of one source a lane takes per step
+constexpr int kStepItems = 6;  // items of one source a lane takes per step
 template <bool ROOT, int TW>
 __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(StepArgs a) {  // waves per SIMD the registers allow
   // 16-byte aligned whatever static LDS precedes it: the table's segment
@@ -427,19 +427,27 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       const Meta *metas = reinterpret_cast<const Meta *>(a.meta);
       int32_t p = 0, g = 0;
       uint4 hd = make_uint4(0, 0, 0, 0);
+      Tuple<TW> t;
+      uint32_t nrme = 0;  // the next source's rightmost-path edge
+      VMask bwd_from = 0;
       if (live) {
+        // the tuple (when its registers allow) and the pattern's first path
+        // edge and backward sources load beside the metadata (none of them
+        // depends on its flags)
         p = a.pid[r];
         g = a.rgraph[r];
         hd = *reinterpret_cast<const uint4 *>(metas + p);
+        if constexpr (TW <= 4) {
+          t.load(a.vt, r, a.S);
+          nrme = metas[p].rm[0];
+          bwd_from = metas[p].bwd_from;
+        }
       }
       const bool work = live && !(hd.w & kPatDead);  // a dead pattern's rows list nothing
       const int nv = (int)(hd.y & 0xFF), nrm = (int)((hd.y >> 8) & 0xFF);
       const int minlabel = (int)((hd.y >> 16) & 0xFF), maxlabel = (int)(hd.y >> 24);
       VMask bwd_mask = hd.z;
-      Tuple<TW> t;
       int32_t vb = 0, ca = 0, ce = 0, na = 0, ne = 0;
-      uint32_t nrme = 0;  // the next source's rightmost-path edge
-      VMask bwd_from = 0;
       // the pattern's metadata and path masks, addressed from p where they
       // are read (two 64-bit pointers held through the item loop cost the
       // registers of a seventh wave per SIMD)
@@ -448,9 +456,12 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       unsigned long long mk = ~0ull;
       const int n = (int)rb;
       if (work) {
-        t.load(a.vt, r, a.S, nv);
+        t.nv = nv;
+        if constexpr (TW > 4) {
+          t.load(a.vt, r, a.S);
+          bwd_from = pat().bwd_from;
+        }
         vb = g;  // the row's graph is stored as its first vertex
-        bwd_from = pat().bwd_from;
         if constexpr (kWide) {
           bwd_mask |= (VMask)pat().bwd_mask_hi << 32;
           bwd_from |= (VMask)pat().bwd_from_hi << 32;
@@ -459,7 +470,7 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
         ca = a.vadj[vb + rv];
         ce = a.vadj[vb + rv + 1];
         if (nrm > 0) {
-          nrme = pat().rm[0];
+          if constexpr (TW > 4) nrme = pat().rm[0];
           const int u = t.get(nrme & 0xFF);
           na = a.vadj[vb + u];
           ne = a.vadj[vb + u + 1];
