@@ -409,14 +409,14 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       };
       while (__ballot(ca < ce)) {
         if (ca < ce) {
-          const int c4 = min(kStepItems, ce - ca);
+          const int nstep = min(kStepItems, ce - ca);
           uint32_t e[kStepItems];
 #pragma unroll
-          for (int u = 0; u < kStepItems; ++u) e[u] = u < c4 ? a.adj[ca + u] : 0u;
+          for (int u = 0; u < kStepItems; ++u) e[u] = u < nstep ? a.adj[ca + u] : 0u;
 #pragma unroll
           for (int u = 0; u < kStepItems; ++u)
-            if (u < c4) eval(e[u]);
-          ca += c4;
+            if (u < nstep) eval(e[u]);
+          ca += nstep;
         }
       }
     } else {
@@ -532,14 +532,14 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       bool more = work && ca < ce;
       while (__ballot(more)) {
         if (more) {
-          const int c4 = min(kStepItems, ce - ca);
+          const int nstep = min(kStepItems, ce - ca);
           uint32_t e[kStepItems];
 #pragma unroll
-          for (int u = 0; u < kStepItems; ++u) e[u] = u < c4 ? a.adj[ca + u] : 0u;
+          for (int u = 0; u < kStepItems; ++u) e[u] = u < nstep ? a.adj[ca + u] : 0u;
 #pragma unroll
           for (int u = 0; u < kStepItems; ++u)
-            if (u < c4) eval(e[u]);
-          ca += c4;
+            if (u < nstep) eval(e[u]);
+          ca += nstep;
           if (ca >= ce) {  // next source (each source has at least one item)
             pre_s += ce - cb;
             ++s;
