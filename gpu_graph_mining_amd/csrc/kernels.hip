@@ -429,6 +429,10 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
       uint4 hd = make_uint4(0, 0, 0, 0);
       Tuple<TW> t;
       uint32_t nrme = 0;  // the next source's rightmost-path edge
+      // one-word tuples have the registers to hold the edge after it too, so
+      // that a source switch need not wait on the pattern's metadata
+      constexpr bool kDeep = TW <= 1;
+      uint32_t nnrme = 0;
       VMask bwd_from = 0;
       if (live) {
         // the tuple (when its registers allow) and the pattern's first path
@@ -440,6 +444,7 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
         if constexpr (TW <= 4) {
           t.load(a.vt, r, a.S);
           nrme = metas[p].rm[0];
+          if constexpr (kDeep) nnrme = metas[p].rm[1];
           bwd_from = metas[p].bwd_from;
         }
       }
@@ -553,10 +558,12 @@ __global__ void __launch_bounds__(64, (TW <= 2 ? 6 : TW <= 8 ? 5 : 4)) k_level(S
               src_kbase = fwd_base + (int)(rme & 0xFF) * nel_nvl;
               if (ms) mk = pmask(s);
               if (s < nrm) {
-                nrme = pat().rm[s];
+                if constexpr (kDeep) nrme = nnrme;
+                else nrme = pat().rm[s];
                 const int u = t.get(nrme & 0xFF);
                 na = a.vadj[vb + u];
                 ne = a.vadj[vb + u + 1];
+                if constexpr (kDeep) nnrme = pat().rm[s + 1];  // s + 1 <= nrm < nv: in the array
               }
               more = ca < ce;
             }
