@@ -941,7 +941,10 @@ template <bool ROOT, int TW>
 __global__ void __launch_bounds__(64, (ROOT ? 5 : TW <= 8 ? 8 : 6)) k_write(StepArgs a) {
   constexpr int kBits = ROOT ? kWriteTableBitsRoot : kWriteTableBits;
   constexpr uint32_t cap = 1u << kBits;
+  // keyed by the candidate's key relative to the chunk's first pattern
+  // (which names one child), with that child's id in cch
   __shared__ uint32_t ckey[cap];
+  __shared__ int32_t cch[cap];
   // per child: its rows and list entries in the chunk while counting, then
   // (cleared at the reservation) the rows and entries written so far
   __shared__ uint32_t ccnt[cap], cbs[cap];
@@ -971,22 +974,26 @@ __global__ void __launch_bounds__(64, (ROOT ? 5 : TW <= 8 ? 8 : 6)) k_write(Step
     const int32_t v = a.sup[pk];
     return v < 0 ? -v - 1 : -1;
   };
-  auto lookup = [&](uint32_t ch) -> uint32_t {
-    uint32_t h = hash_slot32(ch, kBits);
+  // the table slot of key gk, or -1 (not a child of this pass)
+  auto lookup = [&](uint32_t gk) -> int {
+    uint32_t h = hash_slot32(gk, kBits);
     for (uint32_t probe = 0; probe < cap; ++probe) {
-      if (ckey[h] == ch) return h;
+      const uint32_t cur = ckey[h];
+      if (cur == gk) return (int)h;
+      if (cur == kEmpty32) return -1;
       h = (h + 1) & (cap - 1);
     }
-    return 0;  // unreachable: every child of the pass was counted
+    return -1;
   };
   // add n rows of child ch, with items bs, to the table (false: past the
   // fill limit; a chunk's records, at most one per table entry of k_level,
   // may fill it completely)
-  auto table_add = [&](uint32_t ch, uint32_t n, uint32_t bs, uint32_t limit) -> bool {
-    uint32_t h = hash_slot32(ch, kBits);
+  auto table_add = [&](uint32_t gk, int32_t ch, uint32_t n, uint32_t bs, uint32_t limit) -> bool {
+    uint32_t h = hash_slot32(gk, kBits);
     for (uint32_t probe = 0; probe < cap; ++probe) {
-      const uint32_t old = atomicCAS(&ckey[h], kEmpty32, ch);
-      if (old == kEmpty32 || old == ch) {
+      const uint32_t old = atomicCAS(&ckey[h], kEmpty32, gk);
+      if (old == kEmpty32 || old == gk) {
+        if (old == kEmpty32) cch[h] = ch;
         atomicAdd(&ccnt[h], n);
         atomicAdd(&cbs[h], bs);
         if (old == kEmpty32 && (uint32_t)atomicAdd(&s_fill, 1) >= limit) return false;
@@ -1014,7 +1021,7 @@ __global__ void __launch_bounds__(64, (ROOT ? 5 : TW <= 8 ? 8 : 6)) k_write(Step
       for (uint32_t i = lane; i < c.nkeys; i += 64) {
         const uint4 kr = rec[i];
         const int32_t ch = child_of((int64_t)(c.p_lo - a.p_base) * a.K + (int64_t)kr.x);
-        if (ch >= 0 && !table_add((uint32_t)ch, kr.y, kr.z, cap)) atomicOr(&s_over, 1);
+        if (ch >= 0 && !table_add(kr.x, ch, kr.y, kr.z, cap)) atomicOr(&s_over, 1);
       }
     } else {
       // count: lane per row
@@ -1037,7 +1044,8 @@ __global__ void __launch_bounds__(64, (ROOT ? 5 : TW <= 8 ? 8 : 6)) k_write(Step
           const int32_t ch = child_of((int64_t)(p - a.p_base) * a.K + key);
           if (ch < 0 || (mod > 1 && ((uint32_t)ch & (mod - 1)) != res)) continue;
           if (__hip_atomic_load(&s_over, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
-          if (!table_add((uint32_t)ch, 1u, ce.y, cap - cap / 8)) atomicOr(&s_over, 1);
+          const uint32_t gk = (uint32_t)((int64_t)(p - c.p_lo) * a.K + key);
+          if (!table_add(gk, ch, 1u, ce.y, cap - cap / 8)) atomicOr(&s_over, 1);
         }
         b += nb;
       }
@@ -1067,7 +1075,7 @@ __global__ void __launch_bounds__(64, (ROOT ? 5 : TW <= 8 ? 8 : 6)) k_write(Step
     for (uint32_t i = lane; i < cap; i += 64)
       if (ckey[i] != kEmpty32) {
         // one atomic reserves both: rows << 32 | list entries
-        const int64_t at = (int64_t)ckey[i] * a.rep + rsel;
+        const int64_t at = (int64_t)cch[i] * a.rep + rsel;
         const unsigned long long v =
             atomicAdd(&a.cursor[at], ((unsigned long long)ccnt[i] << 32) | (unsigned long long)cbs[i]);
         cbase[i] = a.cur_rows[at] + (v >> 32);
@@ -1113,7 +1121,7 @@ __global__ void __launch_bounds__(64, (ROOT ? 5 : TW <= 8 ? 8 : 6)) k_write(Step
       for (int jb0 = 0; jb0 < T; jb0 += 64 * kWriteUnroll) {
         int kk[kWriteUnroll];
         uint2 ent[kWriteUnroll];
-        int32_t chv[kWriteUnroll];
+        int hv[kWriteUnroll];
 #pragma unroll
         for (int u = 0; u < kWriteUnroll; ++u) {
           const int j = jb0 + u * 64 + lane;
@@ -1132,11 +1140,11 @@ __global__ void __launch_bounds__(64, (ROOT ? 5 : TW <= 8 ? 8 : 6)) k_write(Step
         }
 #pragma unroll
         for (int u = 0; u < kWriteUnroll; ++u) {
-          chv[u] = -1;
+          hv[u] = -1;
           if (jb0 + u * 64 + lane < T) {
             int key, w;
             cand_decode(ROOT, ent[u].x, key, w);
-            chv[u] = child_of((int64_t)(s_p[kk[u]] - a.p_base) * a.K + key);
+            hv[u] = lookup((uint32_t)((int64_t)(s_p[kk[u]] - c.p_lo) * a.K + key));
           }
         }
 #pragma unroll
@@ -1144,9 +1152,8 @@ __global__ void __launch_bounds__(64, (ROOT ? 5 : TW <= 8 ? 8 : 6)) k_write(Step
           const int k = kk[u];
           int key, w;
           cand_decode(ROOT, ent[u].x, key, w);
-          int32_t ch = chv[u];
-          if (ch >= 0 && mod > 1 && ((uint32_t)ch & (mod - 1)) != res) ch = -1;
-          const uint32_t h = ch >= 0 ? lookup((uint32_t)ch) : 0u;
+          const int32_t ch = hv[u] >= 0 ? cch[hv[u]] : -1;  // the pass's children only
+          const uint32_t h = hv[u] >= 0 ? (uint32_t)hv[u] : 0u;
           uint64_t active = __ballot(ch >= 0);
           written += (unsigned long long)__popcll(active);
           // list regions: any disjoint order within the child's reserved range
